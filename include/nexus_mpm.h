@@ -1,0 +1,224 @@
+/*
+ * nexus_mpm.h — C ABI of the MI355X (gfx950) sparse-grid MLS-MPM substep.
+ *
+ * Drop-in for the nexus_mpm pipeline API of dimforge/nexus (non-CPIC path):
+ * particle / grid / model / collider setup on an MpmState-like scene object and a
+ * step() that enqueues one substep (binning sort -> P2G -> grid update with gravity,
+ * velocity clamp and one-way cuboid colliders -> G2P fused with the constitutive
+ * update).  Every entry point below names the reference item it replaces
+ * (file:line in the dimforge/nexus tree).
+ *
+ * Conventions
+ *  - Plain C types only; host arrays are caller-owned and copied in/out; no pointer
+ *    passed in is retained after a call returns.
+ *  - Every function returns an nxm_status (0 = success).  nxm_last_error() gives a
+ *    thread-local message for the last failure on the calling thread.
+ *  - Matrices are 3x3 column-major (9 floats: column 0, 1, 2), the layout of the
+ *    reference's glam Mat3.
+ *  - nxm_pipeline_step / nxm_batch_step enqueue work on the scene's (batch's) HIP
+ *    stream and return; errors detected on the device (grid capacity, domain range)
+ *    are latched and reported by the next synchronizing call (nxm_*_synchronize,
+ *    nxm_scene_read_*).  The reference discards step errors and leaves hash-map
+ *    overflow undefined (src/pipeline.rs:83-85, src_mpm_shaders/grid/grid.rs:455-457).
+ *  - One host thread at a time per scene or batch; a context may be shared by
+ *    threads working on different scenes.
+ */
+#ifndef NEXUS_MPM_H
+#define NEXUS_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int nxm_status;
+#define NXM_OK 0
+#define NXM_ERR_INVALID_ARGUMENT 1
+#define NXM_ERR_OUT_OF_MEMORY 2
+#define NXM_ERR_GRID_CAPACITY 3     /* more active blocks than the grid capacity */
+#define NXM_ERR_DOMAIN 4            /* a block coordinate outside the packable range */
+#define NXM_ERR_TOO_MANY_COLLIDERS 5
+#define NXM_ERR_UNSUPPORTED_MODEL 6
+#define NXM_ERR_DEVICE 7
+#define NXM_ERR_BUSY 8              /* scene is attached to a batch */
+
+/* Model tags of GpuParticleModel (src_mpm_shaders/models/default.rs:20-31). */
+#define NXM_MODEL_ELASTIC_LINEAR 0
+#define NXM_MODEL_ELASTIC_NEO_HOOKEAN 1
+#define NXM_MODEL_SAND_LINEAR 2
+#define NXM_MODEL_SAND_NEO_HOOKEAN 3
+#define NXM_MODEL_FLUID 4
+#define NXM_MODEL_SNOW 5            /* accepted by the reference; rejected here */
+#define NXM_MODEL_WORDS 24
+
+/* BoundaryCondition::ty (src_mpm_shaders/solver/boundary_condition.rs:4-7). */
+#define NXM_BC_STICK 0
+#define NXM_BC_SLIP 1
+#define NXM_BC_SEPARATE 2
+#define NXM_BC_NON_REFLECTING 3
+
+#define NXM_MAX_COLLIDERS 16        /* MAX_COLLISION_BODIES, boundary_condition.rs:44 */
+
+typedef struct nxm_ctx nxm_ctx;
+typedef struct nxm_scene nxm_scene;
+typedef struct nxm_batch nxm_batch;
+
+/* MpmCapacities (src_mpm/pipeline.rs:28-43). */
+typedef struct {
+    uint32_t grid_size;          /* block capacity, rounded up to a power of two */
+    uint32_t particles_capacity; /* particles reserved up front */
+} nxm_capacities;
+
+/* SimulationParams (src_mpm_shaders/solver/params.rs:9-17). */
+typedef struct {
+    float gravity[3];
+    float dt;
+} nxm_sim_params;
+
+/* Particle (src_mpm/solver/particle.rs:26-33) = position + ParticleDynamics
+ * (:65-98) + GpuParticleModel (src_mpm_shaders/models/default.rs:41-44). */
+typedef struct {
+    float position[3];
+    float velocity[3];
+    float def_grad[9];
+    float affine[9];
+    float init_volume;
+    float init_radius;
+    float mass;
+    float damping;
+    float phase;
+    float boundary_friction;
+    uint32_t group_id;
+    uint32_t enabled;
+    uint32_t fixed;
+    uint32_t model_tag;
+    uint32_t model_data[NXM_MODEL_WORDS];
+} nxm_particle;
+
+/* Structure-of-arrays particle upload/readback (same content as nxm_particle,
+ * without per-particle padding).  Any pointer may be NULL on readback to skip it;
+ * on upload NULL means the ParticleDynamics::new default (velocity 0, F = I,
+ * affine 0, damping 0, phase 1, boundary_friction 1, group 0, enabled, not fixed). */
+typedef struct {
+    float *position;     /* n*3 */
+    float *velocity;     /* n*3 */
+    float *def_grad;     /* n*9 */
+    float *affine;       /* n*9 */
+    float *init_volume;  /* n */
+    float *init_radius;  /* n */
+    float *mass;         /* n */
+    float *damping;      /* n */
+    float *phase;        /* n */
+    float *boundary_friction; /* n */
+    uint32_t *group_id;  /* n */
+    uint32_t *enabled;   /* n */
+    uint32_t *fixed;     /* n */
+    uint32_t *model_tag; /* n */
+    uint32_t *model_data;/* n*24 */
+} nxm_particles_soa;
+
+/* A fixed or kinematic cuboid collider coupled one way (rapier collider + body +
+ * BoundaryCondition, src/state.rs:418-450; projection by parry3d Cuboid through
+ * src_rbd_shaders/shapes/shape.rs:378-384).  The body's centre of mass is its
+ * translation. */
+typedef struct {
+    float half_extents[3];
+    float rotation[4];   /* unit quaternion x, y, z, w */
+    float translation[3];
+    float linvel[3];
+    float angvel[3];
+    uint32_t bc_type;
+    float bc_friction;
+} nxm_cuboid;
+
+/* Per-stage device times of the last step, milliseconds (the reference's
+ * GpuTimestamps passes, src_mpm/pipeline.rs:394-511). */
+typedef struct {
+    float sort_ms;         /* "[MPM] Grid sort" */
+    float p2g_ms;          /* "[MPM] P2G" */
+    float grid_update_ms;  /* "[MPM] Grid update" */
+    float g2p_update_ms;   /* "[MPM] G2P" + "[MPM] Particle update" (fused) */
+    float total_ms;
+} nxm_stage_times;
+
+/* Canonical form of the block sort (the part of the reference's sort that does not
+ * depend on atomic ordering): per active block its virtual id, the number of
+ * primary particles, of particles including spilled extras, the ten bucket counts
+ * (grid/sort.rs:41-62), and per particle its primary block and local cell. */
+typedef struct {
+    int32_t *block_vid;        /* nblocks*3 */
+    uint32_t *block_np;        /* nblocks: num_particles */
+    uint32_t *block_npe;       /* nblocks: num_particles_with_extras */
+    uint32_t *block_buckets;   /* nblocks*10 */
+    uint32_t *particle_block;  /* n: index into block_vid of the primary block */
+    uint32_t *particle_cell;   /* n: local cell x + 4y + 16z inside the primary block */
+} nxm_sort_canonical;
+
+/* ---- context (MpmPipeline::new, src_mpm/pipeline.rs:366) ---- */
+nxm_status nxm_ctx_create(int device, nxm_ctx **out);
+void nxm_ctx_destroy(nxm_ctx *ctx);
+const char *nxm_last_error(void);
+const char *nxm_version(void);
+
+/* ---- scene (MpmState, src_mpm/pipeline.rs:65-362) ---- */
+/* MpmState::empty (src_mpm/pipeline.rs:107): gravity 0, dt 1/60, cell width 1. */
+nxm_status nxm_scene_create(nxm_ctx *ctx, const nxm_capacities *caps, nxm_scene **out);
+void nxm_scene_destroy(nxm_scene *scene);
+/* MpmState::set_simulation_params (src_mpm/pipeline.rs:169). */
+nxm_status nxm_scene_set_simulation_params(nxm_scene *scene, const nxm_sim_params *params);
+/* MpmState::write_substep_params (src_mpm/pipeline.rs:183): dt = base dt / max(n, 1). */
+nxm_status nxm_scene_write_substep_params(nxm_scene *scene, uint32_t num_substeps);
+/* MpmState::set_cell_width (src_mpm/pipeline.rs:200). */
+nxm_status nxm_scene_set_cell_width(nxm_scene *scene, float cell_width, uint32_t grid_capacity);
+/* GpuParticles::append (src_mpm/solver/particle.rs:533) / from_particles (:391). */
+nxm_status nxm_scene_add_particles(nxm_scene *scene, const nxm_particle *particles, size_t n);
+nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa *p, size_t n);
+/* MpmState::set_coupling (src_mpm/pipeline.rs:220) restricted to one-way cuboids,
+ * non-CPIC (use_cpic = false). */
+nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
+size_t nxm_scene_len(const nxm_scene *scene);
+/* GpuParticles::read_positions (src_mpm/solver/particle.rs:634-647); synchronizes. */
+nxm_status nxm_scene_read_positions(nxm_scene *scene, float *xyz, size_t n);
+/* Full particle state in original particle order; synchronizes. */
+nxm_status nxm_scene_read_particles_soa(nxm_scene *scene, const nxm_particles_soa *out, size_t n);
+/* Block count of the last step (0 before the first step); synchronizes. */
+nxm_status nxm_scene_sort_size(nxm_scene *scene, size_t *nblocks);
+/* Canonical sort of the last step; arrays sized by nxm_scene_sort_size and len. */
+nxm_status nxm_scene_read_sort_canonical(nxm_scene *scene, const nxm_sort_canonical *out);
+/* Grid of the last step: per block (same order as read_sort_canonical) 64 nodes
+ * (x + 4y + 16z) of {mass, momentum.xyz} after P2G and velocity.xyz after the grid
+ * update and collider boundary.  mass: nblocks*64, momentum/velocity: nblocks*64*3. */
+nxm_status nxm_scene_read_grid(nxm_scene *scene, float *mass, float *momentum, float *velocity);
+nxm_status nxm_scene_synchronize(nxm_scene *scene);
+/* Keep the per-node momentum after P2G for nxm_scene_read_grid (costs one extra
+ * write per node per step).  No counterpart in the reference: a test hook. */
+nxm_status nxm_scene_set_debug_grid(nxm_scene *scene, int enabled);
+
+/* ---- pipeline step (MpmPipeline::step, src_mpm/pipeline.rs:385) ---- */
+/* Enqueues one substep of a standalone scene.  times != NULL synchronizes and
+ * fills per-stage device times. */
+nxm_status nxm_pipeline_step(nxm_ctx *ctx, nxm_scene *scene, nxm_stage_times *times);
+/* Enqueues n substeps (the NexusPipeline::simulate substep loop, src/pipeline.rs:84-86). */
+nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *scene, uint32_t substeps);
+
+/* ---- batching: many independent scenes stepped by one launch per stage ----
+ * No counterpart in the reference (its MPM state is single-scene,
+ * src/state.rs:444-446).  A scene added to a batch is stepped only through the
+ * batch until it is removed. */
+nxm_status nxm_batch_create(nxm_ctx *ctx, nxm_batch **out);
+void nxm_batch_destroy(nxm_batch *batch);
+nxm_status nxm_batch_add_scene(nxm_batch *batch, nxm_scene *scene);
+nxm_status nxm_batch_remove_scene(nxm_batch *batch, nxm_scene *scene);
+nxm_status nxm_batch_step(nxm_batch *batch, nxm_stage_times *times);
+nxm_status nxm_batch_run(nxm_batch *batch, uint32_t substeps);
+nxm_status nxm_batch_synchronize(nxm_batch *batch);
+size_t nxm_batch_len(const nxm_batch *batch);
+/* Use a captured HIP graph for each substep of this batch (default on). */
+nxm_status nxm_batch_set_graph(nxm_batch *batch, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1063 @@
+// mpm_api.hip — host runtime and C ABI (include/nexus_mpm.h).
+//
+// A World owns the device state of a set of scenes stepped together: two particle
+// arenas (read in the previous step's sorted order, written in this step's order),
+// per-particle sort scratch, per-block tables, hash regions, and the scene / material /
+// collider tables.  A standalone scene (the reference's MpmState) is a World holding
+// one scene; a batch is a World holding many.  Scenes enter and leave a World between
+// steps; every step re-packs the live scenes' particles contiguously in slot order.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nexus_mpm.h"
+#include "mpm_kernels.h"
+
+namespace nxm {
+
+static thread_local std::string g_last_error;
+
+static nxm_status fail(nxm_status code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+static nxm_status hip_fail(hipError_t e, const char *what)
+{
+    nxm_status code = (e == hipErrorOutOfMemory) ? NXM_ERR_OUT_OF_MEMORY : NXM_ERR_DEVICE;
+    return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                        \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);    \
+    } while (0)
+
+static uint32_t next_pow2(uint32_t v)
+{
+    uint32_t p = 1;
+    while (p < v && p < (1u << 31)) p <<= 1;
+    return p;
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grows (discarding contents) to at least `want` elements.
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n) return hipSuccess;
+        release();
+        size_t c = std::max<size_t>(want, 1);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), c * sizeof(T));
+        if (e == hipSuccess) n = c;
+        return e;
+    }
+};
+
+}  // namespace nxm
+
+using namespace nxm;
+
+struct nxm_ctx {
+    int device = 0;
+    int num_cus = 256;
+};
+
+namespace nxm {
+
+class World;
+
+// Host copy of a scene.  Particle state lives here while the scene is not resident
+// in a World; static per-particle properties always live here (indexed by id).
+struct HostParticles {
+    size_t n = 0;
+    // state
+    std::vector<float> x, v, F, C, mass, vol0, pstate;
+    std::vector<uint32_t> enabled, fixed;
+    // static
+    std::vector<float> radius, phase, bfric, damping;
+    std::vector<uint32_t> group, tag, words, cls;
+
+    void resize(size_t m)
+    {
+        n = m;
+        x.resize(3 * m); v.resize(3 * m); F.resize(9 * m); C.resize(9 * m);
+        mass.resize(m); vol0.resize(m); pstate.resize(3 * m);
+        enabled.resize(m); fixed.resize(m);
+        radius.resize(m); phase.resize(m); bfric.resize(m); damping.resize(m);
+        group.resize(m); tag.resize(m); words.resize(24 * m); cls.resize(m);
+    }
+};
+
+struct SceneRange {
+    uint32_t start, count;
+};
+
+}  // namespace nxm
+
+struct nxm_scene {
+    nxm_ctx *ctx = nullptr;
+    float base_dt = 1.0f / 60.0f;
+    float dt = 1.0f / 60.0f;
+    float h = 1.0f;
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t grid_cap = 32768;
+    std::vector<ColliderDev> cols;
+    std::vector<ClassDev> classes;
+    std::map<std::array<uint32_t, 26>, uint32_t> class_index;
+    HostParticles hp;
+    bool on_host = true;                // hp state authoritative
+    World *world = nullptr;             // resident world (own or a batch's)
+    uint32_t slot = 0;
+    std::vector<SceneRange> ranges;     // particle ranges in the world's input arena
+    std::unique_ptr<World> own;         // standalone world
+    nxm_batch *batch = nullptr;
+    uint32_t error = 0;                 // latched device error bits
+    bool debug_grid = false;
+};
+
+namespace nxm {
+
+enum SlotState { SLOT_FREE = 0, SLOT_ACTIVE, SLOT_RETIRING };
+
+struct Slot {
+    nxm_scene *scene = nullptr;
+    SlotState state = SLOT_FREE;
+};
+
+class World {
+  public:
+    explicit World(nxm_ctx *ctx) : ctx_(ctx) {}
+    ~World()
+    {
+        if (stream_) {
+            (void)hipStreamSynchronize(stream_);
+            for (auto &e : ev_) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(stream_);
+        }
+    }
+
+    nxm_status init()
+    {
+        HIP_TRY(hipSetDevice(ctx_->device));
+        HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        for (auto &e : ev_) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(g_.reserve(1));
+        HIP_TRY(hipMemsetAsync(g_.p, 0, sizeof(Globals), stream_));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        return NXM_OK;
+    }
+
+    hipStream_t stream() const { return stream_; }
+    uint32_t particles() const { return p_in_; }
+    size_t scenes() const
+    {
+        size_t c = 0;
+        for (auto &s : slots_) c += s.state == SLOT_ACTIVE;
+        return c;
+    }
+
+    nxm_status sync()
+    {
+        HIP_TRY(hipSetDevice(ctx_->device));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        return collect_errors();
+    }
+
+    // Adds a scene whose state is on the host; uploads its particles.
+    nxm_status add(nxm_scene *sc)
+    {
+        uint32_t slot = NONE;
+        for (uint32_t i = 0; i < slots_.size(); ++i)
+            if (slots_[i].state == SLOT_FREE) { slot = i; break; }
+        if (slot == NONE) {
+            if (slots_.size() >= MAX_SCENES) return fail(NXM_ERR_INVALID_ARGUMENT, "too many scenes in one world");
+            slot = (uint32_t)slots_.size();
+            slots_.push_back(Slot{});
+        }
+        slots_[slot].scene = sc;
+        slots_[slot].state = SLOT_ACTIVE;
+        sc->world = this;
+        sc->slot = slot;
+        sc->ranges.clear();
+        dirty_ = true;
+        nxm_status st = upload(sc, 0, sc->hp.n);
+        if (st != NXM_OK) return st;
+        sc->on_host = false;
+        return NXM_OK;
+    }
+
+    // Appends particles [first, first + count) of sc's host arrays to the input arena.
+    nxm_status upload(nxm_scene *sc, size_t first, size_t count)
+    {
+        if (count == 0) return NXM_OK;
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        st = ensure_particles(p_in_ + (uint32_t)count);
+        if (st != NXM_OK) return st;
+        const HostParticles &hp = sc->hp;
+        std::vector<float> plane(count);
+        Arena a = arena(cur_);
+        uint32_t at = p_in_;
+        auto put = [&](int p, const float *src, int stride, int comp) -> hipError_t {
+            for (size_t i = 0; i < count; ++i) plane[i] = src[(first + i) * stride + comp];
+            return hipMemcpy(a.plane(p) + at, plane.data(), count * sizeof(float), hipMemcpyHostToDevice);
+        };
+        for (int c = 0; c < 3; ++c) HIP_TRY(put(PX + c, hp.x.data(), 3, c));
+        for (int c = 0; c < 3; ++c) HIP_TRY(put(PVX + c, hp.v.data(), 3, c));
+        for (int c = 0; c < 9; ++c) HIP_TRY(put(PC0 + c, hp.C.data(), 9, c));
+        for (int c = 0; c < 9; ++c) HIP_TRY(put(PF0 + c, hp.F.data(), 9, c));
+        HIP_TRY(put(PMASS, hp.mass.data(), 1, 0));
+        HIP_TRY(put(PVOL0, hp.vol0.data(), 1, 0));
+        for (int c = 0; c < 3; ++c) HIP_TRY(put(PS0 + c, hp.pstate.data(), 3, c));
+        std::vector<uint32_t> u(count);
+        for (size_t i = 0; i < count; ++i)
+            u[i] = make_flags(hp.enabled[first + i] != 0, hp.fixed[first + i] != 0, hp.cls[first + i], sc->slot);
+        HIP_TRY(hipMemcpy(a.uplane(PFLAGS) + at, u.data(), count * 4, hipMemcpyHostToDevice));
+        for (size_t i = 0; i < count; ++i) u[i] = (uint32_t)(first + i);
+        HIP_TRY(hipMemcpy(a.uplane(PID) + at, u.data(), count * 4, hipMemcpyHostToDevice));
+        sc->ranges.push_back(SceneRange{at, (uint32_t)count});
+        p_in_ += (uint32_t)count;
+        pin_dirty_ = true;
+        return NXM_OK;
+    }
+
+    // Copies the scene's state back to its host arrays (id order) and detaches it.
+    nxm_status remove(nxm_scene *sc)
+    {
+        nxm_status st = pull(sc);
+        if (st != NXM_OK) return st;
+        slots_[sc->slot].state = SLOT_RETIRING;
+        slots_[sc->slot].scene = nullptr;
+        sc->world = nullptr;
+        sc->ranges.clear();
+        sc->on_host = true;
+        dirty_ = true;
+        return NXM_OK;
+    }
+
+    void mark_dirty() { dirty_ = true; }
+
+    // Reads the scene's particle state into its host arrays (id order).
+    nxm_status pull(nxm_scene *sc)
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        HostParticles &hp = sc->hp;
+        Arena a = arena(cur_);
+        std::vector<float> buf;
+        std::vector<uint32_t> ids, fl;
+        for (const SceneRange &r : sc->ranges) {
+            buf.resize(r.count);
+            ids.resize(r.count);
+            fl.resize(r.count);
+            HIP_TRY(hipMemcpy(ids.data(), a.uplane(PID) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(fl.data(), a.uplane(PFLAGS) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < r.count; ++i) {
+                hp.enabled[ids[i]] = (fl[i] & FLAG_ENABLED) ? 1u : 0u;
+                hp.fixed[ids[i]] = (fl[i] & FLAG_FIXED) ? 1u : 0u;
+            }
+            auto get = [&](int p, float *dst, int stride, int comp) -> hipError_t {
+                hipError_t e = hipMemcpy(buf.data(), a.plane(p) + r.start, r.count * 4, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) return e;
+                for (uint32_t i = 0; i < r.count; ++i) dst[(size_t)ids[i] * stride + comp] = buf[i];
+                return hipSuccess;
+            };
+            for (int c = 0; c < 3; ++c) HIP_TRY(get(PX + c, hp.x.data(), 3, c));
+            for (int c = 0; c < 3; ++c) HIP_TRY(get(PVX + c, hp.v.data(), 3, c));
+            for (int c = 0; c < 9; ++c) HIP_TRY(get(PC0 + c, hp.C.data(), 9, c));
+            for (int c = 0; c < 9; ++c) HIP_TRY(get(PF0 + c, hp.F.data(), 9, c));
+            HIP_TRY(get(PMASS, hp.mass.data(), 1, 0));
+            HIP_TRY(get(PVOL0, hp.vol0.data(), 1, 0));
+            // plastic state: only the sand tags carry it
+            std::vector<float> ps(3 * (size_t)r.count);
+            for (int c = 0; c < 3; ++c) {
+                HIP_TRY(hipMemcpy(buf.data(), a.plane(PS0 + c) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+                for (uint32_t i = 0; i < r.count; ++i) {
+                    uint32_t id = ids[i];
+                    if (hp.tag[id] == 2u || hp.tag[id] == 3u) hp.pstate[3 * (size_t)id + c] = buf[i];
+                }
+            }
+        }
+        return NXM_OK;
+    }
+
+    nxm_status positions(nxm_scene *sc, float *xyz)
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        Arena a = arena(cur_);
+        std::vector<float> buf;
+        std::vector<uint32_t> ids;
+        for (const SceneRange &r : sc->ranges) {
+            buf.resize(r.count);
+            ids.resize(r.count);
+            HIP_TRY(hipMemcpy(ids.data(), a.uplane(PID) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            for (int c = 0; c < 3; ++c) {
+                HIP_TRY(hipMemcpy(buf.data(), a.plane(PX + c) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+                for (uint32_t i = 0; i < r.count; ++i) xyz[3 * (size_t)ids[i] + c] = buf[i];
+            }
+        }
+        return NXM_OK;
+    }
+
+    // Enqueues one substep of every active scene.
+    nxm_status step(nxm_stage_times *times)
+    {
+        HIP_TRY(hipSetDevice(ctx_->device));
+        if (dirty_) {
+            nxm_status st = rebuild_tables();
+            if (st != NXM_OK) return st;
+        }
+        if (pin_dirty_) {
+            Globals gh{};
+            gh.p_in = p_in_;
+            HIP_TRY(hipMemcpyAsync(g_.p, &gh, sizeof(Globals), hipMemcpyHostToDevice, stream_));
+            HIP_TRY(hipStreamSynchronize(stream_));
+            pin_dirty_ = false;
+        }
+        if (p_in_ == 0 || b_total_ == 0 || n_regions_ == 0) return NXM_OK;
+        WorldDev w = view();
+        LaunchConfig lc{};
+        lc.p_hint = p_in_;
+        lc.persistent_blocks = ctx_->num_cus * 8;
+        lc.p2g_blocks = ctx_->num_cus * 32;
+        launch_substep(w, lc, stream_, times ? ev_ : nullptr);
+        HIP_TRY(hipGetLastError());
+        // The output arena holds every live particle, scenes in slot order.
+        cur_ ^= 1;
+        uint32_t at = 0;
+        for (uint32_t i = 0; i < slots_.size(); ++i) {
+            if (slots_[i].state == SLOT_RETIRING) { slots_[i].state = SLOT_FREE; dirty_ = true; }
+            if (slots_[i].state != SLOT_ACTIVE) continue;
+            nxm_scene *sc = slots_[i].scene;
+            uint32_t n = (uint32_t)sc->hp.n;
+            sc->ranges.assign(1, SceneRange{at, n});
+            at += n;
+        }
+        if (at != p_in_) {
+            p_in_ = at;
+            pin_dirty_ = true;
+        }
+        // p_in on the device: the step's p_out, which equals p_in unless scenes retired.
+        if (pin_dirty_) {
+            HIP_TRY(hipMemcpyAsync(&g_.p->p_in, &g_.p->p_out, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+            pin_dirty_ = false;
+        }
+        stepped_ = true;
+        if (times) {
+            HIP_TRY(hipEventSynchronize(ev_[4]));
+            float a = 0, b = 0, c = 0, d = 0;
+            HIP_TRY(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+            HIP_TRY(hipEventElapsedTime(&b, ev_[1], ev_[2]));
+            HIP_TRY(hipEventElapsedTime(&c, ev_[2], ev_[3]));
+            HIP_TRY(hipEventElapsedTime(&d, ev_[3], ev_[4]));
+            times->sort_ms = a;
+            times->p2g_ms = b;
+            times->grid_update_ms = c;
+            times->g2p_update_ms = d;
+            times->total_ms = a + b + c + d;
+            return collect_errors();
+        }
+        return NXM_OK;
+    }
+
+    // ---- inspection of the last step ----
+    nxm_status sort_size(nxm_scene *sc, size_t *nb)
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        *nb = 0;
+        if (!stepped_) return NXM_OK;
+        SceneCtr c;
+        HIP_TRY(hipMemcpy(&c, ctr_.p + sc->slot, sizeof(c), hipMemcpyDeviceToHost));
+        *nb = c.b_count > 0 ? c.b_count - 1 : 0;   // the overflow block is not reported
+        last_b_start_ = c.b_start;
+        return NXM_OK;
+    }
+
+    nxm_status canonical(nxm_scene *sc, const nxm_sort_canonical *out)
+    {
+        size_t nb = 0;
+        nxm_status st = sort_size(sc, &nb);
+        if (st != NXM_OK) return st;
+        if (!stepped_) return fail(NXM_ERR_INVALID_ARGUMENT, "no step has run");
+        const uint32_t b0 = last_b_start_;
+        DevBuf<uint32_t> pcell, buckets;
+        HIP_TRY(pcell.reserve(p_cap_));
+        HIP_TRY(buckets.reserve((size_t)10 * b_total_));
+        HIP_TRY(hipMemsetAsync(buckets.p, 0, sizeof(uint32_t) * 10 * b_total_, stream_));
+        WorldDev w = view();
+        // The last step read arena cur_^1 (pre-step positions) and wrote arena cur_.
+        launch_canonical(w, arena(cur_ ^ 1), pcell.p, buckets.p, p_cap_, stream_);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream_));
+        std::vector<int4> vid(nb);
+        std::vector<uint32_t> np(nb), bk(10 * nb);
+        if (nb) {
+            HIP_TRY(hipMemcpy(vid.data(), bvid_.p + b0, nb * sizeof(int4), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(np.data(), bnp_.p + b0, nb * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(bk.data(), buckets.p + 10 * (size_t)b0, nb * 40, hipMemcpyDeviceToHost));
+        }
+        for (size_t b = 0; b < nb; ++b) {
+            if (out->block_vid) { out->block_vid[3 * b] = vid[b].x; out->block_vid[3 * b + 1] = vid[b].y; out->block_vid[3 * b + 2] = vid[b].z; }
+            if (out->block_np) out->block_np[b] = np[b];
+            uint32_t tot = 0;
+            for (int k = 0; k < 10; ++k) {
+                tot += bk[10 * b + k];
+                if (out->block_buckets) out->block_buckets[10 * b + k] = bk[10 * b + k];
+            }
+            if (out->block_npe) out->block_npe[b] = tot;
+        }
+        // particles of the scene: its output range
+        for (const SceneRange &r : sc->ranges) {
+            std::vector<uint32_t> ids(r.count), sb(r.count), pc(r.count);
+            HIP_TRY(hipMemcpy(ids.data(), arena(cur_).uplane(PID) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(sb.data(), sblock_.p + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(pc.data(), pcell.p + r.start, r.count * 4, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < r.count; ++i) {
+                if (out->particle_block) out->particle_block[ids[i]] = sb[i] - b0;
+                if (out->particle_cell) out->particle_cell[ids[i]] = pc[i];
+            }
+        }
+        return NXM_OK;
+    }
+
+    nxm_status grid(nxm_scene *sc, float *mass, float *mom, float *vel)
+    {
+        size_t nb = 0;
+        nxm_status st = sort_size(sc, &nb);
+        if (st != NXM_OK) return st;
+        const uint32_t b0 = last_b_start_;
+        std::vector<float4> gv(nb * 64);
+        if (nb) HIP_TRY(hipMemcpy(gv.data(), gridv_.p + (size_t)b0 * 64, nb * 64 * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nb * 64; ++i) {
+            if (mass) mass[i] = gv[i].w;
+            if (vel) { vel[3 * i] = gv[i].x; vel[3 * i + 1] = gv[i].y; vel[3 * i + 2] = gv[i].z; }
+        }
+        if (mom) {
+            if (!dbg_mom_.p) return fail(NXM_ERR_INVALID_ARGUMENT, "grid momentum needs nxm_scene_set_debug_grid before the step");
+            HIP_TRY(hipMemcpy(mom, dbg_mom_.p + (size_t)b0 * 64 * 3, nb * 64 * 3 * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        return NXM_OK;
+    }
+
+  private:
+    Arena arena(int k) const { return Arena{arenas_[k].p, p_cap_}; }
+
+    WorldDev view() const
+    {
+        WorldDev w{};
+        w.in = arena(cur_);
+        w.out = arena(cur_ ^ 1);
+        w.pkey = pkey_.p; w.pdense = pdense_.p; w.prank = prank_.p; w.perm = perm_.p; w.sblock = sblock_.p;
+        w.bscene = bscene_.p; w.bvid = bvid_.p; w.bnp = bnp_.p; w.bfirst = bfirst_.p;
+        w.bup = bup_.p; w.bdn = bdn_.p; w.stage = stage_.p; w.gridv = gridv_.p;
+        w.hkeys = hkeys_.p; w.hvals = hvals_.p; w.hmark = hmark_.p; w.bprim = bprim_.p;
+        w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
+        w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
+        w.g = g_.p; w.scan_tmp = scan_tmp_.p;
+        w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
+        w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
+        return w;
+    }
+
+    nxm_status ensure_particles(uint32_t need)
+    {
+        if (need <= p_cap_) return NXM_OK;
+        uint32_t cap = std::max<uint32_t>(need, p_cap_ + p_cap_ / 2);
+        cap = (cap + 255u) & ~255u;
+        DevBuf<float> na[2];
+        HIP_TRY(na[0].reserve((size_t)NUM_PLANES * cap));
+        HIP_TRY(na[1].reserve((size_t)NUM_PLANES * cap));
+        if (p_in_) {
+            for (int p = 0; p < NUM_PLANES; ++p)
+                HIP_TRY(hipMemcpy(na[cur_].p + (size_t)p * cap, arenas_[cur_].p + (size_t)p * p_cap_, (size_t)p_in_ * 4,
+                                  hipMemcpyDeviceToDevice));
+        }
+        for (int k = 0; k < 2; ++k) {
+            arenas_[k].release();
+            std::swap(arenas_[k].p, na[k].p);
+            std::swap(arenas_[k].n, na[k].n);
+        }
+        p_cap_ = cap;
+        HIP_TRY(pkey_.reserve(cap));
+        HIP_TRY(pdense_.reserve(cap));
+        HIP_TRY(prank_.reserve(cap));
+        HIP_TRY(perm_.reserve(cap));
+        HIP_TRY(sblock_.reserve(cap));
+        return NXM_OK;
+    }
+
+    // Lays out block ids, hash regions, classes and colliders of the active scenes
+    // in slot order and uploads the tables.
+    nxm_status rebuild_tables()
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        layout_.assign(slots_.size(), SceneDev{});
+        std::vector<ClassDev> cls;
+        std::vector<ColliderDev> cols;
+        std::vector<uint32_t> region_start, region_scene;
+        uint32_t b = 0, hsum = 0;
+        for (uint32_t i = 0; i < slots_.size(); ++i) {
+            SceneDev &L = layout_[i];
+            if (slots_[i].state != SLOT_ACTIVE) continue;
+            nxm_scene *sc = slots_[i].scene;
+            L.h = sc->h;
+            L.inv_h = 1.0f / sc->h;
+            L.dt = sc->dt;
+            L.vlim = sc->h / sc->dt;
+            L.gx = sc->g[0]; L.gy = sc->g[1]; L.gz = sc->g[2];
+            L.inv_d = 4.0f / (sc->h * sc->h);
+            L.active = 1;
+            L.b_cap = sc->grid_cap;
+            b += sc->grid_cap + 1;
+            L.h_base = hsum;
+            L.h_cap = next_pow2(2 * sc->grid_cap);
+            hsum += L.h_cap + 1;
+            region_start.push_back(L.h_base);
+            region_scene.push_back(i);
+            L.col_start = (uint32_t)cols.size();
+            L.col_count = (uint32_t)sc->cols.size();
+            cols.insert(cols.end(), sc->cols.begin(), sc->cols.end());
+            L.class_base = (uint32_t)cls.size();
+            cls.insert(cls.end(), sc->classes.begin(), sc->classes.end());
+        }
+        // Dense block ids are ranks of occupied hash slots: sizing the block tables by
+        // the slot count keeps every rank (and each scene's overflow block) in range
+        // even when a scene exceeds its capacity (reported as an error).
+        (void)b;
+        h_total_ = hsum;
+        b_total_ = hsum;
+        HIP_TRY(scenes_.reserve(std::max<size_t>(slots_.size(), 1)));
+        HIP_TRY(ctr_.reserve(std::max<size_t>(slots_.size(), 1)));
+        if (ctr_.n > ctr_zeroed_) {
+            HIP_TRY(hipMemset(ctr_.p, 0, ctr_.n * sizeof(SceneCtr)));
+            ctr_zeroed_ = ctr_.n;
+        }
+        HIP_TRY(classes_.reserve(std::max<size_t>(cls.size(), 1)));
+        HIP_TRY(cols_.reserve(std::max<size_t>(cols.size(), 1)));
+        if (!layout_.empty())
+            HIP_TRY(hipMemcpy(scenes_.p, layout_.data(), layout_.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
+        if (!cls.empty()) HIP_TRY(hipMemcpy(classes_.p, cls.data(), cls.size() * sizeof(ClassDev), hipMemcpyHostToDevice));
+        if (!cols.empty()) HIP_TRY(hipMemcpy(cols_.p, cols.data(), cols.size() * sizeof(ColliderDev), hipMemcpyHostToDevice));
+        if (b_total_ > bscene_.n) {
+            size_t nb = (size_t)b_total_ + b_total_ / 4;
+            HIP_TRY(bscene_.reserve(nb));
+            HIP_TRY(bprim_.reserve(nb));
+            HIP_TRY(bvid_.reserve(nb));
+            HIP_TRY(bnp_.reserve(nb));
+            HIP_TRY(bfirst_.reserve(nb));
+            HIP_TRY(bup_.reserve(8 * nb));
+            HIP_TRY(bdn_.reserve(8 * nb));
+            HIP_TRY(stage_.reserve((size_t)4 * TILE_NODES * nb));
+            HIP_TRY(gridv_.reserve((size_t)NODES_PER_BLOCK * nb));
+        }
+        HIP_TRY(scan_tmp_.reserve(std::max<size_t>(b_total_, h_total_) / 1024 + 2));
+        if (debug_) HIP_TRY(dbg_mom_.reserve((size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(b_total_, 1)));
+        HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
+        HIP_TRY(hvals_.reserve(std::max<uint32_t>(h_total_, 1)));
+        HIP_TRY(hmark_.reserve(std::max<uint32_t>(h_total_, 1)));
+        n_regions_ = (uint32_t)region_start.size();
+        HIP_TRY(region_start_.reserve(std::max<uint32_t>(n_regions_, 1)));
+        HIP_TRY(region_scene_.reserve(std::max<uint32_t>(n_regions_, 1)));
+        if (n_regions_) {
+            HIP_TRY(hipMemcpy(region_start_.p, region_start.data(), n_regions_ * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(region_scene_.p, region_scene.data(), n_regions_ * 4, hipMemcpyHostToDevice));
+        }
+        dirty_ = false;
+        return NXM_OK;
+    }
+
+    nxm_status collect_errors()
+    {
+        if (slots_.empty() || !ctr_.p) return NXM_OK;
+        std::vector<SceneCtr> c(slots_.size());
+        HIP_TRY(hipMemcpy(c.data(), ctr_.p, c.size() * sizeof(SceneCtr), hipMemcpyDeviceToHost));
+        nxm_status first = NXM_OK;
+        bool any = false;
+        for (uint32_t i = 0; i < slots_.size(); ++i) {
+            if (!c[i].err) continue;
+            any = true;
+            if (slots_[i].scene) slots_[i].scene->error |= c[i].err;
+        }
+        if (any) {
+            HIP_TRY(hipMemset(ctr_.p, 0, c.size() * sizeof(SceneCtr)));
+            for (uint32_t i = 0; i < slots_.size(); ++i) {
+                if (!c[i].err) continue;
+                if (first == NXM_OK) {
+                    if (c[i].err & ERR_GRID_CAPACITY)
+                        first = fail(NXM_ERR_GRID_CAPACITY, "active blocks exceed the grid capacity (scene slot " + std::to_string(i) + ")");
+                    else if (c[i].err & ERR_DOMAIN)
+                        first = fail(NXM_ERR_DOMAIN, "particle block outside the packable domain (scene slot " + std::to_string(i) + ")");
+                    else
+                        first = fail(NXM_ERR_DEVICE, "device error bits " + std::to_string(c[i].err));
+                }
+            }
+        }
+        return first;
+    }
+
+  public:
+    void set_debug(bool on)
+    {
+        if (on && !debug_) dirty_ = true;
+        debug_ = on;
+    }
+
+  private:
+    nxm_ctx *ctx_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[5] = {};
+    std::vector<Slot> slots_;
+    std::vector<SceneDev> layout_;
+    bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false;
+    int cur_ = 0;
+    uint32_t p_in_ = 0, p_cap_ = 0, b_total_ = 0, h_total_ = 0;
+    size_t ctr_zeroed_ = 0;
+    DevBuf<float> arenas_[2];
+    DevBuf<uint32_t> pkey_, pdense_, prank_, perm_, sblock_;
+    DevBuf<uint32_t> bscene_, bprim_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_, scan_tmp_;
+    DevBuf<uint32_t> region_start_, region_scene_;
+    uint32_t n_regions_ = 0, last_b_start_ = 0;
+    DevBuf<int4> bvid_;
+    DevBuf<float> stage_, dbg_mom_;
+    DevBuf<float4> gridv_;
+    DevBuf<SceneDev> scenes_;
+    DevBuf<SceneCtr> ctr_;
+    DevBuf<ClassDev> classes_;
+    DevBuf<ColliderDev> cols_;
+    DevBuf<Globals> g_;
+};
+
+}  // namespace nxm
+
+struct nxm_batch {
+    nxm_ctx *ctx = nullptr;
+    std::unique_ptr<World> world;
+    std::vector<nxm_scene *> members;
+};
+
+// ======================================================================================
+// Scene helpers
+
+static bool is_sand(uint32_t tag) { return tag == 2u || tag == 3u; }
+
+// Interns the material class of one particle: the model tag, its parameter words (the
+// sand tags' first three words are per-particle plastic state, not parameters) and the
+// damping.
+static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *words, float damping, uint32_t *out)
+{
+    std::array<uint32_t, 26> key{};
+    key[0] = tag;
+    for (int i = 0; i < 24; ++i) key[1 + i] = (is_sand(tag) && i < 3) ? 0u : words[i];
+    std::memcpy(&key[25], &damping, 4);
+    auto it = sc->class_index.find(key);
+    if (it != sc->class_index.end()) {
+        *out = it->second;
+        return NXM_OK;
+    }
+    if (sc->classes.size() >= MAX_CLASSES) return fail(NXM_ERR_INVALID_ARGUMENT, "more than 16384 distinct materials in one scene");
+    ClassDev c{};
+    c.tag = tag;
+    c.damping = damping;
+    std::memcpy(c.w, words, sizeof(c.w));
+    uint32_t id = (uint32_t)sc->classes.size();
+    sc->classes.push_back(c);
+    sc->class_index.emplace(key, id);
+    *out = id;
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+static nxm_status ensure_own_world(nxm_scene *sc)
+{
+    if (sc->batch) return NXM_OK;
+    if (!sc->own) {
+        sc->own = std::make_unique<World>(sc->ctx);
+        nxm_status st = sc->own->init();
+        if (st != NXM_OK) { sc->own.reset(); return st; }
+    }
+    if (sc->world == nullptr) return sc->own->add(sc);
+    return NXM_OK;
+}
+
+static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const nxm_particle *aos, size_t n)
+{
+    HostParticles &hp = sc->hp;
+    size_t base = hp.n;
+    // state of already-resident particles is kept on the device; only the new
+    // particles are staged on the host and uploaded
+    hp.resize(base + n);
+    for (size_t k = 0; k < n; ++k) {
+        size_t i = base + k;
+        float pos[3], vel[3] = {0, 0, 0}, F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, C[9] = {0};
+        float vol, rad, mass, damp = 0.0f, phase = 1.0f, bfr = 1.0f;
+        uint32_t grp = 0, en = 1, fx = 0, tag;
+        const uint32_t *words;
+        if (aos) {
+            const nxm_particle &a = aos[k];
+            std::memcpy(pos, a.position, 12);
+            std::memcpy(vel, a.velocity, 12);
+            std::memcpy(F, a.def_grad, 36);
+            std::memcpy(C, a.affine, 36);
+            vol = a.init_volume; rad = a.init_radius; mass = a.mass; damp = a.damping; phase = a.phase;
+            bfr = a.boundary_friction; grp = a.group_id; en = a.enabled; fx = a.fixed; tag = a.model_tag;
+            words = a.model_data;
+        } else {
+            if (!p->position || !p->init_volume || !p->mass || !p->model_tag || !p->model_data)
+                return fail(NXM_ERR_INVALID_ARGUMENT, "position, init_volume, mass, model_tag and model_data are required");
+            std::memcpy(pos, p->position + 3 * k, 12);
+            if (p->velocity) std::memcpy(vel, p->velocity + 3 * k, 12);
+            if (p->def_grad) std::memcpy(F, p->def_grad + 9 * k, 36);
+            if (p->affine) std::memcpy(C, p->affine + 9 * k, 36);
+            vol = p->init_volume[k];
+            rad = p->init_radius ? p->init_radius[k] : 0.5f * std::cbrt(vol);
+            mass = p->mass[k];
+            if (p->damping) damp = p->damping[k];
+            if (p->phase) phase = p->phase[k];
+            if (p->boundary_friction) bfr = p->boundary_friction[k];
+            if (p->group_id) grp = p->group_id[k];
+            if (p->enabled) en = p->enabled[k];
+            if (p->fixed) fx = p->fixed[k];
+            tag = p->model_tag[k];
+            words = p->model_data + 24 * k;
+        }
+        if (tag == NXM_MODEL_SNOW) return fail(NXM_ERR_UNSUPPORTED_MODEL, "the snow model is not supported");
+        std::memcpy(&hp.x[3 * i], pos, 12);
+        std::memcpy(&hp.v[3 * i], vel, 12);
+        std::memcpy(&hp.F[9 * i], F, 36);
+        std::memcpy(&hp.C[9 * i], C, 36);
+        hp.mass[i] = mass;
+        hp.vol0[i] = vol;
+        hp.radius[i] = rad;
+        hp.damping[i] = damp;
+        hp.phase[i] = phase;
+        hp.bfric[i] = bfr;
+        hp.group[i] = grp;
+        hp.enabled[i] = en;
+        hp.fixed[i] = fx;
+        hp.tag[i] = tag;
+        std::memcpy(&hp.words[24 * i], words, 96);
+        if (is_sand(tag)) std::memcpy(&hp.pstate[3 * i], words, 12);
+        else hp.pstate[3 * i] = hp.pstate[3 * i + 1] = hp.pstate[3 * i + 2] = 0.0f;
+        nxm_status st = intern_class(sc, tag, words, damp, &hp.cls[i]);
+        if (st != NXM_OK) return st;
+    }
+    if (sc->world) return sc->world->upload(sc, base, n);
+    return NXM_OK;
+}
+
+static World *world_of(nxm_scene *sc) { return sc->world; }
+
+// ======================================================================================
+// C ABI
+
+extern "C" {
+
+const char *nxm_last_error(void) { return g_last_error.c_str(); }
+const char *nxm_version(void) { return "nexus_amd mpm 0.1 (gfx950)"; }
+
+nxm_status nxm_ctx_create(int device, nxm_ctx **out)
+{
+    if (!out) return fail(NXM_ERR_INVALID_ARGUMENT, "out is null");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(NXM_ERR_INVALID_ARGUMENT, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    auto *c = new nxm_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    *out = c;
+    return NXM_OK;
+}
+
+void nxm_ctx_destroy(nxm_ctx *ctx) { delete ctx; }
+
+nxm_status nxm_scene_create(nxm_ctx *ctx, const nxm_capacities *caps, nxm_scene **out)
+{
+    if (!ctx || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    auto *sc = new nxm_scene();
+    sc->ctx = ctx;
+    if (caps && caps->grid_size) sc->grid_cap = next_pow2(caps->grid_size);
+    nxm_status st = ensure_own_world(sc);
+    if (st != NXM_OK) { delete sc; return st; }
+    *out = sc;
+    return NXM_OK;
+}
+
+void nxm_scene_destroy(nxm_scene *sc)
+{
+    if (!sc) return;
+    if (sc->batch) {
+        auto &m = sc->batch->members;
+        m.erase(std::remove(m.begin(), m.end(), sc), m.end());
+        if (sc->world) (void)sc->world->remove(sc);
+    }
+    sc->own.reset();
+    delete sc;
+}
+
+nxm_status nxm_scene_set_simulation_params(nxm_scene *sc, const nxm_sim_params *p)
+{
+    if (!sc || !p) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(p->dt > 0.0f)) return fail(NXM_ERR_INVALID_ARGUMENT, "dt must be positive");
+    std::memcpy(sc->g, p->gravity, 12);
+    sc->base_dt = p->dt;
+    sc->dt = p->dt;
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_write_substep_params(nxm_scene *sc, uint32_t num_substeps)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    sc->dt = sc->base_dt / (float)std::max<uint32_t>(num_substeps, 1u);
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_set_cell_width(nxm_scene *sc, float h, uint32_t grid_capacity)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (!(h > 0.0f)) return fail(NXM_ERR_INVALID_ARGUMENT, "cell width must be positive");
+    if (grid_capacity == 0 || grid_capacity > (1u << 26)) return fail(NXM_ERR_INVALID_ARGUMENT, "grid capacity out of range");
+    sc->h = h;
+    sc->grid_cap = next_pow2(grid_capacity);
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_add_particles(nxm_scene *sc, const nxm_particle *particles, size_t n)
+{
+    if (!sc || (!particles && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    return append_host(sc, nullptr, particles, n);
+}
+
+nxm_status nxm_scene_add_particles_soa(nxm_scene *sc, const nxm_particles_soa *p, size_t n)
+{
+    if (!sc || (!p && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    return append_host(sc, p, nullptr, n);
+}
+
+nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
+{
+    if (!sc || (!c && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
+    sc->cols.resize(n);
+    static_assert(sizeof(ColliderDev) == sizeof(nxm_cuboid), "collider layout");
+    if (n) std::memcpy(sc->cols.data(), c, n * sizeof(nxm_cuboid));
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+size_t nxm_scene_len(const nxm_scene *sc) { return sc ? sc->hp.n : 0; }
+
+static nxm_status latched(nxm_scene *sc, nxm_status st)
+{
+    if (st != NXM_OK) return st;
+    if (sc->error) {
+        uint32_t e = sc->error;
+        if (e & ERR_GRID_CAPACITY) return fail(NXM_ERR_GRID_CAPACITY, "active blocks exceeded the grid capacity");
+        if (e & ERR_DOMAIN) return fail(NXM_ERR_DOMAIN, "a particle block is outside the packable domain");
+        return fail(NXM_ERR_DEVICE, "device error");
+    }
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_synchronize(nxm_scene *sc)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    nxm_status st = sc->world ? sc->world->sync() : NXM_OK;
+    return latched(sc, st);
+}
+
+nxm_status nxm_scene_read_positions(nxm_scene *sc, float *xyz, size_t n)
+{
+    if (!sc || (!xyz && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < sc->hp.n) return fail(NXM_ERR_INVALID_ARGUMENT, "output too small");
+    if (!sc->world) {
+        std::memcpy(xyz, sc->hp.x.data(), sc->hp.n * 12);
+        return latched(sc, NXM_OK);
+    }
+    return latched(sc, sc->world->positions(sc, xyz));
+}
+
+nxm_status nxm_scene_read_particles_soa(nxm_scene *sc, const nxm_particles_soa *o, size_t n)
+{
+    if (!sc || !o) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < sc->hp.n) return fail(NXM_ERR_INVALID_ARGUMENT, "output too small");
+    if (sc->world) {
+        nxm_status st = sc->world->pull(sc);
+        if (st != NXM_OK) return st;
+    }
+    const HostParticles &hp = sc->hp;
+    size_t m = hp.n;
+    if (o->position) std::memcpy(o->position, hp.x.data(), m * 12);
+    if (o->velocity) std::memcpy(o->velocity, hp.v.data(), m * 12);
+    if (o->def_grad) std::memcpy(o->def_grad, hp.F.data(), m * 36);
+    if (o->affine) std::memcpy(o->affine, hp.C.data(), m * 36);
+    if (o->init_volume) std::memcpy(o->init_volume, hp.vol0.data(), m * 4);
+    if (o->init_radius) std::memcpy(o->init_radius, hp.radius.data(), m * 4);
+    if (o->mass) std::memcpy(o->mass, hp.mass.data(), m * 4);
+    if (o->damping) std::memcpy(o->damping, hp.damping.data(), m * 4);
+    if (o->phase) std::memcpy(o->phase, hp.phase.data(), m * 4);
+    if (o->boundary_friction) std::memcpy(o->boundary_friction, hp.bfric.data(), m * 4);
+    if (o->group_id) std::memcpy(o->group_id, hp.group.data(), m * 4);
+    if (o->enabled) std::memcpy(o->enabled, hp.enabled.data(), m * 4);
+    if (o->fixed) std::memcpy(o->fixed, hp.fixed.data(), m * 4);
+    if (o->model_tag) std::memcpy(o->model_tag, hp.tag.data(), m * 4);
+    if (o->model_data) {
+        for (size_t i = 0; i < m; ++i) {
+            std::memcpy(o->model_data + 24 * i, &hp.words[24 * i], 96);
+            if (is_sand(hp.tag[i])) std::memcpy(o->model_data + 24 * i, &hp.pstate[3 * i], 12);
+        }
+    }
+    return latched(sc, NXM_OK);
+}
+
+nxm_status nxm_scene_sort_size(nxm_scene *sc, size_t *nb)
+{
+    if (!sc || !nb) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!sc->world) { *nb = 0; return NXM_OK; }
+    return sc->world->sort_size(sc, nb);
+}
+
+nxm_status nxm_scene_read_sort_canonical(nxm_scene *sc, const nxm_sort_canonical *out)
+{
+    if (!sc || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    return sc->world->canonical(sc, out);
+}
+
+nxm_status nxm_scene_read_grid(nxm_scene *sc, float *mass, float *momentum, float *velocity)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    return sc->world->grid(sc, mass, momentum, velocity);
+}
+
+nxm_status nxm_scene_set_debug_grid(nxm_scene *sc, int enabled)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    sc->debug_grid = enabled != 0;
+    if (sc->world) sc->world->set_debug(enabled != 0);
+    return NXM_OK;
+}
+
+nxm_status nxm_pipeline_step(nxm_ctx *ctx, nxm_scene *sc, nxm_stage_times *times)
+{
+    if (!ctx || !sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (sc->batch) return fail(NXM_ERR_BUSY, "scene belongs to a batch");
+    nxm_status st = ensure_own_world(sc);
+    if (st != NXM_OK) return st;
+    return sc->world->step(times);
+}
+
+nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *sc, uint32_t substeps)
+{
+    for (uint32_t i = 0; i < substeps; ++i) {
+        nxm_status st = nxm_pipeline_step(ctx, sc, nullptr);
+        if (st != NXM_OK) return st;
+    }
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_create(nxm_ctx *ctx, nxm_batch **out)
+{
+    if (!ctx || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    auto *b = new nxm_batch();
+    b->ctx = ctx;
+    b->world = std::make_unique<World>(ctx);
+    nxm_status st = b->world->init();
+    if (st != NXM_OK) { delete b; return st; }
+    *out = b;
+    return NXM_OK;
+}
+
+void nxm_batch_destroy(nxm_batch *b)
+{
+    if (!b) return;
+    std::vector<nxm_scene *> m = b->members;
+    for (nxm_scene *sc : m) (void)nxm_batch_remove_scene(b, sc);
+    delete b;
+}
+
+nxm_status nxm_batch_add_scene(nxm_batch *b, nxm_scene *sc)
+{
+    if (!b || !sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (sc->batch) return fail(NXM_ERR_BUSY, "scene already belongs to a batch");
+    if (sc->world) {
+        nxm_status st = sc->world->remove(sc);
+        if (st != NXM_OK) return st;
+    }
+    sc->batch = b;
+    nxm_status st = b->world->add(sc);
+    if (st != NXM_OK) { sc->batch = nullptr; return st; }
+    if (sc->debug_grid) b->world->set_debug(true);
+    b->members.push_back(sc);
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_remove_scene(nxm_batch *b, nxm_scene *sc)
+{
+    if (!b || !sc || sc->batch != b) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not in this batch");
+    nxm_status st = b->world->remove(sc);
+    if (st != NXM_OK) return st;
+    sc->batch = nullptr;
+    auto &m = b->members;
+    m.erase(std::remove(m.begin(), m.end(), sc), m.end());
+    return ensure_own_world(sc);
+}
+
+nxm_status nxm_batch_step(nxm_batch *b, nxm_stage_times *times)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->world->step(times);
+}
+
+nxm_status nxm_batch_run(nxm_batch *b, uint32_t substeps)
+{
+    for (uint32_t i = 0; i < substeps; ++i) {
+        nxm_status st = nxm_batch_step(b, nullptr);
+        if (st != NXM_OK) return st;
+    }
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_synchronize(nxm_batch *b)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->world->sync();
+}
+
+size_t nxm_batch_len(const nxm_batch *b) { return b ? b->members.size() : 0; }
+
+nxm_status nxm_batch_set_graph(nxm_batch *b, int enabled)
+{
+    (void)enabled;
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return NXM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,472 @@
+// mpm_device.h — device-side data layout and per-particle / per-node math of the
+// MI355X MLS-MPM substep.  Everything here is __device__ code shared by the kernels in
+// mpm_world.hip.  Citations are file:line in the dimforge/nexus tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nxm {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
+constexpr int WAVE = 64;
+constexpr int TILE = 6;                   // 6^3 node tile: a block plus its +1 halo
+constexpr int TILE_NODES = TILE * TILE * TILE;
+constexpr int NODES_PER_BLOCK = 64;       // 4^3 (src_mpm_shaders/grid/grid.rs:25)
+constexpr float MAX_FLT_REF = 1.0e18f;    // src_rbd_shaders/lib.rs:283
+
+// Particle flags word: bit 0 enabled, bit 1 fixed, bits 2..15 material class,
+// bits 16..31 scene slot.
+constexpr uint32_t FLAG_ENABLED = 1u;
+constexpr uint32_t FLAG_FIXED = 2u;
+__host__ __device__ inline uint32_t flags_class(uint32_t f) { return (f >> 2) & 0x3FFFu; }
+__host__ __device__ inline uint32_t flags_scene(uint32_t f) { return f >> 16; }
+__host__ __device__ inline uint32_t make_flags(bool enabled, bool fixed, uint32_t cls, uint32_t scene)
+{
+    return (enabled ? FLAG_ENABLED : 0u) | (fixed ? FLAG_FIXED : 0u) | ((cls & 0x3FFFu) << 2) | (scene << 16);
+}
+constexpr uint32_t MAX_CLASSES = 1u << 14;
+constexpr uint32_t MAX_SCENES = 1u << 16;
+
+// Particle arena planes (structure of arrays, one f32/u32 plane per component).
+enum Plane : int {
+    PX = 0, PY, PZ,
+    PVX, PVY, PVZ,
+    PC0, PC1, PC2, PC3, PC4, PC5, PC6, PC7, PC8,   // affine, column-major
+    PF0, PF1, PF2, PF3, PF4, PF5, PF6, PF7, PF8,   // deformation gradient, column-major
+    PMASS, PVOL0,
+    PS0, PS1, PS2,                                  // Drucker-Prager plastic state
+    PFLAGS, PID,
+    NUM_PLANES
+};
+
+struct Arena {
+    float *base;       // NUM_PLANES planes of `stride` elements
+    uint32_t stride;
+    __device__ __host__ float *plane(int p) const { return base + (size_t)p * stride; }
+    __device__ __host__ uint32_t *uplane(int p) const { return reinterpret_cast<uint32_t *>(base + (size_t)p * stride); }
+};
+
+// One scene of a world.
+struct SceneDev {
+    float h, inv_h, dt, vlim;     // vlim = h / dt
+    float gx, gy, gz, inv_d;      // inv_d = 4 / h^2
+    uint32_t active;
+    uint32_t b_cap;               // block capacity (the overflow block not included)
+    uint32_t h_base, h_cap;       // hash region [h_base, h_base + h_cap) (h_cap a power of two), then one overflow slot
+    uint32_t col_start, col_count;
+    uint32_t class_base;          // first entry of the scene's material classes
+};
+
+struct SceneCtr {
+    uint32_t b_start;  // first dense block id of the scene in the last step
+    uint32_t b_count;  // its blocks, the overflow block (last) included
+    uint32_t err;      // error bits
+    uint32_t pad;
+};
+constexpr uint32_t ERR_GRID_CAPACITY = 1u;
+constexpr uint32_t ERR_DOMAIN = 2u;
+constexpr uint32_t ERR_MODEL = 4u;
+
+struct Globals {
+    uint32_t p_in;       // particles in the input arena
+    uint32_t p_out;      // particles placed in the output arena (scan total)
+    uint32_t n_blocks;   // dense block ids in use (scan total of occupied hash slots)
+    uint32_t pad;
+};
+
+// Material class: model tag and words (src_mpm_shaders/models/default.rs:41-44),
+// plus the particle damping (ParticleProperties::damping, solver/particle.rs:166).
+struct ClassDev {
+    uint32_t tag;
+    float damping;
+    float w[24];
+    uint32_t pad[2];
+};
+
+// nxm_cuboid layout.
+struct ColliderDev {
+    float he[3];
+    float q[4];
+    float t[3];
+    float linvel[3];
+    float angvel[3];
+    uint32_t bc_type;
+    float bc_friction;
+};
+
+// Block virtual id marker of a scene's overflow block (particles that could not be
+// given a block: grid capacity exceeded or coordinates outside the packable range).
+constexpr int32_t OVERFLOW_VID = INT32_MIN;
+
+// ------------------------------------------------------------------------------------
+// Keys: block of a position (src_mpm_shaders/solver/particle.rs:208-225,
+// grid/grid.rs:122-134), packing (grid/grid.rs:102-106).
+
+// Rust's saturating f32 -> i32 cast (NaN -> 0).
+__device__ inline int32_t sat_i32(float f)
+{
+    if (f != f) return 0;
+    if (f >= 2147483520.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+
+__device__ inline void cell_of(float x, float h, int32_t &blk, uint32_t &local)
+{
+    float cell = roundf(x / h) - 1.0f;            // round half away from zero
+    float b = floorf(cell / 4.0f);
+    blk = sat_i32(b);
+    float l = cell - b * 4.0f;
+    local = (l != l) ? 0u : (uint32_t)l;
+}
+
+// Packable range: primaries keep their +1 neighbours inside the 11/10/11-bit key.
+__device__ inline bool vid_in_range(int32_t x, int32_t y, int32_t z)
+{
+    return x >= -1023 && x <= 1022 && y >= -511 && y <= 510 && z >= -1023 && z <= 1022;
+}
+
+__device__ inline uint32_t pack_vid(int32_t x, int32_t y, int32_t z)
+{
+    return ((uint32_t)(x + 0x3FF) & 0x7FFu) | (((uint32_t)(y + 0x1FF) & 0x3FFu) << 11) |
+           (((uint32_t)(z + 0x3FF) & 0x7FFu) << 21);
+}
+
+__device__ inline uint32_t hash_key(uint32_t k)
+{
+    k *= 0xCC9E2D51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1B873593u;
+    return k;
+}
+
+// +1 neighbour offsets in the reference's blocks_associated_to_block order
+// (grid/grid.rs:175-212); index 0 is the block itself.
+__device__ __constant__ const int8_t ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                                                     {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+// ------------------------------------------------------------------------------------
+// Small linear algebra (column-major 3x3).
+
+struct f3 { float x, y, z; };
+struct m3 { f3 c0, c1, c2; };
+
+__device__ inline f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ inline f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 cross(f3 a, f3 b) { return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+__device__ inline float length(f3 a) { return sqrtf(dot(a, a)); }
+
+__device__ inline m3 mzero() { return m3{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}; }
+__device__ inline m3 mident() { return m3{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}; }
+__device__ inline f3 mul(const m3 &m, f3 v) { return m.c0 * v.x + m.c1 * v.y + m.c2 * v.z; }
+__device__ inline m3 mul(const m3 &a, const m3 &b) { return m3{mul(a, b.c0), mul(a, b.c1), mul(a, b.c2)}; }
+__device__ inline m3 operator+(const m3 &a, const m3 &b) { return m3{a.c0 + b.c0, a.c1 + b.c1, a.c2 + b.c2}; }
+__device__ inline m3 operator-(const m3 &a, const m3 &b) { return m3{a.c0 - b.c0, a.c1 - b.c1, a.c2 - b.c2}; }
+__device__ inline m3 operator*(const m3 &a, float s) { return m3{a.c0 * s, a.c1 * s, a.c2 * s}; }
+__device__ inline m3 transpose(const m3 &a)
+{
+    return m3{{a.c0.x, a.c1.x, a.c2.x}, {a.c0.y, a.c1.y, a.c2.y}, {a.c0.z, a.c1.z, a.c2.z}};
+}
+__device__ inline float det(const m3 &a) { return dot(a.c2, cross(a.c0, a.c1)); }
+__device__ inline float trace(const m3 &a) { return a.c0.x + a.c1.y + a.c2.z; }
+
+// ------------------------------------------------------------------------------------
+// 3x3 SVD in registers: one-sided Jacobi (Hestenes' method) on the columns of F,
+// cyclic pairs (0,1), (0,2), (1,2), at most 8 sweeps with early exit once every
+// column pair is orthogonal to FP32 precision.  Returns F = U diag(s) V^T with U, V
+// rotations; s is not sorted, and its smallest entry carries the sign of det F.
+__device__ inline void jacobi_pair(f3 &ap, f3 &aq, f3 &vp, f3 &vq, float &off)
+{
+    float alpha = dot(ap, ap), beta = dot(aq, aq), gamma = dot(ap, aq);
+    float scale = sqrtf(alpha * beta);
+    float rel = scale > 0.0f ? fabsf(gamma) / scale : 0.0f;
+    off = fmaxf(off, rel);
+    if (!(rel > 1.0e-7f)) return;
+    float zeta = (beta - alpha) / (2.0f * gamma);
+    float az = fabsf(zeta);
+    float t = az > 1.0e15f ? 0.5f / zeta : copysignf(1.0f, zeta) / (az + sqrtf(1.0f + zeta * zeta));
+    float c = 1.0f / sqrtf(1.0f + t * t);
+    float s = c * t;
+    f3 a0 = ap, a1 = aq, w0 = vp, w1 = vq;
+    ap = a0 * c - a1 * s;
+    aq = a0 * s + a1 * c;
+    vp = w0 * c - w1 * s;
+    vq = w0 * s + w1 * c;
+}
+
+__device__ inline f3 any_orthogonal(f3 a)
+{
+    f3 e = fabsf(a.x) < 0.57f ? mk3(1, 0, 0) : (fabsf(a.y) < 0.57f ? mk3(0, 1, 0) : mk3(0, 0, 1));
+    f3 c = cross(a, e);
+    return c / length(c);
+}
+
+__device__ inline void svd3(const m3 &F, m3 &U, f3 &s, m3 &V)
+{
+    f3 a0 = F.c0, a1 = F.c1, a2 = F.c2;
+    f3 w0 = mk3(1, 0, 0), w1 = mk3(0, 1, 0), w2 = mk3(0, 0, 1);
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        float off = 0.0f;
+        jacobi_pair(a0, a1, w0, w1, off);
+        jacobi_pair(a0, a2, w0, w2, off);
+        jacobi_pair(a1, a2, w1, w2, off);
+        if (!(off > 1.0e-7f)) break;
+    }
+    float s0 = length(a0), s1 = length(a1), s2 = length(a2);
+    float big = fmaxf(s0, fmaxf(s1, s2));
+    float tiny = big * 1.0e-30f;
+    f3 u0 = s0 > tiny ? a0 / s0 : mk3(0, 0, 0);
+    f3 u1 = s1 > tiny ? a1 / s1 : mk3(0, 0, 0);
+    f3 u2 = s2 > tiny ? a2 / s2 : mk3(0, 0, 0);
+    // Complete U where columns vanished (rank-deficient F).
+    int nz = (s0 > tiny) + (s1 > tiny) + (s2 > tiny);
+    if (nz < 3) {
+        if (nz == 0) { u0 = mk3(1, 0, 0); u1 = mk3(0, 1, 0); u2 = mk3(0, 0, 1); }
+        else {
+            // move a valid column to u0 for the completion
+            f3 ref = s0 > tiny ? u0 : (s1 > tiny ? u1 : u2);
+            if (nz == 1) {
+                f3 o1 = any_orthogonal(ref), o2 = cross(ref, o1);
+                if (s0 > tiny) { u1 = o1; u2 = o2; }
+                else if (s1 > tiny) { u0 = o1; u2 = o2; }
+                else { u0 = o1; u1 = o2; }
+            } else {
+                if (!(s0 > tiny)) u0 = cross(u1, u2);
+                else if (!(s1 > tiny)) u1 = cross(u2, u0);
+                else u2 = cross(u0, u1);
+            }
+        }
+    }
+    U = m3{u0, u1, u2};
+    V = m3{w0, w1, w2};
+    s = mk3(s0, s1, s2);
+    if (det(V) < 0.0f) { V.c2 = V.c2 * -1.0f; U.c2 = U.c2 * -1.0f; }
+    if (det(U) < 0.0f) {
+        // flip the column of the smallest singular value
+        if (s.x <= s.y && s.x <= s.z) { U.c0 = U.c0 * -1.0f; s.x = -s.x; }
+        else if (s.y <= s.z) { U.c1 = U.c1 * -1.0f; s.y = -s.y; }
+        else { U.c2 = U.c2 * -1.0f; s.z = -s.z; }
+    }
+}
+
+// U diag(d) U^T (symmetric), used by the corotated stress with a known SVD.
+__device__ inline m3 u_diag_ut(const m3 &U, f3 d)
+{
+    m3 Ud = m3{U.c0 * d.x, U.c1 * d.y, U.c2 * d.z};
+    return mul(Ud, transpose(U));
+}
+
+__device__ inline m3 recompose(const m3 &U, f3 s, const m3 &V)
+{
+    m3 Us = m3{U.c0 * s.x, U.c1 * s.y, U.c2 * s.z};
+    return mul(Us, transpose(V));
+}
+
+// ------------------------------------------------------------------------------------
+// Constitutive models (src_mpm_shaders/models/*).
+
+// NeoHookeanModel::kirchoff_stress (neo_hookean_elasticity.rs:26-39)
+__device__ inline m3 nh_stress(float lambda, float mu, const m3 &F)
+{
+    float j = fmaxf(det(F), 1.0e-10f);
+    float dv = lambda * logf(j) - mu;
+    m3 st = mul(F, transpose(F)) * mu;
+    st.c0.x += dv;
+    st.c1.y += dv;
+    st.c2.z += dv;
+    return st;
+}
+
+// LinearElasticModel::kirchoff_stress (linear_elasticity.rs:30-54) given the SVD of F:
+// 2 mu U diag(s - 1) V^T F^T + lambda (J - 1) J I = 2 mu U diag((s - 1) s) U^T + ...
+__device__ inline m3 linear_stress_svd(float lambda, float mu, const m3 &U, f3 s)
+{
+    float j = s.x * s.y * s.z;
+    m3 st = u_diag_ut(U, mk3((s.x - 1.0f) * s.x, (s.y - 1.0f) * s.y, (s.z - 1.0f) * s.z)) * (2.0f * mu);
+    float dv = lambda * (j - 1.0f) * j;
+    st.c0.x += dv;
+    st.c1.y += dv;
+    st.c2.z += dv;
+    return st;
+}
+
+// DruckerPragerPlasticity::alpha (drucker_prager.rs:64-68); dp = {ha, hb, hc, hd, lambda, mu, cohesion}
+__device__ inline float dp_alpha(const float *dp, float q)
+{
+    float angle = dp[0] + (dp[1] * q - dp[3]) * expf(-dp[2] * q);
+    float sa = sinf(angle);
+    return sqrtf(2.0f / 3.0f) * (2.0f * sa) / (3.0f - sa);
+}
+
+// DruckerPragerPlasticity::project (drucker_prager.rs:224-267 with :121-167).  Updates
+// F and the plastic state {det Fp, q, log volume gain} in place; returns the SVD of
+// the projected F in (U, s, V) for the elastic backbone.
+__device__ inline void dp_project(const float *dp, float &pdet, float &q, float &lvg, m3 &F, m3 &U, f3 &s, m3 &V)
+{
+    svd3(F, U, s, V);
+    if (dp[4] == 0.0f) return;
+    float alpha = dp_alpha(dp, q);
+    const float d = 3.0f;
+    float lg = lvg / d;
+    f3 strain = mk3(logf(s.x) + lg, logf(s.y) + lg, logf(s.z) + lg);
+    float tr = strain.x + strain.y + strain.z;
+    float m = tr / d;
+    f3 dev = mk3(strain.x - m, strain.y - m, strain.z - m);
+    float shifted = tr - dp[6];
+    f3 ns;
+    float dq;
+    if (shifted > 0.0f || (dev.x == 0.0f && dev.y == 0.0f && dev.z == 0.0f)) {
+        float e = expf(dp[6] / d);
+        ns = mk3(e, e, e);
+        dq = length(strain);
+    } else {
+        float dn = length(dev);
+        float gamma = dn + (d * dp[4] + 2.0f * dp[5]) / (2.0f * dp[5]) * shifted * alpha;
+        if (gamma <= 0.0f) return;   // elastic: state and F unchanged
+        float k = gamma / dn;
+        ns = mk3(expf(strain.x - dev.x * k), expf(strain.y - dev.y * k), expf(strain.z - dev.z * k));
+        dq = gamma;
+    }
+    float prev_det = s.x * s.y * s.z;
+    float new_det = ns.x * ns.y * ns.z;
+    pdet = pdet * prev_det / new_det;
+    lvg = lvg + logf(prev_det) - logf(new_det);
+    q = q + dq;
+    s = ns;
+    F = recompose(U, s, V);
+}
+
+// FluidModel (fluid.rs:54-84)
+__device__ inline float fluid_pressure(const float *fl, float j)
+{
+    j = fmaxf(j, 1.0e-6f);
+    if (j <= 1.0f) return fl[0] * (expf(-fl[1] * logf(j)) - 1.0f);
+    return -fl[0] * fl[4] * (j - 1.0f);
+}
+
+__device__ inline m3 fluid_stress(const float *fl, const m3 &F, const m3 &gradv)
+{
+    float j = fmaxf(det(F), 1.0e-6f);
+    m3 sr = (gradv + transpose(gradv)) * 0.5f;
+    float sph = trace(sr) / 3.0f;
+    sr.c0.x -= sph;
+    sr.c1.y -= sph;
+    sr.c2.z -= sph;
+    m3 st = sr * (2.0f * fl[2] * j);
+    float dv = -fluid_pressure(fl, j) * j;
+    st.c0.x += dv;
+    st.c1.y += dv;
+    st.c2.z += dv;
+    return st;
+}
+
+// ------------------------------------------------------------------------------------
+// Cuboid collider boundary on grid nodes (solver/grid_update_collide.rs:30-183).
+
+// v (w^2 - b.b) + b (2 b.v) + (b x v)(2 w): rotation of v by the quaternion (b, w).
+__device__ inline f3 quat_rotate(float qx, float qy, float qz, float qw, f3 v)
+{
+    f3 b = mk3(qx, qy, qz);
+    float b2 = dot(b, b);
+    return v * (qw * qw - b2) + b * (dot(v, b) * 2.0f) + cross(b, v) * (qw * 2.0f);
+}
+
+// Closest point on the boundary of a posed cuboid: outside, the clamp to the box;
+// inside, the nearest face (first axis wins ties; the min face wins a same-axis tie),
+// as parry3d's Cuboid::project_point with solid = false
+// (src_rbd_shaders/shapes/shape.rs:378-384).
+__device__ inline f3 project_cuboid(const ColliderDev &c, f3 pt, bool &inside)
+{
+    f3 t = mk3(c.t[0], c.t[1], c.t[2]);
+    f3 local = quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], pt - t);
+    float lp[3] = {local.x, local.y, local.z};
+    float sh[3];
+    bool zero = true;
+    for (int i = 0; i < 3; ++i) {
+        float mins_pt = -c.he[i] - lp[i];
+        float pt_maxs = lp[i] - c.he[i];
+        sh[i] = fmaxf(mins_pt, 0.0f) - fmaxf(pt_maxs, 0.0f);
+        zero = zero && sh[i] == 0.0f;
+    }
+    if (!zero) {
+        inside = false;
+        for (int i = 0; i < 3; ++i) lp[i] += sh[i];
+    } else {
+        inside = true;
+        float best = -3.0e38f;
+        bool is_mins = false;
+        int best_id = 0;
+        for (int i = 0; i < 3; ++i) {
+            float a = -c.he[i] - lp[i], b = lp[i] - c.he[i];
+            if (a < b) {
+                if (b > best) { best_id = i; is_mins = false; best = b; }
+            } else if (a > best) {
+                best_id = i; is_mins = true; best = a;
+            }
+        }
+        for (int i = 0; i < 3; ++i)
+            if (i == best_id) lp[i] += is_mins ? best : -best;
+    }
+    return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], mk3(lp[0], lp[1], lp[2])) + t;
+}
+
+// BoundaryCondition::project_velocity (solver/boundary_condition.rs:77-114)
+__device__ inline f3 bc_project(uint32_t ty, float friction, f3 vel, f3 n)
+{
+    if (ty == 0u) return mk3(0, 0, 0);
+    if (ty == 1u) return vel - n * dot(vel, n);
+    if (ty == 2u) {
+        float nv = dot(vel, n);
+        if (nv < 0.0f) {
+            f3 tv = vel - n * nv;
+            float tl = length(tv);
+            f3 dir = tl > 1.0e-8f ? tv / tl : mk3(0, 0, 0);
+            float pl = tl + friction * nv;
+            return dir * (pl > 0.0f ? pl : 0.0f);
+        }
+        return vel;
+    }
+    return vel;
+}
+
+__device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, float h, float dt, f3 pt, f3 vel)
+{
+    float cap = h * 1.5f;
+    float best = MAX_FLT_REF;
+    f3 normal = mk3(0, 0, 0);
+    uint32_t best_id = 0;
+    for (uint32_t i = 0; i < ncol; ++i) {
+        bool inside;
+        f3 proj = project_cuboid(cols[i], pt, inside);
+        f3 dpt = proj - pt;
+        bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;
+        if (inside || within) {
+            float sign = inside ? -1.0f : 1.0f;
+            float dist = length(dpt);
+            f3 n = dpt / (dist * -sign);
+            float sd = sign * dist;
+            if (sd < best) { best = sd; normal = n; best_id = i; }
+        }
+    }
+    if (best != MAX_FLT_REF) {
+        const ColliderDev &c = cols[best_id];
+        f3 com = mk3(c.t[0], c.t[1], c.t[2]);
+        f3 bv = mk3(c.linvel[0], c.linvel[1], c.linvel[2]) + cross(mk3(c.angvel[0], c.angvel[1], c.angvel[2]), pt - com);
+        f3 delta = vel - bv;
+        float nv = dot(delta, normal);
+        if (best <= h) {
+            vel = bv + bc_project(c.bc_type, c.bc_friction, delta, normal);
+        } else if (-nv * dt > best - h) {
+            vel = vel - normal * (nv + (best - h) / dt);
+        }
+    }
+    return vel;
+}
+
+}  // namespace nxm

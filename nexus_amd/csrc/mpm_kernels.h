@@ -1,0 +1,50 @@
+// mpm_kernels.h — device view of a world (all scenes stepped together) and the
+// launch entry points of mpm_kernels.hip.
+#pragma once
+
+#include "mpm_device.h"
+
+namespace nxm {
+
+struct WorldDev {
+    Arena in, out;                 // particle arenas: read (previous order), written (sorted order)
+    uint32_t *pkey;                // per input slot: packed block key
+    uint32_t *pdense;              // per input slot: dense block id
+    uint32_t *prank;               // per input slot: rank inside its block
+    uint32_t *perm;                // per sorted position: input slot
+    uint32_t *sblock;              // per sorted position: dense block id
+    uint32_t *bscene;              // per block: scene slot (NONE = unused id)
+    int4 *bvid;                    // per block: virtual block coordinates
+    uint32_t *bnp;                 // per block: primary particles
+    uint32_t *bfirst;              // per block: first sorted position
+    uint32_t *bup;                 // per block: 8 ids of vid + {0,1}^3 (reference order)
+    uint32_t *bdn;                 // per block: 8 ids of vid - {0,1}^3
+    float *stage;                  // per block: P2G tile, 4 x 216 floats
+    float4 *gridv;                 // per node: velocity after the grid update, mass in w
+    uint32_t *hkeys, *hvals;       // hash regions; hvals: dense block id of each occupied slot
+    uint32_t *hmark;               // per hash slot: 1 if a particle's primary block
+    uint32_t *bprim;               // per block: 1 if primary (holds particles)
+    const uint32_t *region_start;  // per active scene, ascending: first hash slot of its region
+    const uint32_t *region_scene;  // per active scene: scene slot
+    uint32_t n_regions;
+    const SceneDev *scenes;
+    SceneCtr *ctr;
+    const ClassDev *classes;
+    const ColliderDev *cols;
+    Globals *g;
+    uint32_t *scan_tmp;
+    float *dbg_mom;                // optional per-node momentum after P2G (3 floats)
+    uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
+};
+
+struct LaunchConfig {
+    uint64_t p_hint;               // upper bound of particles (grid sizing only)
+    int persistent_blocks;         // cap on workgroups of grid-stride kernels
+    int p2g_blocks;                // workgroups of the P2G work queue
+};
+
+void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev);
+void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,
+                      hipStream_t st);
+
+}  // namespace nxm

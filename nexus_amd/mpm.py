@@ -1,0 +1,365 @@
+"""Python mirror of the nexus_mpm pipeline API over the C ABI (include/nexus_mpm.h).
+
+Names follow the reference's Rust API so that a caller of nexus_mpm finds the same
+objects and calls:
+
+  MpmPipeline(device)                      MpmPipeline::new (src_mpm/pipeline.rs:366)
+  MpmState.empty(pipeline, capacities)     MpmState::empty (:107)
+  state.set_simulation_params(g, dt)       MpmState::set_simulation_params (:169)
+  state.write_substep_params(n)            MpmState::write_substep_params (:183)
+  state.set_cell_width(h, capacity)        MpmState::set_cell_width (:200)
+  state.add_particles(...)                 GpuParticles::append (src_mpm/solver/particle.rs:533)
+  state.set_colliders(cuboids)             MpmState::set_coupling (:220), one-way cuboids
+  pipeline.step(state)                     MpmPipeline::step (:385)
+  state.read_positions()                   GpuParticles::read_positions (particle.rs:634)
+
+The library is the HIP build in nexus_amd/libnexus_mpm.so; there is no CPU
+fallback: every call fails loudly if the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import scenes as _scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnexus_mpm.so")
+
+NXM_OK = 0
+STATUS_NAMES = {
+    1: "NXM_ERR_INVALID_ARGUMENT", 2: "NXM_ERR_OUT_OF_MEMORY", 3: "NXM_ERR_GRID_CAPACITY", 4: "NXM_ERR_DOMAIN",
+    5: "NXM_ERR_TOO_MANY_COLLIDERS", 6: "NXM_ERR_UNSUPPORTED_MODEL", 7: "NXM_ERR_DEVICE", 8: "NXM_ERR_BUSY",
+}
+
+# Every symbol include/nexus_mpm.h declares.
+EXPORTS = [
+    "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
+    "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
+    "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
+    "nxm_scene_len", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
+    "nxm_pipeline_step", "nxm_pipeline_run",
+    "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
+    "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
+]
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+class _Caps(ct.Structure):
+    _fields_ = [("grid_size", ct.c_uint32), ("particles_capacity", ct.c_uint32)]
+
+
+class _SimParams(ct.Structure):
+    _fields_ = [("gravity", ct.c_float * 3), ("dt", ct.c_float)]
+
+
+class _SoA(ct.Structure):
+    _fields_ = [(name, ct.c_void_p) for name in (
+        "position", "velocity", "def_grad", "affine", "init_volume", "init_radius", "mass", "damping", "phase",
+        "boundary_friction", "group_id", "enabled", "fixed", "model_tag", "model_data")]
+
+
+class _StageTimes(ct.Structure):
+    _fields_ = [("sort_ms", ct.c_float), ("p2g_ms", ct.c_float), ("grid_update_ms", ct.c_float),
+                ("g2p_update_ms", ct.c_float), ("total_ms", ct.c_float)]
+
+
+class _Canonical(ct.Structure):
+    _fields_ = [(name, ct.c_void_p) for name in (
+        "block_vid", "block_np", "block_npe", "block_buckets", "particle_block", "particle_cell")]
+
+
+_LIB = None
+
+
+def library() -> ct.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m nexus_amd.build`")
+    L = ct.CDLL(LIB_PATH)
+    P, I = ct.c_void_p, ct.c_int
+    L.nxm_last_error.restype = ct.c_char_p
+    L.nxm_version.restype = ct.c_char_p
+    L.nxm_ctx_create.argtypes = [I, ct.POINTER(P)]
+    L.nxm_ctx_destroy.argtypes = [P]
+    L.nxm_scene_create.argtypes = [P, P, ct.POINTER(P)]
+    L.nxm_scene_destroy.argtypes = [P]
+    L.nxm_scene_set_simulation_params.argtypes = [P, P]
+    L.nxm_scene_write_substep_params.argtypes = [P, ct.c_uint32]
+    L.nxm_scene_set_cell_width.argtypes = [P, ct.c_float, ct.c_uint32]
+    L.nxm_scene_add_particles.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_add_particles_soa.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_set_colliders.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_len.argtypes = [P]
+    L.nxm_scene_len.restype = ct.c_size_t
+    L.nxm_scene_read_positions.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_read_particles_soa.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_sort_size.argtypes = [P, ct.POINTER(ct.c_size_t)]
+    L.nxm_scene_read_sort_canonical.argtypes = [P, P]
+    L.nxm_scene_read_grid.argtypes = [P, P, P, P]
+    L.nxm_scene_synchronize.argtypes = [P]
+    L.nxm_scene_set_debug_grid.argtypes = [P, I]
+    L.nxm_pipeline_step.argtypes = [P, P, P]
+    L.nxm_pipeline_run.argtypes = [P, P, ct.c_uint32]
+    L.nxm_batch_create.argtypes = [P, ct.POINTER(P)]
+    L.nxm_batch_destroy.argtypes = [P]
+    L.nxm_batch_add_scene.argtypes = [P, P]
+    L.nxm_batch_remove_scene.argtypes = [P, P]
+    L.nxm_batch_step.argtypes = [P, P]
+    L.nxm_batch_run.argtypes = [P, ct.c_uint32]
+    L.nxm_batch_synchronize.argtypes = [P]
+    L.nxm_batch_len.argtypes = [P]
+    L.nxm_batch_len.restype = ct.c_size_t
+    L.nxm_batch_set_graph.argtypes = [P, I]
+    _LIB = L
+    return L
+
+
+def _check(code: int) -> None:
+    if code != NXM_OK:
+        raise MpmError(code, library().nxm_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+@dataclass
+class StageTimes:
+    sort_ms: float
+    p2g_ms: float
+    grid_update_ms: float
+    g2p_update_ms: float
+    total_ms: float
+
+
+class MpmPipeline:
+    """The compiled pipeline on one device (MpmPipeline::new, src_mpm/pipeline.rs:366)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = library()
+        h = ct.c_void_p()
+        _check(self.lib.nxm_ctx_create(device, ct.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.nxm_ctx_destroy(self.handle)
+            self.handle = None
+
+    def step(self, state: "MpmState", timings: bool = False):
+        """MpmPipeline::step (src_mpm/pipeline.rs:385): enqueues one substep."""
+        if timings:
+            t = _StageTimes()
+            _check(self.lib.nxm_pipeline_step(self.handle, state.handle, ct.byref(t)))
+            return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms)
+        _check(self.lib.nxm_pipeline_step(self.handle, state.handle, None))
+        return None
+
+    def run(self, state: "MpmState", substeps: int) -> None:
+        _check(self.lib.nxm_pipeline_run(self.handle, state.handle, substeps))
+
+
+class MpmState:
+    """Device-resident MPM scene (MpmState, src_mpm/pipeline.rs:65)."""
+
+    def __init__(self, pipeline: MpmPipeline, grid_size: int = 32768, particles_capacity: int = 65536):
+        self.lib = pipeline.lib
+        self.pipeline = pipeline
+        caps = _Caps(grid_size, particles_capacity)
+        h = ct.c_void_p()
+        _check(self.lib.nxm_scene_create(pipeline.handle, ct.byref(caps), ct.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def empty(cls, pipeline: MpmPipeline, grid_size: int = 32768, particles_capacity: int = 65536) -> "MpmState":
+        return cls(pipeline, grid_size, particles_capacity)
+
+    @classmethod
+    def from_scene(cls, pipeline: MpmPipeline, scene: "_scenes.Scene", grid_size: int | None = None) -> "MpmState":
+        """Builds a device scene from a host Scene (positions, models, colliders, params)."""
+        cap = grid_size if grid_size is not None else default_grid_capacity(scene)
+        st = cls(pipeline, cap, scene.n)
+        st.set_cell_width(scene.cell_width, cap)
+        st.set_simulation_params(scene.gravity, scene.dt)
+        if scene.colliders.size:
+            st.set_colliders(scene.colliders)
+        st.add_particles(scene)
+        return st
+
+    def close(self):
+        if self.handle:
+            self.lib.nxm_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(self.lib.nxm_scene_len(self.handle))
+
+    def set_simulation_params(self, gravity, dt: float) -> None:
+        p = _SimParams((ct.c_float * 3)(*gravity), dt)
+        _check(self.lib.nxm_scene_set_simulation_params(self.handle, ct.byref(p)))
+
+    def write_substep_params(self, num_substeps: int) -> None:
+        _check(self.lib.nxm_scene_write_substep_params(self.handle, num_substeps))
+
+    def set_cell_width(self, cell_width: float, grid_capacity: int) -> None:
+        _check(self.lib.nxm_scene_set_cell_width(self.handle, cell_width, grid_capacity))
+
+    def set_colliders(self, cuboids: np.ndarray) -> None:
+        c = np.ascontiguousarray(cuboids, dtype=_scenes.CUBOID_DTYPE)
+        _check(self.lib.nxm_scene_set_colliders(self.handle, _ptr(c) if c.size else None, c.size))
+
+    def set_debug_grid(self, enabled: bool = True) -> None:
+        _check(self.lib.nxm_scene_set_debug_grid(self.handle, int(enabled)))
+
+    def add_particles(self, scene: "_scenes.Scene") -> None:
+        arrs = dict(
+            position=np.ascontiguousarray(scene.x, np.float32), velocity=np.ascontiguousarray(scene.v, np.float32),
+            def_grad=np.ascontiguousarray(scene.F, np.float32), affine=np.ascontiguousarray(scene.C, np.float32),
+            init_volume=np.ascontiguousarray(scene.vol0, np.float32),
+            init_radius=np.ascontiguousarray(scene.radius, np.float32),
+            mass=np.ascontiguousarray(scene.mass, np.float32), damping=np.ascontiguousarray(scene.damping, np.float32),
+            enabled=np.ascontiguousarray(scene.enabled, np.uint32), fixed=np.ascontiguousarray(scene.fixed, np.uint32),
+            model_tag=np.ascontiguousarray(scene.tag, np.uint32),
+            model_data=np.ascontiguousarray(scene.model, np.uint32),
+        )
+        soa = _SoA(**{k: _ptr(v) for k, v in arrs.items()})
+        _check(self.lib.nxm_scene_add_particles_soa(self.handle, ct.byref(soa), scene.n))
+
+    def synchronize(self) -> None:
+        _check(self.lib.nxm_scene_synchronize(self.handle))
+
+    def read_positions(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros((n, 3), np.float32)
+        _check(self.lib.nxm_scene_read_positions(self.handle, _ptr(out), n))
+        return out
+
+    def read_particles(self) -> dict:
+        n = len(self)
+        out = dict(
+            x=np.zeros((n, 3), np.float32), v=np.zeros((n, 3), np.float32), F=np.zeros((n, 9), np.float32),
+            C=np.zeros((n, 9), np.float32), vol0=np.zeros(n, np.float32), radius=np.zeros(n, np.float32),
+            mass=np.zeros(n, np.float32), damping=np.zeros(n, np.float32), phase=np.zeros(n, np.float32),
+            boundary_friction=np.zeros(n, np.float32), group=np.zeros(n, np.uint32),
+            enabled=np.zeros(n, np.uint32), fixed=np.zeros(n, np.uint32), tag=np.zeros(n, np.uint32),
+            model=np.zeros((n, 24), np.uint32),
+        )
+        soa = _SoA(position=_ptr(out["x"]), velocity=_ptr(out["v"]), def_grad=_ptr(out["F"]), affine=_ptr(out["C"]),
+                   init_volume=_ptr(out["vol0"]), init_radius=_ptr(out["radius"]), mass=_ptr(out["mass"]),
+                   damping=_ptr(out["damping"]), phase=_ptr(out["phase"]),
+                   boundary_friction=_ptr(out["boundary_friction"]), group_id=_ptr(out["group"]),
+                   enabled=_ptr(out["enabled"]), fixed=_ptr(out["fixed"]), model_tag=_ptr(out["tag"]),
+                   model_data=_ptr(out["model"]))
+        _check(self.lib.nxm_scene_read_particles_soa(self.handle, ct.byref(soa), n))
+        return out
+
+    def sort_size(self) -> int:
+        nb = ct.c_size_t(0)
+        _check(self.lib.nxm_scene_sort_size(self.handle, ct.byref(nb)))
+        return int(nb.value)
+
+    def read_sort_canonical(self) -> dict:
+        nb = self.sort_size()
+        n = len(self)
+        out = dict(vid=np.zeros((nb, 3), np.int32), np=np.zeros(nb, np.uint32), npe=np.zeros(nb, np.uint32),
+                   buckets=np.zeros((nb, 10), np.uint32), particle_block=np.zeros(n, np.uint32),
+                   particle_cell=np.zeros(n, np.uint32))
+        c = _Canonical(block_vid=_ptr(out["vid"]), block_np=_ptr(out["np"]), block_npe=_ptr(out["npe"]),
+                       block_buckets=_ptr(out["buckets"]), particle_block=_ptr(out["particle_block"]),
+                       particle_cell=_ptr(out["particle_cell"]))
+        _check(self.lib.nxm_scene_read_sort_canonical(self.handle, ct.byref(c)))
+        return out
+
+    def read_grid(self, momentum: bool = False) -> dict:
+        nb = self.sort_size()
+        out = dict(m=np.zeros((nb, 64), np.float32), vel=np.zeros((nb, 64, 3), np.float32))
+        mom = np.zeros((nb, 64, 3), np.float32) if momentum else None
+        _check(self.lib.nxm_scene_read_grid(self.handle, _ptr(out["m"]), _ptr(mom), _ptr(out["vel"])))
+        if momentum:
+            out["mv"] = mom
+        return out
+
+
+class Batch:
+    """Many independent scenes stepped together, one launch per stage."""
+
+    def __init__(self, pipeline: MpmPipeline):
+        self.lib = pipeline.lib
+        h = ct.c_void_p()
+        _check(self.lib.nxm_batch_create(pipeline.handle, ct.byref(h)))
+        self.handle = h
+        self.members: list[MpmState] = []
+
+    def close(self):
+        if self.handle:
+            self.lib.nxm_batch_destroy(self.handle)
+            self.handle = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return int(self.lib.nxm_batch_len(self.handle))
+
+    def add(self, state: MpmState) -> None:
+        _check(self.lib.nxm_batch_add_scene(self.handle, state.handle))
+        self.members.append(state)
+
+    def remove(self, state: MpmState) -> None:
+        _check(self.lib.nxm_batch_remove_scene(self.handle, state.handle))
+        self.members.remove(state)
+
+    def step(self, timings: bool = False):
+        if timings:
+            t = _StageTimes()
+            _check(self.lib.nxm_batch_step(self.handle, ct.byref(t)))
+            return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms)
+        _check(self.lib.nxm_batch_step(self.handle, None))
+        return None
+
+    def run(self, substeps: int) -> None:
+        _check(self.lib.nxm_batch_run(self.handle, substeps))
+
+    def synchronize(self) -> None:
+        _check(self.lib.nxm_batch_synchronize(self.handle))
+
+    def set_graph(self, enabled: bool) -> None:
+        _check(self.lib.nxm_batch_set_graph(self.handle, int(enabled)))
+
+
+def default_grid_capacity(scene: "_scenes.Scene") -> int:
+    """A block capacity covering the scene's occupied blocks with room for motion: the
+    blocks of the particle bounding box (plus a one-block halo), times two."""
+    if scene.n == 0:
+        return 1024
+    lo = np.floor(scene.x.min(0) / scene.cell_width / 4) - 2
+    hi = np.floor(scene.x.max(0) / scene.cell_width / 4) + 2
+    vol = float(np.prod(hi - lo + 1))
+    est = min(vol, 8.0 * scene.n + 64)
+    cap = 1024
+    while cap < 2 * est:
+        cap *= 2
+    return min(cap, 1 << 26)

@@ -1,0 +1,337 @@
+"""Seeded scene generators for the MLS-MPM workloads.
+
+A scene is a ``Scene``: structure-of-arrays particle data in the layout the C ABI
+uploads (``include/nexus_mpm.h``), the simulation parameters and the colliders.
+
+Particle construction follows the reference's ``Particle::new`` /
+``ParticleDynamics::new`` (src_mpm/solver/particle.rs:37-43, 102-122): initial
+volume (2r)^3, mass = density * volume, F = I, affine = 0, enabled, not fixed.
+Model words follow ``From<ParticleModel> for GpuParticleModel``
+(src_mpm/solver/particle_model.rs:282-325): a tag plus 24 u32 words.
+
+Workloads (seeds are 0x4E584D00 + workload number; per-scene streams derive from
+the workload seed and the scene index):
+
+  1  neo-Hookean cube, 32^3 jittered lattice, h = 1/64, six stick walls, dt = 1e-4
+  2  Drucker-Prager sand column, 1,048,576 particles, h = 1/256, 16 cuboids, dt = 5e-5
+  3  weakly-compressible fluid, 8,388,608 particles in 64 Gaussian clumps, h = 1/512, dt = 2e-5
+  4  256 independent scenes, 2k-200k particles each, mixed materials, h in [1/128, 1/64]
+  5  arrivals of scenes as in 4 with 1k-500k particles and 50-500 substep horizons
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MODEL_ELASTIC_LINEAR = 0
+MODEL_ELASTIC_NEO_HOOKEAN = 1
+MODEL_SAND_LINEAR = 2
+MODEL_SAND_NEO_HOOKEAN = 3
+MODEL_FLUID = 4
+MODEL_SNOW = 5
+
+BC_STICK = 0
+BC_SLIP = 1
+BC_SEPARATE = 2
+BC_NON_REFLECTING = 3
+
+SEED_BASE = 0x4E584D00
+
+# nxm_cuboid layout: half_extents[3], rotation[4] (x, y, z, w), translation[3],
+# linvel[3], angvel[3], bc_type (u32), bc_friction (f32).
+CUBOID_DTYPE = np.dtype(
+    [
+        ("half_extents", "<f4", 3),
+        ("rotation", "<f4", 4),
+        ("translation", "<f4", 3),
+        ("linvel", "<f4", 3),
+        ("angvel", "<f4", 3),
+        ("bc_type", "<u4"),
+        ("bc_friction", "<f4"),
+    ]
+)
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def lame_lambda_mu(young: float, poisson: float):
+    """Lame parameters in f32 arithmetic (src_mpm/models/mod.rs:19-30)."""
+    e, nu = f32(young), f32(poisson)
+    lam = e * nu / ((f32(1) + nu) * (f32(1) - f32(2) * nu))
+    mu = e / (f32(2) * (f32(1) + nu))
+    return f32(lam), f32(mu)
+
+
+def _words(values) -> np.ndarray:
+    w = np.zeros(24, dtype=np.uint32)
+    arr = np.asarray(values, dtype=np.float32)
+    w[: arr.size] = arr.view(np.uint32)
+    return w
+
+
+def model_neo_hookean(young: float, poisson: float):
+    """ParticleModel::elastic_neo_hookean (src_mpm/solver/particle_model.rs:59-64)."""
+    lam, mu = lame_lambda_mu(young, poisson)
+    return MODEL_ELASTIC_NEO_HOOKEAN, _words([lam, mu, 0.5])
+
+
+def model_linear(young: float, poisson: float):
+    """ParticleModel::elastic (src_mpm/solver/particle_model.rs:52-57)."""
+    lam, mu = lame_lambda_mu(young, poisson)
+    return MODEL_ELASTIC_LINEAR, _words([lam, mu, 0.5])
+
+
+def model_sand(young: float, poisson: float, friction_deg: float | None = None, *, neo_hookean=False):
+    """Drucker-Prager sand (src_mpm/solver/particle_model.rs:67-77, 143-153).
+
+    With ``friction_deg`` the hardening law is flattened to a constant angle
+    (ha = angle, hb = hd = 0), which the reference allows because the
+    DruckerPragerPlasticity fields are public; without it the reference's default
+    hardening (35, 9, 0.2, 10 degrees; src_mpm/models/drucker_prager.rs:46-56) is used.
+    """
+    lam, mu = lame_lambda_mu(young, poisson)
+    if friction_deg is None:
+        ha, hb, hc, hd = (np.float32(np.deg2rad(35.0)), np.float32(np.deg2rad(9.0)),
+                          f32(0.2), np.float32(np.deg2rad(10.0)))
+    else:
+        ha, hb, hc, hd = np.float32(np.deg2rad(friction_deg)), f32(0), f32(0.2), f32(0)
+    tag = MODEL_SAND_NEO_HOOKEAN if neo_hookean else MODEL_SAND_LINEAR
+    # plastic state (det Fp, hardening q, log volume gain) = (1, 1, 0)
+    return tag, _words([1.0, 1.0, 0.0, ha, hb, hc, hd, lam, mu, 0.0, lam, mu, 0.5])
+
+
+def model_fluid(bulk: float, gamma: float = 7.0, viscosity: float = 0.001, tensile: float = 0.25):
+    """ParticleModel::fluid / water (src_mpm/solver/particle_model.rs:161-178)."""
+    return MODEL_FLUID, _words([bulk, gamma, viscosity, 0.5, tensile])
+
+
+@dataclass
+class Scene:
+    """Host copy of one MPM scene (structure of arrays, float32/uint32)."""
+
+    x: np.ndarray            # (n, 3) f32 positions
+    v: np.ndarray            # (n, 3) f32 velocities
+    F: np.ndarray            # (n, 9) f32 deformation gradient, column-major
+    C: np.ndarray            # (n, 9) f32 APIC affine matrix, column-major
+    mass: np.ndarray         # (n,) f32
+    vol0: np.ndarray         # (n,) f32 initial volume
+    radius: np.ndarray       # (n,) f32 initial radius
+    damping: np.ndarray      # (n,) f32
+    enabled: np.ndarray      # (n,) u32
+    fixed: np.ndarray        # (n,) u32
+    tag: np.ndarray          # (n,) u32 model tag
+    model: np.ndarray        # (n, 24) u32 model words
+    cell_width: float
+    dt: float
+    gravity: tuple = (0.0, -9.81, 0.0)
+    colliders: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=CUBOID_DTYPE))
+    substeps: int = 1
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    def copy(self) -> "Scene":
+        out = Scene(**{k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+                       for k, v in self.__dict__.items()})
+        return out
+
+
+def particles(positions, radius, density, model, *, velocity=None) -> dict:
+    """Arrays for particles built like Particle::new (src_mpm/solver/particle.rs:37-43)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    r = np.broadcast_to(np.asarray(radius, dtype=np.float32), (n,)).copy()
+    vol = (r * f32(2.0)) ** 3
+    vol = vol.astype(np.float32)
+    mass = (vol * np.float32(density)).astype(np.float32)
+    eye = np.zeros((n, 9), dtype=np.float32)
+    eye[:, 0] = eye[:, 4] = eye[:, 8] = 1.0
+    tag, words = model
+    vel = np.zeros((n, 3), np.float32) if velocity is None else np.ascontiguousarray(velocity, np.float32).reshape(n, 3)
+    return dict(
+        x=pos, v=vel, F=eye, C=np.zeros((n, 9), np.float32), mass=mass, vol0=vol, radius=r,
+        damping=np.zeros(n, np.float32), enabled=np.ones(n, np.uint32), fixed=np.zeros(n, np.uint32),
+        tag=np.full(n, tag, np.uint32), model=np.tile(words, (n, 1)),
+    )
+
+
+def concat(parts: list[dict]) -> dict:
+    return {k: np.concatenate([p[k] for p in parts], axis=0) for k in parts[0]}
+
+
+def cuboid(center, half_extents, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0,
+           linvel=(0, 0, 0), angvel=(0, 0, 0)):
+    c = np.zeros(1, dtype=CUBOID_DTYPE)
+    c["half_extents"] = half_extents
+    c["rotation"] = rotation
+    c["translation"] = center
+    c["linvel"] = linvel
+    c["angvel"] = angvel
+    c["bc_type"] = bc
+    c["bc_friction"] = friction
+    return c
+
+
+def box_walls(h: float, bc=BC_STICK, friction=0.0, lo=0.0, hi=1.0) -> np.ndarray:
+    """Six cuboid slabs of thickness 4h enclosing [lo, hi]^3 (the reference has no
+    built-in walls: its examples build them from fixed cuboids, e.g.
+    crates/examples3d/mpm_sand3.rs:56-72)."""
+    t = 2.0 * h
+    mid = 0.5 * (lo + hi)
+    ext = 0.5 * (hi - lo) + 2.0 * t
+    walls = []
+    for axis in range(3):
+        for side in (lo - t, hi + t):
+            c = [mid, mid, mid]
+            c[axis] = side
+            he = [ext, ext, ext]
+            he[axis] = t
+            walls.append(cuboid(c, he, bc=bc, friction=friction))
+    return np.concatenate(walls)
+
+
+def random_unit_quaternion(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    return q.astype(np.float32)
+
+
+def make_scene(parts: dict, cell_width, dt, *, colliders=None, gravity=(0.0, -9.81, 0.0), substeps=1, name=""):
+    cols = np.zeros(0, dtype=CUBOID_DTYPE) if colliders is None else colliders
+    return Scene(cell_width=float(np.float32(cell_width)), dt=float(np.float32(dt)), gravity=gravity,
+                 colliders=cols, substeps=substeps, name=name, **parts)
+
+
+# --------------------------------------------------------------------------- #
+# Workloads
+
+
+def elastic_cube(n_side: int = 32, seed: int = SEED_BASE + 1, *, cell_width=1.0 / 64, side=0.25,
+                 center=(0.5, 0.6, 0.5)) -> Scene:
+    """Workload 1: neo-Hookean cube (E = 1e5, nu = 0.3, density 1000) on an n_side^3
+    lattice with +-spacing/4 uniform jitter, six stick walls, dt = 1e-4, 100 substeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = side / n_side
+    idx = np.stack(np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3).astype(np.float64)
+    jitter = rng.uniform(-0.25 * s, 0.25 * s, size=idx.shape)
+    pos = np.asarray(center) - side / 2 + (idx + 0.5) * s + jitter
+    parts = particles(pos, s / 2, 1000.0, model_neo_hookean(1e5, 0.3))
+    return make_scene(parts, cell_width, 1e-4, colliders=box_walls(cell_width), substeps=100,
+                      name=f"neo-Hookean cube {n_side ** 3}")
+
+
+def sand_column(n: int = 1_048_576, seed: int = SEED_BASE + 2, *, cell_width=1.0 / 256,
+                n_colliders: int = 16, lo=(0.35, 0.05, 0.35), hi=(0.65, 0.65, 0.65)) -> Scene:
+    """Workload 2: Drucker-Prager sand (friction 30 degrees, E = 3.5e5, nu = 0.3,
+    density 2200), uniform random positions in a 0.3 x 0.6 x 0.3 m box, 16 fixed
+    cuboids with random poses and half-extents 0.02-0.1 m (separate, friction 1),
+    h = 1/256, dt = 5e-5, 1000 substeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = np.asarray(lo), np.asarray(hi)
+    pos = lo + rng.random((n, 3)) * (hi - lo)
+    vol = float(np.prod(hi - lo))
+    r = 0.5 * (vol / n) ** (1.0 / 3.0)
+    parts = particles(pos, r, 2200.0, model_sand(3.5e5, 0.3, friction_deg=30.0))
+    cols = []
+    for _ in range(n_colliders):
+        q = random_unit_quaternion(rng)
+        c = rng.random(3)
+        he = rng.uniform(0.02, 0.1, size=3)
+        cols.append(cuboid(c, he, rotation=q, bc=BC_SEPARATE, friction=1.0))
+    colliders = np.concatenate(cols) if cols else None
+    return make_scene(parts, cell_width, 5e-5, colliders=colliders, substeps=1000,
+                      name=f"Drucker-Prager sand column {n}")
+
+
+def fluid_clumps(n: int = 8_388_608, seed: int = SEED_BASE + 3, *, n_clumps: int = 64,
+                 cell_width=1.0 / 512, sigma=(0.01, 0.05)) -> Scene:
+    """Workload 3: weakly-compressible fluid (K = 1e4, gamma = 7, density 1000),
+    positions from 64 isotropic Gaussian clumps (sigma 0.01-0.05 m, centres in
+    [0.1, 0.9]^3, samples outside the unit box redrawn), h = 1/512, r = h/4,
+    dt = 2e-5, 200 substeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    per = n // n_clumps
+    out = []
+    for k in range(n_clumps):
+        m = per if k < n_clumps - 1 else n - per * (n_clumps - 1)
+        c = rng.uniform(0.1, 0.9, size=3)
+        s = rng.uniform(sigma[0], sigma[1])
+        pts = np.empty((0, 3))
+        while pts.shape[0] < m:
+            cand = c + rng.normal(scale=s, size=(m - pts.shape[0], 3))
+            ok = np.all((cand > 0.0) & (cand < 1.0), axis=1)
+            pts = np.concatenate([pts, cand[ok]])
+        out.append(pts[:m])
+    pos = np.concatenate(out)
+    parts = particles(pos, cell_width / 4, 1000.0, model_fluid(1e4))
+    return make_scene(parts, cell_width, 2e-5, substeps=200, name=f"fluid clumps {n}")
+
+
+def material_box(rng, n: int, material: int, cell_width: float, dt: float = 1e-4) -> Scene:
+    """A jittered box of ``n`` particles (2 per cell per axis) at a random position
+    in the unit domain, for the batched and served workloads.  material: 0 neo-Hookean
+    (C1 parameters), 1 Drucker-Prager (C2 parameters), 2 fluid (C3 parameters)."""
+    s = cell_width / 2
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    idx = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3)[:n].astype(np.float64)
+    extent = side * s
+    origin = rng.uniform(0.05, max(0.05, 0.95 - extent), size=3)
+    pos = origin + (idx + 0.5) * s + rng.uniform(-0.25 * s, 0.25 * s, size=idx.shape)
+    if material == 0:
+        parts = particles(pos, s / 2, 1000.0, model_neo_hookean(1e5, 0.3))
+    elif material == 1:
+        parts = particles(pos, s / 2, 2200.0, model_sand(3.5e5, 0.3, friction_deg=30.0))
+    else:
+        parts = particles(pos, s / 2, 1000.0, model_fluid(1e4))
+    return make_scene(parts, cell_width, dt, name=f"box {n} mat {material}")
+
+
+def batch_scenes(count: int = 256, seed: int = SEED_BASE + 4, *, n_range=(2_000, 200_000),
+                 substeps: int = 500) -> list[Scene]:
+    """Workload 4: ``count`` independent scenes; N log-uniform in n_range, material
+    uniform over {neo-Hookean, Drucker-Prager, fluid}, h = 1/k with k uniform in
+    [64, 128], dt = 1e-4."""
+    out = []
+    for i in range(count):
+        rng = np.random.Generator(np.random.PCG64([seed, i]))
+        n = int(math.floor(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        mat = int(rng.integers(0, 3))
+        h = 1.0 / int(rng.integers(64, 129))
+        sc = material_box(rng, n, mat, h)
+        sc.substeps = substeps
+        sc.name = f"scene {i}: {n} particles, material {mat}, h=1/{round(1 / h)}"
+        out.append(sc)
+    return out
+
+
+def served_requests(duration_s: float, seed: int = SEED_BASE + 5, *, rate=200.0, n_range=(1_000, 500_000),
+                    horizon=(50, 500)):
+    """Workload 5: Poisson arrivals (rate per second) of scenes as in workload 4 with
+    N log-uniform in n_range and step horizons uniform in horizon.  Yields
+    (arrival_time_s, Scene) in arrival order."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    i = 0
+    out = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        srng = np.random.Generator(np.random.PCG64([seed, i]))
+        n = int(math.floor(math.exp(srng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        mat = int(srng.integers(0, 3))
+        h = 1.0 / int(srng.integers(64, 129))
+        sc = material_box(srng, n, mat, h)
+        sc.substeps = int(srng.integers(horizon[0], horizon[1] + 1))
+        sc.name = f"request {i}"
+        out.append((t, sc))
+        i += 1
+    return out

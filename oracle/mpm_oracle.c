@@ -1,0 +1,1012 @@
+/*
+ * mpm_oracle.c — TEST INFRASTRUCTURE ONLY (interface in mpm_oracle.h).
+ *
+ * A CPU restatement of dimforge/nexus's non-CPIC MLS-MPM substep, written from
+ * the reference's shader sources.  It exists to check the HIP library; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline may load it.
+ * Citations are file:line in the dimforge/nexus tree.
+ *
+ * Built twice from this one file (oracle/Makefile):
+ *   REAL=float   FP32 mirror: same operation order as the reference shaders,
+ *                -ffp-contract=off, IEEE division, roundf (half away from zero).
+ *   REAL=double  FP64 oracle: identical algorithm in double precision.
+ *
+ * Pass order of one substep (src_mpm/pipeline.rs:385-526, use_cpic = false):
+ *   block activation (src_mpm_shaders/grid/grid.rs:447-578, grid/sort.rs:104-163)
+ *   -> neighbour ids (grid/sort.rs:261-279) -> bucket counts (grid/sort.rs:287-349)
+ *   -> exclusive scan + cursors (grid/sort.rs:357-420) -> scatter (grid/sort.rs:430-491)
+ *   -> P2G (solver/p2g.rs:64-351) -> grid update (solver/grid_update.rs:21-88)
+ *   -> collider boundary (solver/grid_update_collide.rs:30-183)
+ *   -> G2P (solver/g2p.rs:42-347) -> particle update (solver/particle_update.rs:60-201).
+ * The reference's atomics become a sequential emulation: header ids in first-touch
+ * order, bucket lists in particle-id order.  That is one of the orders the
+ * reference can produce; its canonical form (active block set, per-block bucket
+ * counts, per-(block, bucket) id sets) is order independent.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   block keys, buckets, spills   brute-force integer tables incl. x/h = k + 0.5 ties
+ *   quadratic kernel              partition of unity, first and second moments
+ *   P2G / G2P                     mass and momentum conservation, affine-field reproduction
+ *   SVD                           numpy.linalg.svd (LAPACK) on random and degenerate F
+ *   cuboid projection             brute-force closest point on the box surface
+ *   NH / fluid / DP               closed forms (DP return map on singular values)
+ *   whole substep                 free-fall closed form of the elastic cube
+ * Parity unpinned (the reference cannot be built here and holds no vectors for this
+ * path): the exact FP32 bits of glamx Mat3::svd (third-party, glamx 0.3), parry3d 0.30's
+ * Cuboid::project_point tie-breaking between equidistant faces, and the precision of
+ * the reference GPU's ln/exp/sin.  This file restates parry's published Aabb
+ * projection (clamp outside; nearest face inside, first axis wins ties, min face wins
+ * same-axis ties) and uses its own one-sided Jacobi SVD.
+ */
+#include "mpm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#if defined(ORACLE_F64)
+#define RSQRT sqrt
+#define RLOG log
+#define REXP exp
+#define RSIN sin
+#define RFABS fabs
+#define RROUND round
+#define RFLOOR floor
+#define RFMAX fmax
+#define RFMIN fmin
+#define JACOBI_EPS 1e-15
+#else
+#define RSQRT sqrtf
+#define RLOG logf
+#define REXP expf
+#define RSIN sinf
+#define RFABS fabsf
+#define RROUND roundf
+#define RFLOOR floorf
+#define RFMAX fmaxf
+#define RFMIN fminf
+#define JACOBI_EPS 1e-7f
+#endif
+
+#define K(c) ((REAL)(c))
+#define NONE 0xFFFFFFFFu
+#define MAX_FLT_REF ((REAL)1.0e18f) /* src_rbd_shaders/lib.rs:283 */
+
+/* ------------------------------------------------------------------------- */
+/* Small vector / matrix algebra with the reference's (glam's) operation order. */
+
+typedef struct { REAL x, y, z; } v3;
+typedef struct { v3 c0, c1, c2; } m3; /* columns */
+
+static inline v3 V3(REAL x, REAL y, REAL z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, REAL s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 vdiv(v3 a, REAL s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline REAL vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b)
+{
+    return V3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline REAL vlen(v3 a) { return RSQRT(vdot(a, a)); }
+static inline REAL vget(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline void vset(v3 *a, int i, REAL s)
+{
+    if (i == 0) a->x = s; else if (i == 1) a->y = s; else a->z = s;
+}
+
+static inline m3 M3(v3 c0, v3 c1, v3 c2) { m3 r; r.c0 = c0; r.c1 = c1; r.c2 = c2; return r; }
+static inline m3 mzero(void) { return M3(V3(0, 0, 0), V3(0, 0, 0), V3(0, 0, 0)); }
+static inline m3 mident(void) { return M3(V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)); }
+static inline m3 mdiag(v3 d) { return M3(V3(d.x, 0, 0), V3(0, d.y, 0), V3(0, 0, d.z)); }
+static inline v3 mcol(m3 m, int j) { return j == 0 ? m.c0 : (j == 1 ? m.c1 : m.c2); }
+/* glam Mat3 * Vec3: (c0*v.x + c1*v.y) + c2*v.z */
+static inline v3 mvmul(m3 m, v3 v)
+{
+    return vadd(vadd(vmul(m.c0, v.x), vmul(m.c1, v.y)), vmul(m.c2, v.z));
+}
+static inline m3 mmul(m3 a, m3 b) { return M3(mvmul(a, b.c0), mvmul(a, b.c1), mvmul(a, b.c2)); }
+static inline m3 madd(m3 a, m3 b) { return M3(vadd(a.c0, b.c0), vadd(a.c1, b.c1), vadd(a.c2, b.c2)); }
+static inline m3 msub(m3 a, m3 b) { return M3(vsub(a.c0, b.c0), vsub(a.c1, b.c1), vsub(a.c2, b.c2)); }
+static inline m3 mscale(m3 a, REAL s) { return M3(vmul(a.c0, s), vmul(a.c1, s), vmul(a.c2, s)); }
+static inline m3 mtrans(m3 a)
+{
+    return M3(V3(a.c0.x, a.c1.x, a.c2.x), V3(a.c0.y, a.c1.y, a.c2.y), V3(a.c0.z, a.c1.z, a.c2.z));
+}
+/* glam Mat3::determinant: z_axis . (x_axis x y_axis) */
+static inline REAL mdet(m3 a) { return vdot(a.c2, vcross(a.c0, a.c1)); }
+static inline REAL mtrace(m3 a) { return a.c0.x + a.c1.y + a.c2.z; }
+/* outer_product(a, b) = from_cols(a*b.x, a*b.y, a*b.z)   (solver/g2p.rs:378-381) */
+static inline m3 mouter(v3 a, v3 b) { return M3(vmul(a, b.x), vmul(a, b.y), vmul(a, b.z)); }
+static inline m3 mload(const REAL *p)
+{
+    return M3(V3(p[0], p[1], p[2]), V3(p[3], p[4], p[5]), V3(p[6], p[7], p[8]));
+}
+static inline void mstore(REAL *p, m3 a)
+{
+    p[0] = a.c0.x; p[1] = a.c0.y; p[2] = a.c0.z;
+    p[3] = a.c1.x; p[4] = a.c1.y; p[5] = a.c1.z;
+    p[6] = a.c2.x; p[7] = a.c2.y; p[8] = a.c2.z;
+}
+static inline v3 vload(const REAL *p) { return V3(p[0], p[1], p[2]); }
+static inline void vstore(REAL *p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+static inline REAL rclamp(REAL x, REAL lo, REAL hi)
+{
+    /* Rust f32::clamp: NaN propagates. */
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+static inline REAL wordf(const float *params, int i) { return (REAL)params[i]; }
+
+/* ------------------------------------------------------------------------- */
+/* Keys, blocks and buckets. */
+
+/* Position::associated_cell_index_in_block_off_by_one + block_associated_to_point
+ * (solver/particle.rs:208-225, grid/grid.rs:122-134). */
+static void particle_block(v3 pt, REAL h, int32_t vid[3], uint32_t local[3])
+{
+    REAL p[3] = {pt.x, pt.y, pt.z};
+    for (int d = 0; d < 3; ++d) {
+        REAL cell = RROUND(p[d] / h) - K(1.0);
+        REAL blk = RFLOOR(cell / K(4.0));
+        vid[d] = (int32_t)blk;
+        local[d] = (uint32_t)(cell - blk * K(4.0));
+    }
+}
+
+/* BlockVirtualId::pack (grid/grid.rs:102-106): 11/10/11 bits. */
+static uint32_t pack_vid(const int32_t v[3])
+{
+    return ((uint32_t)(v[0] + 0x3FF) & 0x7FFu) | (((uint32_t)(v[1] + 0x1FF) & 0x3FFu) << 11) |
+           (((uint32_t)(v[2] + 0x3FF) & 0x7FFu) << 21);
+}
+
+/* hash (grid/grid.rs:430-436) */
+static uint32_t hash_key(uint32_t k)
+{
+    k *= 0xCC9E2D51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1B873593u;
+    return k;
+}
+
+/* blocks_associated_to_block order (grid/grid.rs:175-212). */
+static const int ASSOC_SHIFT[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                                      {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+/* NBH_SHIFTS (grid/kernel.rs:42-70): the G2P summation order. */
+static const int NBH_SHIFTS[27][3] = {
+    {2, 2, 2}, {2, 0, 2}, {2, 1, 2}, {0, 2, 2}, {0, 0, 2}, {0, 1, 2}, {1, 2, 2}, {1, 0, 2}, {1, 1, 2},
+    {2, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {1, 0, 0}, {1, 1, 0},
+    {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
+
+#define NUM_BUCKETS 10
+#define NUM_PRIMARY_BUCKETS 4
+#define EXTRA_MIN_SHIFT 2u /* grid/sort.rs:35 */
+
+/* extra_sort_bucket (grid/sort.rs:56-62) */
+static int extra_bucket(const uint32_t local[3], const int bshift[3])
+{
+    int l = (int)local[2] - bshift[2] * 4;
+    if (l < -2) l = -2;
+    return NUM_PRIMARY_BUCKETS + (l + 2);
+}
+
+/* spill rule (grid/sort.rs:316-328) */
+static int spills(const uint32_t local[3], const int bshift[3])
+{
+    for (int d = 0; d < 3; ++d)
+        if (bshift[d] != 0 && local[d] < EXTRA_MIN_SHIFT) return 0;
+    return 1;
+}
+
+typedef struct {
+    uint32_t *keys, *vals;
+    uint32_t cap;
+    uint32_t len;
+} hmap;
+
+static void hmap_init(hmap *m, uint32_t cap)
+{
+    m->cap = cap;
+    m->len = 0;
+    m->keys = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    m->vals = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    memset(m->keys, 0xFF, sizeof(uint32_t) * cap);
+}
+static void hmap_free(hmap *m) { free(m->keys); free(m->vals); }
+static uint32_t hmap_find(const hmap *m, uint32_t key)
+{
+    uint32_t s = hash_key(key) & (m->cap - 1);
+    for (uint32_t i = 0; i < m->cap; ++i) {
+        if (m->keys[s] == key) return m->vals[s];
+        if (m->keys[s] == NONE) return NONE;
+        s = (s + 1) & (m->cap - 1);
+    }
+    return NONE;
+}
+static int hmap_insert(hmap *m, uint32_t key, uint32_t val);
+static void hmap_grow(hmap *m)
+{
+    hmap bigger;
+    hmap_init(&bigger, m->cap * 2);
+    for (uint32_t i = 0; i < m->cap; ++i)
+        if (m->keys[i] != NONE) hmap_insert(&bigger, m->keys[i], m->vals[i]);
+    hmap_free(m);
+    *m = bigger;
+}
+/* Returns 1 if inserted (new), 0 if present. */
+static int hmap_insert(hmap *m, uint32_t key, uint32_t val)
+{
+    if (2 * (m->len + 1) > m->cap) hmap_grow(m);
+    uint32_t s = hash_key(key) & (m->cap - 1);
+    for (;;) {
+        if (m->keys[s] == key) return 0;
+        if (m->keys[s] == NONE) {
+            m->keys[s] = key;
+            m->vals[s] = val;
+            m->len++;
+            return 1;
+        }
+        s = (s + 1) & (m->cap - 1);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Quadratic B-spline (grid/kernel.rs:103-180). */
+
+static inline REAL qk_eval(REAL x)
+{
+    REAL xa = RFABS(x);
+    REAL part1 = K(0.75) - xa * xa;
+    REAL part2 = K(0.5) * (K(1.5) - xa) * (K(1.5) - xa);
+    REAL lt05 = (xa < K(0.5)) ? K(1.0) : K(0.0);
+    REAL lt15 = (xa < K(1.5)) ? K(1.0) : K(0.0);
+    return lt05 * part1 + (K(1.0) - lt05) * lt15 * part2;
+}
+
+static inline v3 qk_eval_all(REAL x)
+{
+    return V3(K(0.5) * (K(1.5) - x) * (K(1.5) - x), K(0.75) - (x - K(1.0)) * (x - K(1.0)),
+              K(0.5) * (x - K(0.5)) * (x - K(0.5)));
+}
+
+static inline REAL qk_inv_d(REAL h) { return K(4.0) / (h * h); }
+
+/* ------------------------------------------------------------------------- */
+/* 3x3 SVD: one-sided Jacobi (Hestenes).  F = U diag(s) V^T with U, V proper
+ * rotations; the smallest singular value carries the sign of det F. */
+
+static void jacobi_rotate(v3 *ap, v3 *aq, v3 *vp, v3 *vq, REAL *off)
+{
+    REAL alpha = vdot(*ap, *ap), beta = vdot(*aq, *aq), gamma = vdot(*ap, *aq);
+    REAL scale = RSQRT(alpha * beta);
+    if (!(scale > K(0.0))) return;
+    REAL rel = RFABS(gamma) / scale;
+    if (rel > *off) *off = rel;
+    if (rel <= JACOBI_EPS) return;
+    REAL zeta = (beta - alpha) / (K(2.0) * gamma);
+    REAL t;
+    if (RFABS(zeta) > K(1.0e15)) {
+        t = K(0.5) / zeta;
+    } else {
+        REAL sgn = zeta >= K(0.0) ? K(1.0) : K(-1.0);
+        t = sgn / (RFABS(zeta) + RSQRT(K(1.0) + zeta * zeta));
+    }
+    REAL c = K(1.0) / RSQRT(K(1.0) + t * t);
+    REAL s = c * t;
+    v3 a0 = *ap, a1 = *aq, w0 = *vp, w1 = *vq;
+    *ap = vsub(vmul(a0, c), vmul(a1, s));
+    *aq = vadd(vmul(a0, s), vmul(a1, c));
+    *vp = vsub(vmul(w0, c), vmul(w1, s));
+    *vq = vadd(vmul(w0, s), vmul(w1, c));
+}
+
+static v3 any_orthogonal(v3 a)
+{
+    v3 e = RFABS(a.x) < K(0.57) ? V3(1, 0, 0) : (RFABS(a.y) < K(0.57) ? V3(0, 1, 0) : V3(0, 0, 1));
+    v3 c = vcross(a, e);
+    return vdiv(c, vlen(c));
+}
+
+static void svd3_impl(m3 F, m3 *U, v3 *s, m3 *V)
+{
+    v3 a[3] = {F.c0, F.c1, F.c2};
+    v3 w[3] = {V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        REAL off = K(0.0);
+        jacobi_rotate(&a[0], &a[1], &w[0], &w[1], &off);
+        jacobi_rotate(&a[0], &a[2], &w[0], &w[2], &off);
+        jacobi_rotate(&a[1], &a[2], &w[1], &w[2], &off);
+        if (off <= JACOBI_EPS) break;
+    }
+    REAL sig[3];
+    v3 u[3];
+    for (int i = 0; i < 3; ++i) sig[i] = vlen(a[i]);
+    /* Order by decreasing singular value. */
+    for (int i = 0; i < 2; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (sig[j] > sig[i]) {
+                REAL ts = sig[i]; sig[i] = sig[j]; sig[j] = ts;
+                v3 ta = a[i]; a[i] = a[j]; a[j] = ta;
+                v3 tw = w[i]; w[i] = w[j]; w[j] = tw;
+            }
+    REAL tiny = sig[0] * K(1.0e-30);
+    for (int i = 0; i < 3; ++i) u[i] = sig[i] > tiny && sig[i] > K(0.0) ? vdiv(a[i], sig[i]) : V3(0, 0, 0);
+    if (!(sig[0] > K(0.0))) { u[0] = V3(1, 0, 0); u[1] = V3(0, 1, 0); u[2] = V3(0, 0, 1); }
+    else if (!(sig[1] > tiny)) { u[1] = any_orthogonal(u[0]); u[2] = vcross(u[0], u[1]); }
+    else if (!(sig[2] > tiny)) { u[2] = vcross(u[0], u[1]); }
+    m3 Um = M3(u[0], u[1], u[2]);
+    m3 Vm = M3(w[0], w[1], w[2]);
+    if (mdet(Vm) < K(0.0)) { Vm.c2 = vmul(Vm.c2, K(-1.0)); Um.c2 = vmul(Um.c2, K(-1.0)); }
+    if (mdet(Um) < K(0.0)) { Um.c2 = vmul(Um.c2, K(-1.0)); sig[2] = -sig[2]; }
+    *U = Um;
+    *V = Vm;
+    *s = V3(sig[0], sig[1], sig[2]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Constitutive models. */
+
+/* NeoHookeanModel::kirchoff_stress (models/neo_hookean_elasticity.rs:26-39) */
+static m3 nh_stress(REAL lambda, REAL mu, m3 F)
+{
+    REAL j = RFMAX(mdet(F), K(1.0e-10));
+    REAL diag_val = lambda * RLOG(j) - mu;
+    m3 st = mscale(mmul(F, mtrans(F)), mu);
+    st.c0.x += diag_val;
+    st.c1.y += diag_val;
+    st.c2.z += diag_val;
+    return st;
+}
+
+/* LinearElasticModel::kirchoff_stress (models/linear_elasticity.rs:30-54) */
+static m3 linear_stress(REAL lambda, REAL mu, m3 F)
+{
+    m3 U, V;
+    v3 s;
+    svd3_impl(F, &U, &s, &V);
+    REAL j = s.x * s.y * s.z;
+    v3 ms = vsub(s, V3(1, 1, 1));
+    m3 rec = mmul(mmul(U, mdiag(ms)), mtrans(V));
+    REAL diag_val = lambda * (j - K(1.0)) * j;
+    m3 r = mscale(mmul(rec, mtrans(F)), K(2.0) * mu);
+    r.c0.x += diag_val;
+    r.c1.y += diag_val;
+    r.c2.z += diag_val;
+    return r;
+}
+
+/* DruckerPragerPlasticity::alpha (models/drucker_prager.rs:64-68) */
+static REAL dp_alpha(const REAL dp[7], REAL q)
+{
+    REAL angle = dp[0] + (dp[1] * q - dp[3]) * REXP(-dp[2] * q);
+    REAL sa = RSIN(angle);
+    return RSQRT(K(2.0) / K(3.0)) * (K(2.0) * sa) / (K(3.0) - sa);
+}
+
+/* DruckerPragerPlasticity::project (models/drucker_prager.rs:224-267) with
+ * project_deformation_gradient (:121-167).  dp = {ha, hb, hc, hd, lambda, mu, cohesion}. */
+static m3 dp_project(const REAL dp[7], REAL st[3], m3 F)
+{
+    if (dp[4] == K(0.0)) return F;
+    m3 U, V;
+    v3 s;
+    svd3_impl(F, &U, &s, &V);
+    REAL alpha = dp_alpha(dp, st[1]);
+    const REAL d = K(3.0);
+    v3 strain = vadd(V3(RLOG(s.x), RLOG(s.y), RLOG(s.z)), V3(st[2] / d, st[2] / d, st[2] / d));
+    REAL tr = strain.x + strain.y + strain.z;
+    v3 dev = vsub(strain, V3(tr / d, tr / d, tr / d));
+    REAL shifted = tr - dp[6];
+    v3 ns;
+    REAL dq;
+    if (shifted > K(0.0) || (dev.x == K(0.0) && dev.y == K(0.0) && dev.z == K(0.0))) {
+        REAL e = REXP(dp[6] / d);
+        ns = V3(e, e, e);
+        dq = vlen(strain);
+    } else {
+        REAL dn = vlen(dev);
+        REAL gamma = dn + (d * dp[4] + K(2.0) * dp[5]) / (K(2.0) * dp[5]) * shifted * alpha;
+        if (gamma <= K(0.0)) return F; /* elastic: state and F unchanged */
+        v3 hh = vsub(strain, vmul(dev, gamma / dn));
+        ns = V3(REXP(hh.x), REXP(hh.y), REXP(hh.z));
+        dq = gamma;
+    }
+    REAL prev_det = s.x * s.y * s.z;
+    REAL new_det = ns.x * ns.y * ns.z;
+    st[0] = st[0] * prev_det / new_det;
+    st[2] = st[2] + RLOG(prev_det) - RLOG(new_det);
+    st[1] = st[1] + dq;
+    return mmul(mmul(U, mdiag(ns)), mtrans(V));
+}
+
+/* FluidModel::pressure / kirchoff_stress (models/fluid.rs:54-84) */
+static REAL fluid_pressure(const REAL fl[5], REAL j)
+{
+    j = RFMAX(j, K(1.0e-6));
+    if (j <= K(1.0)) {
+        REAL ratio = REXP(-fl[1] * RLOG(j));
+        return fl[0] * (ratio - K(1.0));
+    }
+    return -fl[0] * fl[4] * (j - K(1.0));
+}
+
+static m3 fluid_stress(const REAL fl[5], m3 F, m3 gradv)
+{
+    REAL j = RFMAX(mdet(F), K(1.0e-6));
+    m3 sr = mscale(madd(gradv, mtrans(gradv)), K(0.5)); /* ParticleUpdateData::strain_rate */
+    REAL sph = mtrace(sr) / K(3.0);                       /* models/utils.rs:60-110 */
+    sr.c0.x -= sph;
+    sr.c1.y -= sph;
+    sr.c2.z -= sph;
+    m3 stress = mscale(sr, K(2.0) * fl[2] * j);
+    REAL diag_val = -fluid_pressure(fl, j) * j;
+    stress.c0.x += diag_val;
+    stress.c1.y += diag_val;
+    stress.c2.z += diag_val;
+    return stress;
+}
+
+/* DefaultParticleModel::update (models/default.rs:208-264).  Returns -1 for the
+ * snow tag (out of scope), 0 otherwise; F and st are updated in place. */
+static int model_update(uint32_t tag, const float *w, REAL st[3], m3 *F, m3 gradv, m3 *tau)
+{
+    switch (tag) {
+    case 0:
+        *tau = linear_stress(wordf(w, 0), wordf(w, 1), *F);
+        return 0;
+    case 1:
+        *tau = nh_stress(wordf(w, 0), wordf(w, 1), *F);
+        return 0;
+    case 2:
+    case 3: {
+        REAL dp[7];
+        for (int i = 0; i < 7; ++i) dp[i] = wordf(w, 3 + i);
+        *F = dp_project(dp, st, *F);
+        if (tag == 2) *tau = linear_stress(wordf(w, 10), wordf(w, 11), *F);
+        else *tau = nh_stress(wordf(w, 10), wordf(w, 11), *F);
+        return 0;
+    }
+    case 4: {
+        REAL fl[5];
+        for (int i = 0; i < 5; ++i) fl[i] = wordf(w, i);
+        *tau = fluid_stress(fl, *F, gradv);
+        return 0;
+    }
+    case 5:
+        *tau = mzero();
+        return -1;
+    default:
+        *tau = mzero();
+        return 0;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Cuboid colliders. */
+
+/* Rotation of v by quaternion (b, w): v (w^2 - b.b) + b (2 b.v) + (b x v)(2 w). */
+static v3 quat_rotate(REAL qx, REAL qy, REAL qz, REAL qw, v3 v)
+{
+    v3 b = V3(qx, qy, qz);
+    REAL b2 = vdot(b, b);
+    return vadd(vadd(vmul(v, qw * qw - b2), vmul(b, vdot(v, b) * K(2.0))),
+                vmul(vcross(b, v), qw * K(2.0)));
+}
+
+/* parry3d Cuboid::project_point(pose, pt, solid = false), reached through
+ * Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-384). */
+static v3 project_cuboid(const orc_cuboid *c, v3 pt, int *inside)
+{
+    REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
+    v3 t = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 local = quat_rotate(-qx, -qy, -qz, qw, vsub(pt, t));
+    v3 he = V3(c->half_extents[0], c->half_extents[1], c->half_extents[2]);
+    v3 mins_pt = vsub(vmul(he, K(-1.0)), local);
+    v3 pt_maxs = vsub(local, he);
+    v3 shift = vsub(V3(RFMAX(mins_pt.x, K(0.0)), RFMAX(mins_pt.y, K(0.0)), RFMAX(mins_pt.z, K(0.0))),
+                    V3(RFMAX(pt_maxs.x, K(0.0)), RFMAX(pt_maxs.y, K(0.0)), RFMAX(pt_maxs.z, K(0.0))));
+    v3 lp;
+    if (!(shift.x == K(0.0) && shift.y == K(0.0) && shift.z == K(0.0))) {
+        *inside = 0;
+        lp = vadd(local, shift);
+    } else {
+        *inside = 1;
+        REAL best = -MAX_FLT_REF * K(1.0e20);
+        int is_mins = 0, best_id = 0;
+        for (int i = 0; i < 3; ++i) {
+            REAL a = vget(mins_pt, i), b = vget(pt_maxs, i);
+            if (a < b) {
+                if (b > best) { best_id = i; is_mins = 0; best = b; }
+            } else if (a > best) {
+                best_id = i; is_mins = 1; best = a;
+            }
+        }
+        v3 sh = V3(0, 0, 0);
+        vset(&sh, best_id, is_mins ? best : -best);
+        lp = vadd(local, sh);
+    }
+    return vadd(quat_rotate(qx, qy, qz, qw, lp), t);
+}
+
+/* BoundaryCondition::project_velocity (solver/boundary_condition.rs:77-114) */
+static v3 bc_project(uint32_t ty, REAL friction, v3 vel, v3 n)
+{
+    if (ty == 0) return V3(0, 0, 0);
+    if (ty == 1) {
+        REAL nv = vdot(vel, n);
+        return vsub(vel, vmul(n, nv));
+    }
+    if (ty == 2) {
+        REAL nv = vdot(vel, n);
+        if (nv < K(0.0)) {
+            v3 tv = vsub(vel, vmul(n, nv));
+            REAL tl = vlen(tv);
+            v3 dir = tl > K(1.0e-8) ? vdiv(tv, tl) : V3(0, 0, 0);
+            REAL pl = tl + friction * nv;
+            pl = pl > K(0.0) ? pl : K(0.0);
+            return vmul(dir, pl);
+        }
+        return vel;
+    }
+    return vel;
+}
+
+/* gpu_grid_update_collide body (solver/grid_update_collide.rs:30-99, 105-183).
+ * Velocity::velocity_at_point (src_rbd_shaders/dynamics/body.rs:239-250) with the
+ * collider's world centre of mass at its pose translation. */
+static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt, v3 vel)
+{
+    REAL cap = h * K(1.5);
+    REAL best = MAX_FLT_REF;
+    v3 normal = V3(0, 0, 0);
+    int best_id = 0;
+    for (int i = 0; i < ncol; ++i) {
+        int inside;
+        v3 proj = project_cuboid(&cols[i], pt, &inside);
+        v3 dpt = vsub(proj, pt);
+        int within = RFABS(dpt.x) <= cap && RFABS(dpt.y) <= cap && RFABS(dpt.z) <= cap;
+        if (inside || within) {
+            REAL sign = inside ? K(-1.0) : K(1.0);
+            REAL dist = vlen(dpt);
+            v3 n = vdiv(dpt, dist * -sign);
+            REAL sd = sign * dist;
+            if (sd < best) { best = sd; normal = n; best_id = i; }
+        }
+    }
+    if (best != MAX_FLT_REF) {
+        const orc_cuboid *c = &cols[best_id];
+        v3 com = V3(c->translation[0], c->translation[1], c->translation[2]);
+        v3 lin = V3(c->linvel[0], c->linvel[1], c->linvel[2]);
+        v3 ang = V3(c->angvel[0], c->angvel[1], c->angvel[2]);
+        v3 bv = vadd(lin, vcross(ang, vsub(pt, com)));
+        v3 delta = vsub(vel, bv);
+        REAL nv = vdot(delta, normal);
+        REAL margin = h;
+        if (best <= margin) {
+            vel = vadd(bv, bc_project(c->bc_type, (REAL)c->bc_friction, delta, normal));
+        } else if (-nv * dt > best - margin) {
+            v3 excess = vmul(normal, nv + (best - margin) / dt);
+            vel = vsub(vel, excess);
+        }
+    }
+    return vel;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Workspace. */
+
+struct orc_ws {
+    int64_t nb, nbase, nsorted;
+    int32_t *vid;      /* nb*3 */
+    uint32_t *np, *npe, *first, *buckets, *nbh; /* nb, nb, nb, nb*10, nb*7 */
+    uint32_t *sorted;  /* nsorted */
+    REAL *mv, *m, *vel;
+    int64_t cap_b, cap_s;
+};
+
+orc_ws *orc_ws_create(void)
+{
+    orc_ws *ws = (orc_ws *)calloc(1, sizeof(orc_ws));
+    return ws;
+}
+
+void orc_ws_destroy(orc_ws *ws)
+{
+    if (!ws) return;
+    free(ws->vid); free(ws->np); free(ws->npe); free(ws->first); free(ws->buckets);
+    free(ws->nbh); free(ws->sorted); free(ws->mv); free(ws->m); free(ws->vel);
+    free(ws);
+}
+
+static void ws_reserve(orc_ws *ws, int64_t nb, int64_t ns)
+{
+    if (nb > ws->cap_b) {
+        int64_t c = nb * 2;
+        ws->vid = (int32_t *)realloc(ws->vid, sizeof(int32_t) * 3 * c);
+        ws->np = (uint32_t *)realloc(ws->np, sizeof(uint32_t) * c);
+        ws->npe = (uint32_t *)realloc(ws->npe, sizeof(uint32_t) * c);
+        ws->first = (uint32_t *)realloc(ws->first, sizeof(uint32_t) * c);
+        ws->buckets = (uint32_t *)realloc(ws->buckets, sizeof(uint32_t) * NUM_BUCKETS * c);
+        ws->nbh = (uint32_t *)realloc(ws->nbh, sizeof(uint32_t) * 7 * c);
+        ws->mv = (REAL *)realloc(ws->mv, sizeof(REAL) * 64 * 3 * c);
+        ws->m = (REAL *)realloc(ws->m, sizeof(REAL) * 64 * c);
+        ws->vel = (REAL *)realloc(ws->vel, sizeof(REAL) * 64 * 3 * c);
+        ws->cap_b = c;
+    }
+    if (ns > ws->cap_s) {
+        int64_t c = ns * 2;
+        ws->sorted = (uint32_t *)realloc(ws->sorted, sizeof(uint32_t) * c);
+        ws->cap_s = c;
+    }
+}
+
+int64_t orc_ws_nblocks(const orc_ws *ws) { return ws->nb; }
+int64_t orc_ws_nprimary_blocks(const orc_ws *ws) { return ws->nbase; }
+int64_t orc_ws_nsorted(const orc_ws *ws) { return ws->nsorted; }
+
+int orc_ws_blocks(const orc_ws *ws, int32_t *vid, uint32_t *np, uint32_t *npe, uint32_t *buckets,
+                  uint32_t *first)
+{
+    memcpy(vid, ws->vid, sizeof(int32_t) * 3 * ws->nb);
+    memcpy(np, ws->np, sizeof(uint32_t) * ws->nb);
+    memcpy(npe, ws->npe, sizeof(uint32_t) * ws->nb);
+    memcpy(buckets, ws->buckets, sizeof(uint32_t) * NUM_BUCKETS * ws->nb);
+    memcpy(first, ws->first, sizeof(uint32_t) * ws->nb);
+    return 0;
+}
+
+int orc_ws_sorted_ids(const orc_ws *ws, uint32_t *ids)
+{
+    memcpy(ids, ws->sorted, sizeof(uint32_t) * ws->nsorted);
+    return 0;
+}
+
+int orc_ws_grid(const orc_ws *ws, REAL *mv, REAL *m, REAL *vel)
+{
+    memcpy(mv, ws->mv, sizeof(REAL) * 64 * 3 * ws->nb);
+    memcpy(m, ws->m, sizeof(REAL) * 64 * ws->nb);
+    memcpy(vel, ws->vel, sizeof(REAL) * 64 * 3 * ws->nb);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* One substep. */
+
+int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
+             const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
+             const uint32_t *fixed, const uint32_t *tag, const float *params, const float *sim,
+             const orc_cuboid *cols, int ncol, int keep_debug)
+{
+    (void)keep_debug;
+    const v3 g = V3((REAL)sim[0], (REAL)sim[1], (REAL)sim[2]);
+    const REAL dt = (REAL)sim[3];
+    const REAL h = (REAL)sim[4];
+    int rc = 0;
+
+    /* --- keys --- */
+    int32_t *pvid = (int32_t *)malloc(sizeof(int32_t) * 3 * (n > 0 ? n : 1));
+    uint32_t *ploc = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (n > 0 ? n : 1));
+    uint32_t *pblk = (uint32_t *)malloc(sizeof(uint32_t) * (n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) particle_block(vload(x + 3 * p), h, pvid + 3 * p, ploc + 3 * p);
+
+    /* --- block activation: primary touch in particle order, then the +1
+     *     neighbours of each base block in header order --- */
+    hmap hm;
+    hmap_init(&hm, 4096);
+    int64_t nb = 0, capb = 1024;
+    int32_t *bvid = (int32_t *)malloc(sizeof(int32_t) * 3 * capb);
+    for (int64_t p = 0; p < n; ++p) {
+        uint32_t key = pack_vid(pvid + 3 * p);
+        if (hmap_insert(&hm, key, (uint32_t)nb)) {
+            if (nb == capb) { capb *= 2; bvid = (int32_t *)realloc(bvid, sizeof(int32_t) * 3 * capb); }
+            memcpy(bvid + 3 * nb, pvid + 3 * p, sizeof(int32_t) * 3);
+            nb++;
+        }
+    }
+    int64_t nbase = nb;
+    for (int64_t b = 0; b < nbase; ++b) {
+        for (int i = 1; i < 8; ++i) {
+            int32_t nv[3] = {bvid[3 * b] + ASSOC_SHIFT[i][0], bvid[3 * b + 1] + ASSOC_SHIFT[i][1],
+                             bvid[3 * b + 2] + ASSOC_SHIFT[i][2]};
+            if (hmap_insert(&hm, pack_vid(nv), (uint32_t)nb)) {
+                if (nb == capb) { capb *= 2; bvid = (int32_t *)realloc(bvid, sizeof(int32_t) * 3 * capb); }
+                memcpy(bvid + 3 * nb, nv, sizeof(nv));
+                nb++;
+            }
+        }
+    }
+    for (int64_t p = 0; p < n; ++p) pblk[p] = hmap_find(&hm, pack_vid(pvid + 3 * p));
+
+    /* --- neighbour ids, bucket counts, scan, cursors, scatter --- */
+    uint32_t *nbh = (uint32_t *)malloc(sizeof(uint32_t) * 7 * nb);
+    for (int64_t b = 0; b < nb; ++b)
+        for (int i = 1; i < 8; ++i) {
+            int32_t nv[3] = {bvid[3 * b] + ASSOC_SHIFT[i][0], bvid[3 * b + 1] + ASSOC_SHIFT[i][1],
+                             bvid[3 * b + 2] + ASSOC_SHIFT[i][2]};
+            nbh[7 * b + i - 1] = hmap_find(&hm, pack_vid(nv));
+        }
+    uint32_t *cnt = (uint32_t *)calloc((size_t)(NUM_BUCKETS * nb + 1), sizeof(uint32_t));
+    for (int64_t p = 0; p < n; ++p) {
+        uint32_t b0 = pblk[p];
+        const uint32_t *loc = ploc + 3 * p;
+        cnt[NUM_BUCKETS * b0 + loc[2]]++;
+        for (int i = 1; i < 8; ++i) {
+            if (spills(loc, ASSOC_SHIFT[i])) {
+                uint32_t bi = nbh[7 * b0 + i - 1];
+                cnt[NUM_BUCKETS * bi + extra_bucket(loc, ASSOC_SHIFT[i])]++;
+            }
+        }
+    }
+    uint32_t *first = (uint32_t *)malloc(sizeof(uint32_t) * (nb + 1));
+    uint32_t *npb = (uint32_t *)malloc(sizeof(uint32_t) * (nb + 1));
+    uint32_t *npe = (uint32_t *)malloc(sizeof(uint32_t) * (nb + 1));
+    uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * (NUM_BUCKETS * nb + 1));
+    uint64_t running_total = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        first[b] = (uint32_t)running_total;
+        uint32_t run = first[b];
+        for (int k = 0; k < NUM_BUCKETS; ++k) {
+            if (k == NUM_PRIMARY_BUCKETS) npb[b] = run - first[b];
+            cur[NUM_BUCKETS * b + k] = run;
+            run += cnt[NUM_BUCKETS * b + k];
+        }
+        npe[b] = run - first[b];
+        running_total += npe[b];
+    }
+    int64_t nsorted = (int64_t)running_total;
+    uint32_t *sorted = (uint32_t *)malloc(sizeof(uint32_t) * (nsorted > 0 ? nsorted : 1));
+    for (int64_t p = 0; p < n; ++p) {
+        uint32_t b0 = pblk[p];
+        const uint32_t *loc = ploc + 3 * p;
+        sorted[cur[NUM_BUCKETS * b0 + loc[2]]++] = (uint32_t)p;
+        for (int i = 1; i < 8; ++i) {
+            if (spills(loc, ASSOC_SHIFT[i])) {
+                uint32_t bi = nbh[7 * b0 + i - 1];
+                sorted[cur[NUM_BUCKETS * bi + extra_bucket(loc, ASSOC_SHIFT[i])]++] = (uint32_t)p;
+            }
+        }
+    }
+
+    /* --- P2G: one gather per node over the block's sorted list (p2g.rs:64-351) --- */
+    REAL *gmv = (REAL *)malloc(sizeof(REAL) * 64 * 3 * (nb > 0 ? nb : 1));
+    REAL *gm = (REAL *)malloc(sizeof(REAL) * 64 * (nb > 0 ? nb : 1));
+    REAL *gvel = (REAL *)malloc(sizeof(REAL) * 64 * 3 * (nb > 0 ? nb : 1));
+    const REAL inv_h = K(1.0) / h;
+#pragma omp parallel
+    {
+        int32_t zkey[64];
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t b = 0; b < nb; ++b) {
+            const int32_t *vid = bvid + 3 * b;
+            uint32_t fst = first[b], last = first[b] + npe[b], pend = first[b] + npb[b];
+            uint32_t num_chunks = (npe[b] + 63) / 64;
+            for (int t = 0; t < 64; ++t) {
+                int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
+                v3 cell_pos = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
+                v3 acc_mv = V3(0, 0, 0);
+                REAL acc_m = K(0.0);
+                for (uint32_t ch = 0; ch < num_chunks; ++ch) {
+                    uint32_t base = fst + ch * 64;
+                    uint32_t len = last - base < 64 ? last - base : 64;
+                    for (uint32_t k = 0; k < len; ++k) {
+                        uint32_t pid = sorted[base + k];
+                        REAL zc = RROUND(x[3 * pid + 2] / h) - K(1.0);
+                        int32_t zk = (int32_t)zc - vid[2] * 4;
+                        zkey[k] = zk < -2 ? -2 : zk;
+                    }
+                    int32_t zmin = zkey[0], zmax = zkey[len - 1];
+                    if (pend > base && pend < base + len) {
+                        uint32_t bb = pend - base;
+                        if (zkey[bb] < zmin) zmin = zkey[bb];
+                        if (zkey[bb - 1] > zmax) zmax = zkey[bb - 1];
+                    }
+                    if (!(tz >= zmin && tz <= zmax + 2)) continue;
+                    for (uint32_t k = 0; k < len; ++k) {
+                        uint32_t pid = sorted[base + k];
+                        if (!enabled[pid]) continue; /* contributes exactly +0 (p2g.rs:202-211) */
+                        v3 pos = vload(x + 3 * pid);
+                        REAL pm = mass[pid];
+                        v3 mom = vmul(vload(v + 3 * pid), pm);
+                        m3 aff = mload(C + 9 * pid);
+                        v3 dpt = vsub(cell_pos, pos);
+                        REAL w = qk_eval(dpt.x * inv_h) * qk_eval(dpt.y * inv_h) * qk_eval(dpt.z * inv_h);
+                        if (w != K(0.0)) {
+                            v3 vc = vmul(vadd(mvmul(aff, dpt), mom), w);
+                            acc_mv = vadd(acc_mv, vc);
+                            acc_m = acc_m + pm * w;
+                        }
+                    }
+                }
+                vstore(gmv + 3 * (64 * b + t), acc_mv);
+                gm[64 * b + t] = acc_m;
+            }
+        }
+    }
+
+    /* --- grid update + collider boundary (grid_update.rs:21-88, grid_update_collide.rs) --- */
+    const REAL vlim = h / dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < 64 * nb; ++i) {
+        int64_t b = i / 64;
+        int t = (int)(i % 64);
+        int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
+        const int32_t *vid = bvid + 3 * b;
+        v3 mom = vload(gmv + 3 * i);
+        REAL m = gm[i];
+        REAL inv_mass = m > K(0.0) ? K(1.0) / m : K(0.0);
+        v3 vel = vmul(vadd(mom, vmul(g, m * dt)), inv_mass);
+        vel = V3(RFMIN(RFMAX(vel.x, -vlim), vlim), RFMIN(RFMAX(vel.y, -vlim), vlim),
+                 RFMIN(RFMAX(vel.z, -vlim), vlim));
+        v3 pt = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
+        vel = collide_node(cols, ncol, h, dt, pt, vel);
+        vstore(gvel + 3 * i, vel);
+    }
+
+    /* --- G2P over each block's primaries (g2p.rs:42-347) --- */
+    REAL *pv = (REAL *)malloc(sizeof(REAL) * 3 * (n > 0 ? n : 1));
+    REAL *pg = (REAL *)malloc(sizeof(REAL) * 9 * (n > 0 ? n : 1));
+    REAL *pdiv = (REAL *)malloc(sizeof(REAL) * (n > 0 ? n : 1));
+    const REAL inv_d = qk_inv_d(h);
+#pragma omp parallel
+    {
+        v3 tile[216];
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t b = 0; b < nb; ++b) {
+            if (npb[b] == 0) continue;
+            const int32_t *vid = bvid + 3 * b;
+            for (int o = 0; o < 8; ++o) {
+                int ox = ASSOC_SHIFT[o][0], oy = ASSOC_SHIFT[o][1], oz = ASSOC_SHIFT[o][2];
+                int32_t nv[3] = {vid[0] + ox, vid[1] + oy, vid[2] + oz};
+                uint32_t hid = hmap_find(&hm, pack_vid(nv));
+                for (int t = 0; t < 64; ++t) {
+                    int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
+                    if ((ox && tx > 1) || (oy && ty > 1) || (oz && tz > 1)) continue;
+                    int si = (ox * 4 + tx) + 6 * (oy * 4 + ty) + 36 * (oz * 4 + tz);
+                    tile[si] = hid != NONE ? vload(gvel + 3 * (64 * (int64_t)hid + t)) : V3(0, 0, 0);
+                }
+            }
+            for (uint32_t k = first[b]; k < first[b] + npb[b]; ++k) {
+                uint32_t pid = sorted[k];
+                v3 vel = V3(0, 0, 0);
+                m3 grad = mzero();
+                REAL div = K(0.0);
+                if (enabled[pid]) {
+                    v3 pos = vload(x + 3 * pid);
+                    v3 refp = vsub(vmul(vsub(V3(RROUND(pos.x / h), RROUND(pos.y / h), RROUND(pos.z / h)), V3(1, 1, 1)), h), pos);
+                    v3 w[3] = {qk_eval_all(-refp.x / h), qk_eval_all(-refp.y / h), qk_eval_all(-refp.z / h)};
+                    const uint32_t *loc = ploc + 3 * pid;
+                    int packed = (int)(loc[0] + 6 * loc[1] + 36 * loc[2]);
+                    for (int i = 0; i < 27; ++i) {
+                        int sx = NBH_SHIFTS[i][0], sy = NBH_SHIFTS[i][1], sz = NBH_SHIFTS[i][2];
+                        v3 cv = tile[packed + sx + 6 * sy + 36 * sz];
+                        v3 dpt = vadd(refp, vmul(V3((REAL)sx, (REAL)sy, (REAL)sz), h));
+                        REAL weight = vget(w[0], sx) * vget(w[1], sy) * vget(w[2], sz);
+                        vel = vadd(vel, vmul(cv, weight));
+                        grad = madd(grad, mscale(mouter(cv, dpt), weight * inv_d));
+                        div = div + weight * inv_d * vdot(cv, dpt);
+                    }
+                }
+                vstore(pv + 3 * pid, vel);
+                mstore(pg + 9 * pid, grad);
+                pdiv[pid] = div;
+            }
+        }
+    }
+
+    /* --- particle update (particle_update.rs:60-201), non-CPIC: cdf = 0 --- */
+    int bad_model = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_model)
+    for (int64_t p = 0; p < n; ++p) {
+        v3 vel = vload(pv + 3 * p);
+        m3 gradv = mload(pg + 9 * p);
+        REAL vl = vlen(vel);
+        if (vl > h / dt) vel = vdiv(vmul(vdiv(vel, vl), h), dt);
+        vel = vdiv(vel, K(1.0) + damping[p] * dt);
+        if (fixed[p]) vel = V3(0, 0, 0);
+        v3 pos = vload(x + 3 * p);
+        v3 npos = vadd(pos, vmul(vel, dt));
+        m3 Fm = mload(F + 9 * p);
+        if (tag[p] != 4) {
+            Fm = madd(Fm, mmul(mscale(gradv, dt), Fm));
+        } else {
+            REAL f0 = Fm.c0.x;
+            REAL rate = pdiv[p] * dt / K(3.0);
+            rate = rclamp(rate, K(-0.1), K(0.1));
+            REAL nd = rclamp(f0 + rate * f0, K(0.5), K(2.0));
+            Fm = mdiag(V3(nd, nd, nd));
+        }
+        REAL st[3] = {0, 0, 0};
+        if (tag[p] == 2 || tag[p] == 3) { st[0] = pstate[3 * p]; st[1] = pstate[3 * p + 1]; st[2] = pstate[3 * p + 2]; }
+        m3 tau;
+        if (model_update(tag[p], params + 24 * (size_t)p, st, &Fm, gradv, &tau) != 0) bad_model = 1;
+        if (tag[p] == 2 || tag[p] == 3) { pstate[3 * p] = st[0]; pstate[3 * p + 1] = st[1]; pstate[3 * p + 2] = st[2]; }
+        m3 aff = msub(mscale(gradv, mass[p]), mscale(tau, vol0[p] * inv_d * dt));
+        int nan_pos = npos.x != npos.x || npos.y != npos.y || npos.z != npos.z;
+        if (!nan_pos && mdet(Fm) > K(0.0)) {
+            vstore(x + 3 * p, npos);
+            mstore(C + 9 * p, aff);
+        } else {
+            enabled[p] = 0;
+            vel = V3(0, 0, 0);
+            Fm = mident();
+            mstore(C + 9 * p, mzero());
+            mass[p] = K(0.0);
+        }
+        vstore(v + 3 * p, vel);
+        mstore(F + 9 * p, Fm);
+    }
+    if (bad_model) rc = -2;
+
+    /* --- keep inspection data --- */
+    ws_reserve(ws, nb, nsorted);
+    ws->nb = nb;
+    ws->nbase = nbase;
+    ws->nsorted = nsorted;
+    memcpy(ws->vid, bvid, sizeof(int32_t) * 3 * nb);
+    memcpy(ws->np, npb, sizeof(uint32_t) * nb);
+    memcpy(ws->npe, npe, sizeof(uint32_t) * nb);
+    memcpy(ws->first, first, sizeof(uint32_t) * nb);
+    memcpy(ws->buckets, cnt, sizeof(uint32_t) * NUM_BUCKETS * nb);
+    memcpy(ws->nbh, nbh, sizeof(uint32_t) * 7 * nb);
+    memcpy(ws->sorted, sorted, sizeof(uint32_t) * nsorted);
+    memcpy(ws->mv, gmv, sizeof(REAL) * 64 * 3 * nb);
+    memcpy(ws->m, gm, sizeof(REAL) * 64 * nb);
+    memcpy(ws->vel, gvel, sizeof(REAL) * 64 * 3 * nb);
+
+    free(pv); free(pg); free(pdiv);
+    free(gmv); free(gm); free(gvel);
+    free(sorted); free(cur); free(npe); free(npb); free(first); free(cnt); free(nbh);
+    free(bvid); hmap_free(&hm);
+    free(pvid); free(ploc); free(pblk);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Building blocks for unit tests. */
+
+int orc_svd3(const REAL *F, REAL *U, REAL *s, REAL *V)
+{
+    m3 u, vv;
+    v3 sv;
+    svd3_impl(mload(F), &u, &sv, &vv);
+    mstore(U, u);
+    mstore(V, vv);
+    vstore(s, sv);
+    return 0;
+}
+
+int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F, const REAL *gradv,
+                     REAL *tau)
+{
+    m3 Fm = mload(F), t;
+    REAL st[3] = {pstate[0], pstate[1], pstate[2]};
+    int rc = model_update(tag, params, st, &Fm, mload(gradv), &t);
+    pstate[0] = st[0]; pstate[1] = st[1]; pstate[2] = st[2];
+    mstore(F, Fm);
+    mstore(tau, t);
+    return rc;
+}
+
+int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *inside)
+{
+    v3 p = project_cuboid(c, vload(pt), inside);
+    vstore(proj, p);
+    return 0;
+}
+
+int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim, const REAL *node_pos, REAL *vel)
+{
+    v3 r = collide_node(cols, ncol, (REAL)sim[4], (REAL)sim[3], vload(node_pos), vload(vel));
+    vstore(vel, r);
+    return 0;
+}
+
+int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid, uint32_t *local)
+{
+    for (int64_t p = 0; p < n; ++p) particle_block(vload(x + 3 * p), (REAL)cell_width, vid + 3 * p, local + 3 * p);
+    return 0;
+}

@@ -1,0 +1,88 @@
+/*
+ * mpm_oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of dimforge/nexus's non-CPIC sparse-grid MLS-MPM substep
+ * (binning/sort -> P2G -> grid update + cuboid colliders -> G2P -> particle
+ * update with the constitutive models).  It is the checker for the HIP
+ * library in nexus_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never links or calls it.
+ *
+ * The same source is compiled twice:
+ *   liboracle32.so  REAL=float   op-for-op FP32 mirror of the reference arithmetic
+ *   liboracle64.so  REAL=double  FP64 oracle (same algorithm, double precision)
+ *
+ * Every array argument is caller-owned; nothing is retained after a call except
+ * the inspection data kept in an orc_ws workspace.
+ *
+ * Matrices are 3x3, column-major (9 values: column 0, column 1, column 2), the
+ * memory layout of the reference's glam Mat3.
+ */
+#ifndef NEXUS_MPM_ORACLE_H
+#define NEXUS_MPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifndef REAL
+#define REAL float
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A fixed cuboid collider with a one-way boundary condition.  Same layout as
+ * nxm_cuboid in include/nexus_mpm.h. */
+typedef struct {
+    float half_extents[3];
+    float rotation[4];      /* unit quaternion x, y, z, w */
+    float translation[3];   /* also the body's world centre of mass */
+    float linvel[3];
+    float angvel[3];
+    uint32_t bc_type;       /* 0 stick, 1 slip, 2 separate, 3 non-reflecting */
+    float bc_friction;
+} orc_cuboid;
+
+typedef struct orc_ws orc_ws;
+
+orc_ws *orc_ws_create(void);
+void orc_ws_destroy(orc_ws *ws);
+
+/*
+ * One substep, in place.  sim = {gx, gy, gz, dt, cell_width}.
+ * x, v: n*3; C (the fused APIC "affine"), F: n*9; pstate: n*3 Drucker-Prager
+ * plastic state {det Fp, hardening q, log volume gain} (only read/written for
+ * the sand tags); params: n*24 model words (f32 bit patterns, words 0..2 of the
+ * sand tags are ignored in favour of pstate).  keep_debug != 0 keeps the sort and
+ * grid of this substep in ws for inspection.  Returns 0 on success.
+ */
+int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
+             const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
+             const uint32_t *fixed, const uint32_t *tag, const float *params,
+             const float *sim, const orc_cuboid *cols, int ncol, int keep_debug);
+
+/* Inspection of the last substep run with keep_debug != 0. */
+int64_t orc_ws_nblocks(const orc_ws *ws);
+int64_t orc_ws_nprimary_blocks(const orc_ws *ws);
+int64_t orc_ws_nsorted(const orc_ws *ws);
+/* vid: nb*3; np, npe, first: nb; buckets: nb*10 (per-bucket counts). */
+int orc_ws_blocks(const orc_ws *ws, int32_t *vid, uint32_t *np, uint32_t *npe,
+                  uint32_t *buckets, uint32_t *first);
+int orc_ws_sorted_ids(const orc_ws *ws, uint32_t *ids);
+/* Grid after P2G (mv: nb*64*3, m: nb*64) and after the grid update with the
+ * collider boundary (vel: nb*64*3).  Node order inside a block: x + 4y + 16z. */
+int orc_ws_grid(const orc_ws *ws, REAL *mv, REAL *m, REAL *vel);
+
+/* Building blocks exposed for unit tests. */
+int orc_svd3(const REAL *F, REAL *U, REAL *s, REAL *V);
+int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F,
+                     const REAL *gradv, REAL *tau);
+int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *inside);
+int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim,
+                     const REAL *node_pos, REAL *vel);
+int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid,
+                   uint32_t *local);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

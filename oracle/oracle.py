@@ -1,0 +1,222 @@
+"""ctypes front end of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module.  It loads oracle/build/liboracle32.so (FP32 mirror of the reference
+arithmetic) or liboracle64.so (FP64) built by oracle/Makefile.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "build")
+
+_LIBS: dict[int, ct.CDLL] = {}
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(bits: int = 32) -> ct.CDLL:
+    if bits in _LIBS:
+        return _LIBS[bits]
+    path = os.path.join(BUILD, f"liboracle{bits}.so")
+    if not os.path.exists(path):
+        build()
+    L = ct.CDLL(path)
+    P = ct.c_void_p
+    L.orc_ws_create.restype = P
+    L.orc_ws_destroy.argtypes = [P]
+    L.orc_step.restype = ct.c_int
+    L.orc_step.argtypes = [P, ct.c_int64] + [P] * 14 + [ct.c_int, ct.c_int]
+    for f in ("orc_ws_nblocks", "orc_ws_nprimary_blocks", "orc_ws_nsorted"):
+        getattr(L, f).restype = ct.c_int64
+        getattr(L, f).argtypes = [P]
+    L.orc_ws_blocks.argtypes = [P] * 6
+    L.orc_ws_sorted_ids.argtypes = [P, P]
+    L.orc_ws_grid.argtypes = [P] * 4
+    L.orc_svd3.argtypes = [P] * 4
+    L.orc_model_update.argtypes = [ct.c_uint32] + [P] * 5
+    L.orc_model_update.restype = ct.c_int
+    L.orc_project_cuboid.argtypes = [P] * 4
+    L.orc_grid_node_bc.argtypes = [P, ct.c_int, P, P, P]
+    L.orc_block_keys.argtypes = [ct.c_int64, P, ct.c_float, P, P]
+    _LIBS[bits] = L
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def real(bits: int):
+    return np.float32 if bits == 32 else np.float64
+
+
+@dataclass
+class OracleState:
+    """Particle state held by the oracle in its own precision."""
+
+    bits: int
+    x: np.ndarray
+    v: np.ndarray
+    C: np.ndarray
+    F: np.ndarray
+    mass: np.ndarray
+    vol0: np.ndarray
+    damping: np.ndarray
+    pstate: np.ndarray
+    enabled: np.ndarray
+    fixed: np.ndarray
+    tag: np.ndarray
+    params: np.ndarray  # (n, 24) f32 view of the model words
+    sim: np.ndarray     # gx, gy, gz, dt, h (f32)
+    colliders: np.ndarray
+
+    @staticmethod
+    def from_scene(scene, bits: int = 32) -> "OracleState":
+        R = real(bits)
+        sand = np.isin(scene.tag, (2, 3))
+        words = scene.model.view(np.float32).reshape(-1, 24)
+        pstate = np.zeros((scene.n, 3), R)
+        pstate[sand] = words[sand, 0:3]
+        sim = np.array([*scene.gravity, scene.dt, scene.cell_width], np.float32)
+        return OracleState(
+            bits=bits,
+            x=np.array(scene.x, R), v=np.array(scene.v, R),
+            C=np.array(scene.C, R), F=np.array(scene.F, R),
+            mass=np.array(scene.mass, R), vol0=np.array(scene.vol0, R),
+            damping=np.array(scene.damping, R), pstate=pstate,
+            enabled=np.ascontiguousarray(scene.enabled, np.uint32).copy(),
+            fixed=np.array(scene.fixed, np.uint32), tag=np.array(scene.tag, np.uint32),
+            params=np.array(words, np.float32), sim=sim,
+            colliders=np.array(scene.colliders),
+        )
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+
+class Oracle:
+    """Runs substeps of the reference algorithm on the CPU."""
+
+    def __init__(self, scene, bits: int = 32):
+        self.L = lib(bits)
+        self.bits = bits
+        self.ws = self.L.orc_ws_create()
+        self.s = OracleState.from_scene(scene, bits)
+
+    def __del__(self):
+        try:
+            self.L.orc_ws_destroy(self.ws)
+        except Exception:
+            pass
+
+    def step(self, count: int = 1, keep_debug: bool = True) -> None:
+        s = self.s
+        for _ in range(count):
+            rc = self.L.orc_step(
+                self.ws, s.n, _p(s.x), _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.damping),
+                _p(s.pstate), _p(s.enabled), _p(s.fixed), _p(s.tag), _p(s.params), _p(s.sim),
+                _p(s.colliders) if s.colliders.size else None, int(s.colliders.size), int(keep_debug))
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed: {rc}")
+
+    # ---- inspection of the last substep ----
+    def blocks(self):
+        L, ws = self.L, self.ws
+        nb = L.orc_ws_nblocks(ws)
+        vid = np.zeros((nb, 3), np.int32)
+        np_ = np.zeros(nb, np.uint32)
+        npe = np.zeros(nb, np.uint32)
+        buckets = np.zeros((nb, 10), np.uint32)
+        first = np.zeros(nb, np.uint32)
+        L.orc_ws_blocks(ws, _p(vid), _p(np_), _p(npe), _p(buckets), _p(first))
+        ids = np.zeros(L.orc_ws_nsorted(ws), np.uint32)
+        L.orc_ws_sorted_ids(ws, _p(ids))
+        return dict(vid=vid, np=np_, npe=npe, buckets=buckets, first=first, sorted=ids,
+                    nprimary=int(L.orc_ws_nprimary_blocks(ws)))
+
+    def grid(self):
+        L, ws = self.L, self.ws
+        nb = L.orc_ws_nblocks(ws)
+        R = real(self.bits)
+        mv = np.zeros((nb, 64, 3), R)
+        m = np.zeros((nb, 64), R)
+        vel = np.zeros((nb, 64, 3), R)
+        L.orc_ws_grid(ws, _p(mv), _p(m), _p(vel))
+        return dict(mv=mv, m=m, vel=vel)
+
+
+def canonical_sort(blocks: dict) -> dict:
+    """Canonical form of a block sort: {vid tuple: (np, npe, bucket counts, per-bucket
+    sorted id tuples)}.  Independent of header order and of the order inside buckets."""
+    out = {}
+    ids = blocks["sorted"]
+    for b in range(blocks["vid"].shape[0]):
+        cnt = blocks["buckets"][b]
+        start = int(blocks["first"][b])
+        lists = []
+        for k in range(10):
+            c = int(cnt[k])
+            lists.append(tuple(sorted(int(i) for i in ids[start:start + c])))
+            start += c
+        out[tuple(int(t) for t in blocks["vid"][b])] = (int(blocks["np"][b]), int(blocks["npe"][b]),
+                                                       tuple(int(c) for c in cnt), tuple(lists))
+    return out
+
+
+def svd3(F: np.ndarray, bits: int = 64):
+    R = real(bits)
+    Fc = np.ascontiguousarray(F.reshape(9), R)
+    U = np.zeros(9, R)
+    s = np.zeros(3, R)
+    V = np.zeros(9, R)
+    lib(bits).orc_svd3(_p(Fc), _p(U), _p(s), _p(V))
+    return U, s, V
+
+
+def model_update(tag: int, words: np.ndarray, F: np.ndarray, gradv: np.ndarray, pstate=None, bits: int = 64):
+    R = real(bits)
+    params = np.ascontiguousarray(np.asarray(words, np.uint32).view(np.float32), np.float32)
+    st = np.zeros(3, R) if pstate is None else np.ascontiguousarray(pstate, R).copy()
+    Fc = np.ascontiguousarray(F.reshape(9), R).copy()
+    g = np.ascontiguousarray(gradv.reshape(9), R)
+    tau = np.zeros(9, R)
+    rc = lib(bits).orc_model_update(tag, _p(params), _p(st), _p(Fc), _p(g), _p(tau))
+    return rc, Fc, st, tau
+
+
+def project_cuboid(col: np.ndarray, pt, bits: int = 64):
+    R = real(bits)
+    p = np.ascontiguousarray(pt, R)
+    out = np.zeros(3, R)
+    inside = ct.c_int(0)
+    lib(bits).orc_project_cuboid(_p(col), _p(p), _p(out), ct.byref(inside))
+    return out, bool(inside.value)
+
+
+def grid_node_bc(cols: np.ndarray, sim: np.ndarray, node_pos, vel, bits: int = 64):
+    R = real(bits)
+    p = np.ascontiguousarray(node_pos, R)
+    v = np.ascontiguousarray(vel, R).copy()
+    lib(bits).orc_grid_node_bc(_p(cols) if cols.size else None, int(cols.size),
+                               _p(np.ascontiguousarray(sim, np.float32)), _p(p), _p(v))
+    return v
+
+
+def block_keys(x: np.ndarray, cell_width: float, bits: int = 32):
+    R = real(bits)
+    xc = np.ascontiguousarray(x, R)
+    n = xc.shape[0]
+    vid = np.zeros((n, 3), np.int32)
+    loc = np.zeros((n, 3), np.uint32)
+    lib(bits).orc_block_keys(n, _p(xc), ct.c_float(cell_width), _p(vid), _p(loc))
+    return vid, loc
