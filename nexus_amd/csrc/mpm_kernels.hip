@@ -387,12 +387,20 @@ __global__ void k_scatter(WorldDev w)
 //
 // The tile accumulates in 64-bit fixed point with integer LDS atomics (ds_add_u64):
 // float LDS atomics serialize on gfx950 (~200 cycles per wave-op against ~8 for the
-// 64-bit integer add).  A first pass bounds the block's contributions; the scale is the
-// power of two 2^(42 - e) with every contribution < 2^e, leaving 2^20 contributions of
-// headroom per node below 2^62.  Integer sums do not depend on the order of the adds,
-// so the tile is bit-reproducible whatever the particle order inside the block.
+// 64-bit integer add).  A first pass bounds the block's contributions by 2^e; a tile
+// node receives at most np of them (one per particle of the block), so the scale
+// 2^(62 - e - ceil(log2 np)) keeps every node sum below 2^62 while resolving
+// contributions down to 2^-62 np of the largest.  Integer sums do not depend on the
+// order of the adds, so the tile is bit-reproducible whatever the particle order.
 
-constexpr int P2G_HEADROOM_BITS = 42;
+__device__ inline float fixed_scale(int e, uint32_t np)
+{
+    int lg = 0;
+    while ((1u << lg) < np && lg < 31) ++lg;
+    int k = 62 - e - lg;
+    k = k > 126 ? 126 : (k < -126 ? -126 : k);
+    return ldexpf(1.0f, k);
+}
 
 __device__ inline int bound_exponent(float b)
 {
@@ -405,7 +413,7 @@ __device__ inline int bound_exponent(float b)
 __device__ inline unsigned long long to_fixed(float v, float scale)
 {
     float x = v * scale;
-    x = fminf(fmaxf(x, -4.0e18f), 4.0e18f);
+    x = fminf(fmaxf(x, -4.6e18f), 4.6e18f);
     return (unsigned long long)__float2ll_rn(x);
 }
 
@@ -452,8 +460,8 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         }
         bmom = wave_max(bmom);
         bm = wave_max(bm);
-        const float smom = ldexpf(1.0f, P2G_HEADROOM_BITS - bound_exponent(bmom));
-        const float sm = ldexpf(1.0f, P2G_HEADROOM_BITS - bound_exponent(bm));
+        const float smom = fixed_scale(bound_exponent(bmom), np);
+        const float sm = fixed_scale(bound_exponent(bm), np);
         for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0ull;
         if (l < (4 * TILE_NODES + 31) / 32) bad[l] = 0u;
         __syncthreads();

@@ -65,11 +65,19 @@ def test_grid_after_p2g_matches_oracle(pipeline):
     b = o64.blocks()
     gidx = {tuple(map(int, can["vid"][i])): i for i in range(can["vid"].shape[0])}
     order = np.array([gidx[tuple(map(int, v))] for v in b["vid"]])
-    for name, g, ref in (("mass", grid["m"][order], g64["m"]), ("momentum", grid["mv"][order], g64["mv"]),
-                         ("velocity", grid["vel"][order], g64["vel"])):
+    for name, g, ref in (("mass", grid["m"][order], g64["m"]), ("momentum", grid["mv"][order], g64["mv"])):
         err = np.max(np.abs(g - ref))
         scale = np.max(np.abs(ref))
         assert err <= 2e-6 * scale + 1e-12, f"{name}: {err:.3e} vs scale {scale:.3e}"
+    # Velocity: wherever the GPU node has mass.  A node whose total weight is below the
+    # P2G fixed-point resolution (2^-62 np of the block's largest contribution) may read
+    # as massless; the oracle's mass there must then be negligible.
+    gm, om = grid["m"][order], g64["m"]
+    has = gm > 0
+    assert np.all(om[~has] <= 1e-12 * om.max())
+    vg, vo = grid["vel"][order][has], g64["vel"][has]
+    err = np.max(np.abs(vg - vo))
+    assert err <= 2e-6 * np.max(np.abs(vo)) + 1e-12, f"velocity: {err:.3e}"
 
 
 def test_free_fall_closed_form(pipeline):
@@ -78,11 +86,12 @@ def test_free_fall_closed_form(pipeline):
     st = run_gpu(pipeline, scene, 100)
     p = st.read_particles()
     dt = float(np.float32(scene.dt))
+    g = -float(np.float32(-9.81))
     n = 100
-    assert np.allclose(p["v"][:, 1].astype(np.float64), -9.81 * n * dt, rtol=2e-5, atol=0)
+    assert np.allclose(p["v"][:, 1].astype(np.float64), -g * n * dt, rtol=2e-5, atol=0)
     com0 = scene.x.astype(np.float64).mean(0)
     com = p["x"].astype(np.float64).mean(0)
-    assert abs(com[1] - (com0[1] - 9.81 * dt * dt * n * (n + 1) / 2)) < 2e-7
+    assert abs(com[1] - (com0[1] - g * dt * dt * n * (n + 1) / 2)) < 2e-7
     assert np.max(np.abs(p["F"] - np.eye(3, dtype=np.float32).reshape(9))) < 1e-5
 
 
