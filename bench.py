@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the MLS-MPM substep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sand|cube|fluid]
+
+Default workload: the Drucker-Prager sand column, 1,048,576 particles, h = 1/256,
+16 seeded cuboid colliders, dt = 5e-5 (the headline configuration); K = 1000 substeps
+are timed after W warm-up substeps.  Particle state is resident in HBM before the
+timed region starts.  Prints one JSON line with the whole-job throughput
+(particle-substeps/s), the roofline figure of the dominant kernel (timed live with HIP
+events on the scene's stream during the timed region) and a CPU baseline (the oracle's
+FP32 restatement of the reference algorithm on the host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s at 1M FP32 MLS-MPM; batched scenes/s + p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def build_scene(name: str):
+    from nexus_amd import scenes
+    if name == "sand":
+        sc = scenes.sand_column()
+        desc = ("Drucker-Prager sand column, 1,048,576 particles (friction 30 deg, E=3.5e5, nu=0.3, "
+                "density 2200), h=1/256, 16 seeded cuboid colliders, dt=5e-5")
+    elif name == "cube":
+        sc = scenes.elastic_cube()
+        desc = "neo-Hookean cube, 32,768 particles, h=1/64, six stick walls, dt=1e-4"
+    elif name == "fluid":
+        sc = scenes.fluid_clumps()
+        desc = "weakly-compressible fluid, 8,388,608 particles in 64 Gaussian clumps, h=1/512, dt=2e-5"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return sc, desc
+
+
+def p2g_algorithmic_bytes(n_particles: int, n_blocks: int) -> int:
+    """Compulsory traffic of one P2G launch: per particle x, v, C, m read (12 + 12 + 36 + 4
+    bytes); per node of every active block its momentum and mass written (16 bytes)."""
+    return 64 * n_particles + 16 * 64 * n_blocks
+
+
+def cpu_baseline(scene, budget_s: float = 15.0, max_steps: int = 30):
+    """The oracle's FP32 restatement of the reference substep, OpenMP on the host cores,
+    on the same scene: substeps until the budget is spent (at least 2)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(scene, 32)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        o.step(1, keep_debug=False)
+        steps += 1
+        el = time.perf_counter() - t0
+        if steps >= 2 and el >= budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": scene.n * steps / el,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} substeps of the full {scene.n}-particle scene, FP32 oracle (C restatement of the "
+                  f"reference algorithm), {threads} OpenMP threads, {el:.1f} s",
+    }
+
+
+def traffic_from_profiles(workload: str):
+    path = os.path.join(ROOT, "profiles", f"p2g_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="sand", choices=["sand", "cube", "fluid"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (scenes shard across GPUs "
+              f"without communication; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from nexus_amd import mpm
+
+    scene, desc = build_scene(args.workload)
+    pipe = mpm.MpmPipeline(0)
+    st = mpm.MpmState.from_scene(pipe, scene)
+    for _ in range(args.warmup):
+        pipe.step(st)
+    st.synchronize()
+
+    st.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pipe.step(st)
+    st.synchronize()
+    t1 = time.perf_counter()
+    stages, nprof = st.profile_end()
+    elapsed = t1 - t0
+    n = scene.n
+    value = n * args.steps / elapsed
+    nblocks = st.sort_size()
+
+    p2g_bytes = p2g_algorithmic_bytes(n, nblocks)
+    achieved = p2g_bytes / (stages.p2g_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": desc, "particles": n, "cell_width": scene.cell_width, "dt": scene.dt,
+                   "colliders": int(scene.colliders.size), "active_blocks": nblocks, "parallelism": "single GPU"},
+        "stages_ms": {"sort": stages.sort_ms, "p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
+                      "g2p_update": stages.g2p_update_ms},
+        "roofline": {
+            "kernel": "k_p2g",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(args.workload),
+            "algorithmic_bytes": p2g_bytes,
+            "avg_launch_ms": stages.p2g_ms,
+            "launches_timed": nprof,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, budget_s=args.cpu_budget)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

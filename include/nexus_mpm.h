@@ -202,6 +202,13 @@ nxm_status nxm_scene_set_debug_grid(nxm_scene *scene, int enabled);
 nxm_status nxm_pipeline_step(nxm_ctx *ctx, nxm_scene *scene, nxm_stage_times *times);
 /* Enqueues n substeps (the NexusPipeline::simulate substep loop, src/pipeline.rs:84-86). */
 nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *scene, uint32_t substeps);
+/* Per-stage device timestamps of every step from begin to end, recorded on the
+ * scene's stream without synchronizing and resolved at end (the reference's
+ * GpuTimestamps passed to step and resolved afterwards, src_mpm/pipeline.rs:389,
+ * 521-523; src_viewer/viewer.rs:675-697).  end synchronizes and returns the
+ * per-step averages. */
+nxm_status nxm_scene_profile_begin(nxm_scene *scene);
+nxm_status nxm_scene_profile_end(nxm_scene *scene, nxm_stage_times *avg, uint32_t *steps);
 
 /* ---- batching: many independent scenes stepped by one launch per stage ----
  * No counterpart in the reference (its MPM state is single-scene,
@@ -217,6 +224,8 @@ nxm_status nxm_batch_synchronize(nxm_batch *batch);
 size_t nxm_batch_len(const nxm_batch *batch);
 /* Use a captured HIP graph for each substep of this batch (default on). */
 nxm_status nxm_batch_set_graph(nxm_batch *batch, int enabled);
+nxm_status nxm_batch_profile_begin(nxm_batch *batch);
+nxm_status nxm_batch_profile_end(nxm_batch *batch, nxm_stage_times *avg, uint32_t *steps);
 
 #ifdef __cplusplus
 }

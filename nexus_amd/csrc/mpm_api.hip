@@ -150,6 +150,7 @@ class World {
         if (stream_) {
             (void)hipStreamSynchronize(stream_);
             for (auto &e : ev_) (void)hipEventDestroy(e);
+            for (auto &e : ring_) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream_);
         }
     }
@@ -339,7 +340,12 @@ class World {
         lc.p_hint = p_in_;
         lc.persistent_blocks = ctx_->num_cus * 8;
         lc.p2g_blocks = ctx_->num_cus * 32;
-        launch_substep(w, lc, stream_, times ? ev_ : nullptr);
+        hipEvent_t *evs = times ? ev_ : nullptr;
+        if (!times && profiling_) {
+            nxm_status st = ring_events(&evs);
+            if (st != NXM_OK) return st;
+        }
+        launch_substep(w, lc, stream_, evs);
         HIP_TRY(hipGetLastError());
         // The output arena holds every live particle, scenes in slot order.
         cur_ ^= 1;
@@ -376,6 +382,40 @@ class World {
             times->total_ms = a + b + c + d;
             return collect_errors();
         }
+        return NXM_OK;
+    }
+
+    // ---- per-stage timing without synchronization (GpuTimestamps analog) ----
+    void profile_begin()
+    {
+        profiling_ = true;
+        ring_used_ = 0;
+    }
+
+    nxm_status profile_end(nxm_stage_times *avg, uint32_t *steps)
+    {
+        profiling_ = false;
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        double acc[4] = {0, 0, 0, 0};
+        for (size_t k = 0; k < ring_used_; ++k) {
+            hipEvent_t *e = &ring_[5 * k];
+            for (int i = 0; i < 4; ++i) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+                acc[i] += ms;
+            }
+        }
+        const double n = ring_used_ ? (double)ring_used_ : 1.0;
+        if (avg) {
+            avg->sort_ms = (float)(acc[0] / n);
+            avg->p2g_ms = (float)(acc[1] / n);
+            avg->grid_update_ms = (float)(acc[2] / n);
+            avg->g2p_update_ms = (float)(acc[3] / n);
+            avg->total_ms = (float)((acc[0] + acc[1] + acc[2] + acc[3]) / n);
+        }
+        if (steps) *steps = (uint32_t)ring_used_;
+        ring_used_ = 0;
         return NXM_OK;
     }
 
@@ -460,6 +500,21 @@ class World {
     }
 
   private:
+    nxm_status ring_events(hipEvent_t **out)
+    {
+        if (5 * (ring_used_ + 1) > ring_.size()) {
+            size_t add = std::max<size_t>(5 * 64, ring_.size());
+            for (size_t i = 0; i < add; ++i) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                ring_.push_back(e);
+            }
+        }
+        *out = &ring_[5 * ring_used_];
+        ++ring_used_;
+        return NXM_OK;
+    }
+
     Arena arena(int k) const { return Arena{arenas_[k].p, p_cap_}; }
 
     WorldDev view() const
@@ -629,7 +684,9 @@ class World {
     hipEvent_t ev_[5] = {};
     std::vector<Slot> slots_;
     std::vector<SceneDev> layout_;
-    bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false;
+    bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false, profiling_ = false;
+    std::vector<hipEvent_t> ring_;
+    size_t ring_used_ = 0;
     int cur_ = 0;
     uint32_t p_in_ = 0, p_cap_ = 0, b_total_ = 0, h_total_ = 0;
     size_t ctr_zeroed_ = 0;
@@ -981,6 +1038,35 @@ nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *sc, uint32_t substeps)
         if (st != NXM_OK) return st;
     }
     return NXM_OK;
+}
+
+nxm_status nxm_scene_profile_begin(nxm_scene *sc)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (sc->batch) return fail(NXM_ERR_BUSY, "scene belongs to a batch");
+    nxm_status st = ensure_own_world(sc);
+    if (st != NXM_OK) return st;
+    sc->world->profile_begin();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_profile_end(nxm_scene *sc, nxm_stage_times *avg, uint32_t *steps)
+{
+    if (!sc || !sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    return sc->world->profile_end(avg, steps);
+}
+
+nxm_status nxm_batch_profile_begin(nxm_batch *b)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    b->world->profile_begin();
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_profile_end(nxm_batch *b, nxm_stage_times *avg, uint32_t *steps)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->world->profile_end(avg, steps);
 }
 
 nxm_status nxm_batch_create(nxm_ctx *ctx, nxm_batch **out)

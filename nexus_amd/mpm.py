@@ -42,9 +42,10 @@ EXPORTS = [
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
     "nxm_scene_len", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
-    "nxm_pipeline_step", "nxm_pipeline_run",
+    "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
+    "nxm_batch_profile_begin", "nxm_batch_profile_end",
 ]
 
 
@@ -123,6 +124,10 @@ def library() -> ct.CDLL:
     L.nxm_batch_len.argtypes = [P]
     L.nxm_batch_len.restype = ct.c_size_t
     L.nxm_batch_set_graph.argtypes = [P, I]
+    for f in ("nxm_scene_profile_begin", "nxm_batch_profile_begin"):
+        getattr(L, f).argtypes = [P]
+    for f in ("nxm_scene_profile_end", "nxm_batch_profile_end"):
+        getattr(L, f).argtypes = [P, P, ct.POINTER(ct.c_uint32)]
     _LIB = L
     return L
 
@@ -247,6 +252,17 @@ class MpmState:
     def synchronize(self) -> None:
         _check(self.lib.nxm_scene_synchronize(self.handle))
 
+    def profile_begin(self) -> None:
+        """Start recording per-stage timestamps of every step (no synchronization)."""
+        _check(self.lib.nxm_scene_profile_begin(self.handle))
+
+    def profile_end(self):
+        """Synchronize; return (per-step average StageTimes, number of steps)."""
+        t = _StageTimes()
+        n = ct.c_uint32(0)
+        _check(self.lib.nxm_scene_profile_end(self.handle, ct.byref(t), ct.byref(n)))
+        return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms), int(n.value)
+
     def read_positions(self) -> np.ndarray:
         n = len(self)
         out = np.zeros((n, 3), np.float32)
@@ -348,6 +364,15 @@ class Batch:
 
     def set_graph(self, enabled: bool) -> None:
         _check(self.lib.nxm_batch_set_graph(self.handle, int(enabled)))
+
+    def profile_begin(self) -> None:
+        _check(self.lib.nxm_batch_profile_begin(self.handle))
+
+    def profile_end(self):
+        t = _StageTimes()
+        n = ct.c_uint32(0)
+        _check(self.lib.nxm_batch_profile_end(self.handle, ct.byref(t), ct.byref(n)))
+        return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms), int(n.value)
 
 
 def default_grid_capacity(scene: "_scenes.Scene") -> int:

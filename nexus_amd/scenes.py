@@ -239,6 +239,8 @@ def sand_column(n: int = 1_048_576, seed: int = SEED_BASE + 2, *, cell_width=1.0
     vol = float(np.prod(hi - lo))
     r = 0.5 * (vol / n) ** (1.0 / 3.0)
     parts = particles(pos, r, 2200.0, model_sand(3.5e5, 0.3, friction_deg=30.0))
+    # colliders from their own stream: the same 16 boxes for any particle count
+    rng = np.random.Generator(np.random.PCG64([seed, 1]))
     cols = []
     for _ in range(n_colliders):
         q = random_unit_quaternion(rng)
