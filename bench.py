@@ -86,6 +86,19 @@ def traffic_from_profiles(workload: str):
         return None
 
 
+def calibrate_copy(nbytes: int = 1 << 30, reps: int = 4):
+    """A plain device-to-device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +107,12 @@ def main(argv=None):
     ap.add_argument("--workload", default="sand", choices=["sand", "cube", "fluid"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args(argv)
+
+    if args.calibrate_copy:
+        return calibrate_copy()
 
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (scenes shard across GPUs "
