@@ -5,7 +5,10 @@
 
 Default workload: the Drucker-Prager sand column, 1,048,576 particles, h = 1/256,
 16 seeded cuboid colliders, dt = 5e-5 (the headline configuration); K = 1000 substeps
-are timed after W warm-up substeps.  Particle state is resident in HBM before the
+are timed after W warm-up substeps.  Two further measurements go into the same line:
+"batched" (256 mixed scenes stepped as one batch, against the plain per-scene loop) and
+"served" (Poisson arrivals served with continuous batching: p50/p99 substep latency and
+sustained throughput).  Particle state is resident in HBM before the
 timed region starts.  Prints one JSON line with the whole-job throughput
 (particle-substeps/s), the roofline figure of the dominant kernel (timed live with HIP
 events on the scene's stream during the timed region) and a CPU baseline (the oracle's
@@ -75,6 +78,77 @@ def cpu_baseline(scene, budget_s: float = 15.0, max_steps: int = 30):
     }
 
 
+def bench_batched(pipe, count: int, substeps: int, loop_substeps: int) -> dict:
+    """Many-scene workload: ``count`` independent scenes (log-uniform sizes, mixed
+    materials and cell widths) stepped together as one batch, against the plain loop
+    that steps each scene on its own through the single-scene API."""
+    from nexus_amd import mpm, scenes
+    scs = scenes.batch_scenes(count, substeps=substeps)
+    total = sum(sc.n for sc in scs)
+
+    states = [mpm.MpmState.from_scene(pipe, sc) for sc in scs]
+    for st in states:                      # upload and warm up
+        pipe.step(st)
+    for st in states:
+        st.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_substeps):
+        for st in states:
+            pipe.step(st)
+    for st in states:
+        st.synchronize()
+    loop_s = time.perf_counter() - t0
+    for st in states:
+        st.close()
+    loop_rate = total * loop_substeps / loop_s
+
+    states = [mpm.MpmState.from_scene(pipe, sc) for sc in scs]
+    batch = mpm.Batch(pipe)
+    for st in states:
+        batch.add(st)
+    batch.run(2)
+    batch.synchronize()
+    batch.profile_begin()
+    t0 = time.perf_counter()
+    batch.run(substeps)
+    batch.synchronize()
+    el = time.perf_counter() - t0
+    stages, _ = batch.profile_end()
+    batch.close()
+    for st in states:
+        st.close()
+    rate = total * substeps / el
+    return {
+        "scenes": count, "particles": total, "substeps": substeps,
+        "particle_substeps_per_s": rate,
+        "scenes_per_s": count / el,                       # full horizons completed per second
+        "ms_per_batch_substep": 1e3 * el / substeps,
+        "stages_ms": {"sort": stages.sort_ms, "p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
+                      "g2p_update": stages.g2p_update_ms},
+        "plain_loop": {"particle_substeps_per_s": loop_rate, "substeps": loop_substeps,
+                       "ms_per_substep_all_scenes": 1e3 * loop_s / loop_substeps},
+        "speedup_vs_plain_loop": rate / loop_rate,
+        "workload": f"{count} scenes, N log-uniform in [2e3, 2e5], materials uniform over neo-Hookean / "
+                    f"Drucker-Prager / fluid, h = 1/64..1/128, dt = 1e-4",
+    }
+
+
+def bench_served(pipe, seconds: float, rate: float, budget: float) -> dict:
+    """Serving workload: Poisson arrivals, continuous batching, HBM-budget admission."""
+    from nexus_amd import scenes, serve
+    specs = scenes.served_requests(seconds, rate=rate)
+    reqs = [serve.Request(sp.arrival, sp, sp.horizon, rid=sp.index) for sp in specs]
+    srv = serve.Server(pipe, hbm_budget_bytes=budget)
+    rep = srv.run(reqs, clock="wall")
+    srv.close()
+    out = rep.summary()
+    out["workload"] = (f"Poisson arrivals at {rate:g} scenes/s for {seconds:g} s ({len(reqs)} requests), "
+                       f"N log-uniform in [1e3, 5e5], horizons uniform in [50, 500] substeps, "
+                       f"HBM budget {budget / 1e9:g} GB")
+    out["offered_particle_substeps_per_s"] = sum(sp.n * sp.horizon for sp in specs) / seconds
+    return out
+
+
 def traffic_from_profiles(workload: str):
     path = os.path.join(ROOT, "profiles", f"p2g_traffic_{workload}.json")
     if not os.path.exists(path):
@@ -107,6 +181,14 @@ def main(argv=None):
     ap.add_argument("--workload", default="sand", choices=["sand", "cube", "fluid"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the many-scene batch measurement")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-substeps", type=int, default=500)
+    ap.add_argument("--loop-substeps", type=int, default=10)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args(argv)
@@ -172,6 +254,11 @@ def main(argv=None):
             "launches_timed": nprof,
         },
     }
+    st.close()
+    if not args.no_batch:
+        out["batched"] = bench_batched(pipe, args.batch_scenes, args.batch_substeps, args.loop_substeps)
+    if not args.no_serve:
+        out["served"] = bench_served(pipe, args.serve_seconds, args.serve_rate, args.serve_budget_gb * 1e9)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, budget_s=args.cpu_budget)
     print(json.dumps(out), flush=True)

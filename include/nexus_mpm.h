@@ -179,6 +179,14 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa
  * non-CPIC (use_cpic = false). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
 size_t nxm_scene_len(const nxm_scene *scene);
+/* Standalone steps of this scene replay one captured HIP graph (default off); as
+ * nxm_batch_set_graph. */
+nxm_status nxm_scene_set_graph(nxm_scene *scene, int enabled);
+/* Device memory the scene occupies while resident (particle arenas, sort arrays and its
+ * grid capacity's block tables); used for admission against an HBM budget.  No
+ * reference counterpart (the reference sizes one scene's buffers at creation,
+ * src_mpm/pipeline.rs:107-160). */
+size_t nxm_scene_device_bytes(const nxm_scene *scene);
 /* GpuParticles::read_positions (src_mpm/solver/particle.rs:634-647); synchronizes. */
 nxm_status nxm_scene_read_positions(nxm_scene *scene, float *xyz, size_t n);
 /* Full particle state in original particle order; synchronizes. */
@@ -222,7 +230,10 @@ nxm_status nxm_batch_step(nxm_batch *batch, nxm_stage_times *times);
 nxm_status nxm_batch_run(nxm_batch *batch, uint32_t substeps);
 nxm_status nxm_batch_synchronize(nxm_batch *batch);
 size_t nxm_batch_len(const nxm_batch *batch);
-/* Use a captured HIP graph for each substep of this batch (default on). */
+/* Replay each substep of this batch as one captured HIP graph (default off).  The graph
+ * is recaptured when scenes join or leave or buffers grow; stage timing steps run
+ * uncaptured.  No reference counterpart (the reference records one compute pass per
+ * step, src_mpm/pipeline.rs:385-526). */
 nxm_status nxm_batch_set_graph(nxm_batch *batch, int enabled);
 nxm_status nxm_batch_profile_begin(nxm_batch *batch);
 nxm_status nxm_batch_profile_end(nxm_batch *batch, nxm_stage_times *avg, uint32_t *steps);

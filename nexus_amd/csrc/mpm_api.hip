@@ -131,6 +131,7 @@ struct nxm_scene {
     nxm_batch *batch = nullptr;
     uint32_t error = 0;                 // latched device error bits
     bool debug_grid = false;
+    bool graph = false;                 // standalone steps replay a captured graph
 };
 
 namespace nxm {
@@ -149,6 +150,7 @@ class World {
     {
         if (stream_) {
             (void)hipStreamSynchronize(stream_);
+            drop_graphs();
             for (auto &e : ev_) (void)hipEventDestroy(e);
             for (auto &e : ring_) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream_);
@@ -345,7 +347,28 @@ class World {
             nxm_status st = ring_events(&evs);
             if (st != NXM_OK) return st;
         }
-        launch_substep(w, lc, stream_, evs);
+        if (graph_on_ && !evs) {
+            // Replays the substep's 15 launches as one graph; recaptured when the launch
+            // shapes or any buffer address change.  The graph holds kernels only.
+            const GraphKey k{p_in_, b_total_, h_total_, gen_};
+            if (!gexec_[cur_] || !(gkey_[cur_] == k)) {
+                if (gexec_[cur_]) { HIP_TRY(hipGraphExecDestroy(gexec_[cur_])); gexec_[cur_] = nullptr; }
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                launch_substep(w, lc, stream_, nullptr);
+                hipError_t le = hipGetLastError();
+                hipError_t ce = hipStreamEndCapture(stream_, &graph);
+                if (le != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return hip_fail(le, "launch_substep (capture)"); }
+                HIP_TRY(ce);
+                hipError_t ie = hipGraphInstantiate(&gexec_[cur_], graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ie);
+                gkey_[cur_] = k;
+            }
+            HIP_TRY(hipGraphLaunch(gexec_[cur_], stream_));
+        } else {
+            launch_substep(w, lc, stream_, evs);
+        }
         HIP_TRY(hipGetLastError());
         // The output arena holds every live particle, scenes in slot order.
         cur_ ^= 1;
@@ -525,7 +548,7 @@ class World {
         w.pkey = pkey_.p; w.pdense = pdense_.p; w.prank = prank_.p; w.perm = perm_.p; w.sblock = sblock_.p;
         w.bscene = bscene_.p; w.bvid = bvid_.p; w.bnp = bnp_.p; w.bfirst = bfirst_.p;
         w.bup = bup_.p; w.bdn = bdn_.p; w.stage = stage_.p; w.gridv = gridv_.p;
-        w.hkeys = hkeys_.p; w.hvals = hvals_.p; w.hmark = hmark_.p; w.bprim = bprim_.p;
+        w.hkeys = hkeys_.p; w.hvals = hvals_.p; w.hmark = hmark_.p; w.bprim = bprim_.p; w.bmask = bmask_.p;
         w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
         w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
         w.g = g_.p; w.scan_tmp = scan_tmp_.p;
@@ -553,6 +576,7 @@ class World {
             std::swap(arenas_[k].n, na[k].n);
         }
         p_cap_ = cap;
+        ++gen_;
         HIP_TRY(pkey_.reserve(cap));
         HIP_TRY(pdense_.reserve(cap));
         HIP_TRY(prank_.reserve(cap));
@@ -618,6 +642,7 @@ class World {
             size_t nb = (size_t)b_total_ + b_total_ / 4;
             HIP_TRY(bscene_.reserve(nb));
             HIP_TRY(bprim_.reserve(nb));
+            HIP_TRY(bmask_.reserve(nb));
             HIP_TRY(bvid_.reserve(nb));
             HIP_TRY(bnp_.reserve(nb));
             HIP_TRY(bfirst_.reserve(nb));
@@ -639,6 +664,7 @@ class World {
             HIP_TRY(hipMemcpy(region_scene_.p, region_scene.data(), n_regions_ * 4, hipMemcpyHostToDevice));
         }
         dirty_ = false;
+        ++gen_;
         return NXM_OK;
     }
 
@@ -676,11 +702,44 @@ class World {
     {
         if (on && !debug_) dirty_ = true;
         debug_ = on;
+        ++gen_;
+    }
+
+    nxm_status set_graph(bool on)
+    {
+        if (!on) {
+            nxm_status st = sync();
+            if (st != NXM_OK) return st;
+            drop_graphs();
+        }
+        graph_on_ = on;
+        return NXM_OK;
     }
 
   private:
+    struct GraphKey {
+        uint32_t p_in, b_total, h_total;
+        uint64_t gen;
+        bool operator==(const GraphKey &o) const
+        {
+            return p_in == o.p_in && b_total == o.b_total && h_total == o.h_total && gen == o.gen;
+        }
+    };
+
+    void drop_graphs()
+    {
+        for (auto &g : gexec_) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+
     nxm_ctx *ctx_;
     hipStream_t stream_ = nullptr;
+    bool graph_on_ = false;
+    uint64_t gen_ = 0;                 // bumped whenever a buffer address or table changes
+    hipGraphExec_t gexec_[2] = {};     // one per arena parity
+    GraphKey gkey_[2] = {};
     hipEvent_t ev_[5] = {};
     std::vector<Slot> slots_;
     std::vector<SceneDev> layout_;
@@ -692,7 +751,7 @@ class World {
     size_t ctr_zeroed_ = 0;
     DevBuf<float> arenas_[2];
     DevBuf<uint32_t> pkey_, pdense_, prank_, perm_, sblock_;
-    DevBuf<uint32_t> bscene_, bprim_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_, scan_tmp_;
+    DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_, scan_tmp_;
     DevBuf<uint32_t> region_start_, region_scene_;
     uint32_t n_regions_ = 0, last_b_start_ = 0;
     DevBuf<int4> bvid_;
@@ -752,6 +811,8 @@ static nxm_status ensure_own_world(nxm_scene *sc)
         sc->own = std::make_unique<World>(sc->ctx);
         nxm_status st = sc->own->init();
         if (st != NXM_OK) { sc->own.reset(); return st; }
+        if (sc->graph) (void)sc->own->set_graph(true);
+        if (sc->debug_grid) sc->own->set_debug(true);
     }
     if (sc->world == nullptr) return sc->own->add(sc);
     return NXM_OK;
@@ -823,7 +884,22 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
     return NXM_OK;
 }
 
-static World *world_of(nxm_scene *sc) { return sc->world; }
+namespace nxm {
+// Device memory a scene occupies in a world: both particle arenas and the per-particle
+// sort arrays, plus, per hash slot (each slot may hold a block), the hash entry, the
+// block tables, the P2G staging tile and the grid nodes.
+size_t scene_device_bytes(size_t n, uint32_t grid_cap)
+{
+    const size_t per_particle = 2 * NUM_PLANES * 4 + 5 * 4;
+    const size_t per_block = 3 * 4                    // hkeys, hvals, hmark
+                             + 4 * 4 + sizeof(int4)   // bscene, bprim, bmask, bnp/bfirst
+                             + 4 + 2 * 8 * 4           // bfirst, bup, bdn
+                             + 4 * TILE_NODES * 4      // staging tile
+                             + NODES_PER_BLOCK * 16;   // grid nodes
+    const size_t slots = (size_t)next_pow2(2 * grid_cap) + 1;
+    return n * per_particle + slots * per_block + sizeof(SceneDev) + sizeof(SceneCtr);
+}
+}  // namespace nxm
 
 // ======================================================================================
 // C ABI
@@ -857,8 +933,7 @@ nxm_status nxm_scene_create(nxm_ctx *ctx, const nxm_capacities *caps, nxm_scene 
     auto *sc = new nxm_scene();
     sc->ctx = ctx;
     if (caps && caps->grid_size) sc->grid_cap = next_pow2(caps->grid_size);
-    nxm_status st = ensure_own_world(sc);
-    if (st != NXM_OK) { delete sc; return st; }
+    // The scene stays on the host until it is stepped or joins a batch.
     *out = sc;
     return NXM_OK;
 }
@@ -929,6 +1004,12 @@ nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
 }
 
 size_t nxm_scene_len(const nxm_scene *sc) { return sc ? sc->hp.n : 0; }
+
+size_t nxm_scene_device_bytes(const nxm_scene *sc)
+{
+    if (!sc) return 0;
+    return nxm::scene_device_bytes(sc->hp.n, sc->grid_cap);
+}
 
 static nxm_status latched(nxm_scene *sc, nxm_status st)
 {
@@ -1113,7 +1194,8 @@ nxm_status nxm_batch_remove_scene(nxm_batch *b, nxm_scene *sc)
     sc->batch = nullptr;
     auto &m = b->members;
     m.erase(std::remove(m.begin(), m.end(), sc), m.end());
-    return ensure_own_world(sc);
+    // The scene's state is back on the host; a later standalone step uploads it again.
+    return NXM_OK;
 }
 
 nxm_status nxm_batch_step(nxm_batch *b, nxm_stage_times *times)
@@ -1141,8 +1223,16 @@ size_t nxm_batch_len(const nxm_batch *b) { return b ? b->members.size() : 0; }
 
 nxm_status nxm_batch_set_graph(nxm_batch *b, int enabled)
 {
-    (void)enabled;
     if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->world->set_graph(enabled != 0);
+}
+
+nxm_status nxm_scene_set_graph(nxm_scene *sc, int enabled)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (sc->batch) return fail(NXM_ERR_BUSY, "scene belongs to a batch");
+    sc->graph = enabled != 0;
+    if (sc->own) return sc->own->set_graph(sc->graph);
     return NXM_OK;
 }
 

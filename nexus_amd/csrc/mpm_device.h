@@ -134,12 +134,24 @@ __device__ inline uint32_t pack_vid(int32_t x, int32_t y, int32_t z)
            (((uint32_t)(z + 0x3FF) & 0x7FFu) << 21);
 }
 
+// Spreads the low 10 bits of v to every third bit.
+__host__ __device__ inline uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// First probe slot of a packed block key: the Morton code of its coordinates (low 10
+// bits of each), so spatially close blocks land in close slots and the dense block ids,
+// which are ranks of occupied slots, follow a Morton order.  Blocks whose codes agree
+// modulo the region size collide and are resolved by linear probing.
 __device__ inline uint32_t hash_key(uint32_t k)
 {
-    k *= 0xCC9E2D51u;
-    k = (k << 15) | (k >> 17);
-    k *= 0x1B873593u;
-    return k;
+    return spread3(k & 0x7FFu) | (spread3((k >> 11) & 0x3FFu) << 1) | (spread3(k >> 21) << 2);
 }
 
 // +1 neighbour offsets in the reference's blocks_associated_to_block order
@@ -435,13 +447,17 @@ __device__ inline f3 bc_project(uint32_t ty, float friction, f3 vel, f3 n)
     return vel;
 }
 
-__device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, float h, float dt, f3 pt, f3 vel)
+// mask: colliders that can reach the node's block (bit i for collider i); the others
+// can be neither within 1.5 h of the node nor contain it, so skipping them leaves the
+// reference's result unchanged.
+__device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32_t mask, float h, float dt, f3 pt, f3 vel)
 {
     float cap = h * 1.5f;
     float best = MAX_FLT_REF;
     f3 normal = mk3(0, 0, 0);
     uint32_t best_id = 0;
     for (uint32_t i = 0; i < ncol; ++i) {
+        if (!(mask & (1u << i))) continue;
         bool inside;
         f3 proj = project_cuboid(cols[i], pt, inside);
         f3 dpt = proj - pt;

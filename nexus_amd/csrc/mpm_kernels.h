@@ -24,6 +24,7 @@ struct WorldDev {
     uint32_t *hkeys, *hvals;       // hash regions; hvals: dense block id of each occupied slot
     uint32_t *hmark;               // per hash slot: 1 if a particle's primary block
     uint32_t *bprim;               // per block: 1 if primary (holds particles)
+    uint32_t *bmask;               // per block: colliders that can reach its nodes
     const uint32_t *region_start;  // per active scene, ascending: first hash slot of its region
     const uint32_t *region_scene;  // per active scene: scene slot
     uint32_t n_regions;

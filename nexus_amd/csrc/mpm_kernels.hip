@@ -39,6 +39,22 @@ __device__ inline uint32_t run_end(uint64_t heads, uint32_t h)
     return above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
 }
 
+// XCD-contiguous work ranges.  Workgroups are dealt to the 8 XCDs round-robin
+// (workgroup i runs on XCD i mod 8), and each XCD has its own L2.  Renumbering the
+// workgroups so that those of one XCD are consecutive, and giving each a contiguous
+// range of the (Morton-ordered) work, keeps neighbouring blocks and their particles in
+// the same L2.  The grid must be a multiple of 8 workgroups.
+constexpr uint32_t NUM_XCD = 8;
+
+__device__ inline void xcd_range(uint32_t n, uint32_t &begin, uint32_t &end)
+{
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t logical = (g % NUM_XCD == 0) ? (b % NUM_XCD) * (g / NUM_XCD) + b / NUM_XCD : b;
+    const uint32_t per = (n + g - 1) / g;
+    begin = min(n, logical * per);
+    end = min(n, begin + per);
+}
+
 // ------------------------------------------------------------------------------------
 // Hash regions: open addressing, linear probing, 32-bit packed block keys
 // (src_mpm_shaders/grid/grid.rs:447-536).  Values are dense block ids.
@@ -158,21 +174,21 @@ __global__ void k_keys_touch(WorldDev w)
     }
 }
 
-// Inserts the 7 +1 neighbours of every primary block (hash slots marked by the touch).
+// Inserts the 7 +1 neighbours of every primary block (hash slots marked by the
+// touch): 8 threads per slot, thread 1..7 inserting one neighbour each.
 __global__ void k_touch_nbrs(WorldDev w)
 {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < w.h_total; j += gridDim.x * blockDim.x) {
-        if (w.hmark[j] != 1u) continue;
+    const uint64_t n = (uint64_t)w.h_total * 8;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(t >> 3), i = (uint32_t)(t & 7);
+        if (i == 0 || w.hmark[j] != 1u) continue;
         const uint32_t key = w.hkeys[j];
         const uint32_t s = scene_of_slot(w, j);
         const SceneDev sc = w.scenes[s];
         const int4 v = unpack_vid(key);
-        uint32_t *keys = w.hkeys + sc.h_base;
-        for (int i = 1; i < 8; ++i) {
-            uint32_t slot;
-            hash_insert(keys, sc.h_cap - 1u, pack_vid(v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]), slot);
-            if (slot == NONE) atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
-        }
+        uint32_t slot;
+        hash_insert(w.hkeys + sc.h_base, sc.h_cap - 1u, pack_vid(v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]), slot);
+        if (slot == NONE) atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
     }
 }
 
@@ -303,6 +319,9 @@ __global__ void __launch_bounds__(256) k_scan_down(WorldDev w)
     }
 }
 
+// Per block, 16 threads: threads 0..7 look up the 8 blocks vid - {0,1}^3 (grid update
+// sources), threads 8..15 the 8 blocks vid + {0,1}^3 (G2P tile, primaries only); thread j
+// also tests collider j against the block (bounding spheres) for the collider mask.
 __global__ void k_block_info(WorldDev w)
 {
     // Per scene: its dense block range; the last block of the range is the overflow block.
@@ -315,18 +334,36 @@ __global__ void k_block_info(WorldDev w)
         w.ctr[s].b_count = last == NONE ? 0u : last + 1u - first;
         if (last != NONE && last - first > sc.b_cap) atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
     }
-    const uint32_t nb = w.g->n_blocks;
-    for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nb; d += gridDim.x * blockDim.x) {
+    const uint64_t n = (uint64_t)w.g->n_blocks * 16;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = (uint32_t)(t >> 4), j = (uint32_t)(t & 15);
         const int4 v = w.bvid[d];
         if (v.x == OVERFLOW_VID) continue;
         const SceneDev sc = w.scenes[w.bscene[d]];
-        w.bdn[8 * d] = d;
-        for (int i = 1; i < 8; ++i)
-            w.bdn[8 * d + i] = find_block(w, sc, v.x - ASSOC[i][0], v.y - ASSOC[i][1], v.z - ASSOC[i][2]);
-        if (w.bprim[d]) {
-            w.bup[8 * d] = d;
-            for (int i = 1; i < 8; ++i)
-                w.bup[8 * d + i] = find_block(w, sc, v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]);
+        if (j < 8) {
+            w.bdn[8 * d + j] = j == 0 ? d : find_block(w, sc, v.x - ASSOC[j][0], v.y - ASSOC[j][1], v.z - ASSOC[j][2]);
+        } else if (w.bprim[d]) {
+            const uint32_t i = j - 8;
+            w.bup[8 * d + i] = i == 0 ? d : find_block(w, sc, v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]);
+        }
+        // Collider mask: collider j can touch a node of this block only if its bounding
+        // sphere comes within 1.5 h (the projection reach) of the block's node sphere.
+        bool hit = false;
+        if (j < sc.col_count) {
+            const ColliderDev &c = w.cols[sc.col_start + j];
+            const float h = sc.h;
+            const f3 center = mk3(((float)(4 * v.x) + 1.5f) * h, ((float)(4 * v.y) + 1.5f) * h, ((float)(4 * v.z) + 1.5f) * h);
+            const float rb = 1.5f * h * 1.7320509f;                  // node sphere of the block
+            const float rc = sqrtf(c.he[0] * c.he[0] + c.he[1] * c.he[1] + c.he[2] * c.he[2]);
+            const f3 dc = center - mk3(c.t[0], c.t[1], c.t[2]);
+            const float reach = rb + rc + 1.5f * h * 1.7320509f + 2.0f * h;
+            hit = dot(dc, dc) <= reach * reach;
+        }
+        // 16 consecutive lanes of a wave hold one block: combine their bits.
+        const uint64_t bits = __ballot(hit);
+        if (j == 0) {
+            const uint32_t lane0 = __lane_id() & ~15u;
+            w.bmask[d] = (uint32_t)((bits >> lane0) & 0xFFFFull);
         }
     }
 }
@@ -402,6 +439,10 @@ __device__ inline float fixed_scale(int e, uint32_t np)
     return ldexpf(1.0f, k);
 }
 
+// Staging tile regions, in the order of the +1 offsets e = 4 ex + 2 ey + ez (the
+// reference's blocks_associated_to_block order): sizes 64, 32, 32, 16, 32, 16, 16, 8.
+__device__ __constant__ const int REGION_OFFSET[8] = {0, 64, 96, 128, 144, 176, 192, 208};
+
 __device__ inline int bound_exponent(float b)
 {
     if (!(b > 0.0f) || !isfinite(b)) return 0;
@@ -432,8 +473,9 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
     const float *VX = w.in.plane(PVX), *VY = w.in.plane(PVY), *VZ = w.in.plane(PVZ);
     const float *M = w.in.plane(PMASS);
     const uint32_t *FL = w.in.uplane(PFLAGS);
-    const uint32_t nblocks = w.g->n_blocks;
-    for (uint32_t d = blockIdx.x; d < nblocks; d += gridDim.x) {
+    uint32_t dbegin, dend;
+    xcd_range(w.g->n_blocks, dbegin, dend);
+    for (uint32_t d = dbegin; d < dend; ++d) {
         const uint32_t s = w.bscene[d];
         const uint32_t np = w.bnp[d];
         const int4 v = w.bvid[d];
@@ -520,13 +562,26 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             }
         }
         __syncthreads();
-        float *st = w.stage + (size_t)d * (4 * TILE_NODES);
-        const float inv_smom = 1.0f / smom, inv_sm = 1.0f / sm;
-        for (int k = l; k < 4 * TILE_NODES; k += WAVE) {
-            const float inv = k < 3 * TILE_NODES ? inv_smom : inv_sm;
-            float val = (float)((double)(long long)tile[k] * (double)inv);
-            if (bad[k >> 5] & (1u << (k & 31))) val = __builtin_nanf("");
-            st[k] = val;
+        // Flush: node t of the 6^3 tile goes to the region of the block it belongs to
+        // (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that block's
+        // node order, so the consumer reads each region as one run of float4.
+        float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
+        const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;
+        for (int t = l; t < TILE_NODES; t += WAVE) {
+            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+            const int e = 4 * ex + 2 * ey + ez;
+            const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+            const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+            float v4[4];
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int k = c * TILE_NODES + t;
+                float val = (float)((double)(long long)tile[k] * (c < 3 ? inv_smom : inv_sm));
+                if (bad[k >> 5] & (1u << (k & 31))) val = __builtin_nanf("");
+                v4[c] = val;
+            }
+            st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
         }
         __syncthreads();
     }
@@ -537,8 +592,9 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 // grid_update_collide.rs:105-183) on the summed tiles.
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
-    const uint32_t nn = w.g->n_blocks * NODES_PER_BLOCK;
-    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < nn; n += gridDim.x * blockDim.x) {
+    uint32_t nbegin, nend;
+    xcd_range(w.g->n_blocks * NODES_PER_BLOCK, nbegin, nend);
+    for (uint32_t n = nbegin + threadIdx.x; n < nend; n += blockDim.x) {
         const uint32_t d = n >> 6;
         const uint32_t s = w.bscene[d];
         const int4 v = w.bvid[d];
@@ -552,12 +608,12 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             if ((ex && i > 1) || (ey && j > 1) || (ez && k > 1)) continue;
             const uint32_t src = w.bdn[8 * d + e];
             if (src == NONE || w.bnp[src] == 0u) continue;
-            const int t = (i + 4 * ex) + TILE * (j + 4 * ey) + TILE * TILE * (k + 4 * ez);
-            const float *st = w.stage + (size_t)src * (4 * TILE_NODES);
-            mx += st[t];
-            my += st[TILE_NODES + t];
-            mz += st[2 * TILE_NODES + t];
-            m += st[3 * TILE_NODES + t];
+            const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+            const float4 q = reinterpret_cast<const float4 *>(w.stage)[(size_t)src * TILE_NODES + REGION_OFFSET[e] + i + nx * (j + ny * k)];
+            mx += q.x;
+            my += q.y;
+            mz += q.z;
+            m += q.w;
         }
         if (w.dbg_mom) {
             w.dbg_mom[3 * (size_t)n] = mx;
@@ -572,9 +628,10 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
         vel.x = fminf(fmaxf(vel.x, -lim), lim);
         vel.y = fminf(fmaxf(vel.y, -lim), lim);
         vel.z = fminf(fmaxf(vel.z, -lim), lim);
-        if (sc.col_count) {
+        const uint32_t mask = w.bmask[d];
+        if (mask) {
             const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
-            vel = collide_node(w.cols + sc.col_start, sc.col_count, sc.h, sc.dt, pt, vel);
+            vel = collide_node(w.cols + sc.col_start, sc.col_count, mask, sc.h, sc.dt, pt, vel);
         }
         w.gridv[n] = make_float4(vel.x, vel.y, vel.z, m);
     }
@@ -592,8 +649,9 @@ __device__ __constant__ const int8_t NBH[27][3] = {
 
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
-    const uint32_t n = w.g->p_out;
-    for (uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
+    uint32_t pbegin, pend;
+    xcd_range(w.g->p_out, pbegin, pend);
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
         const uint32_t slot = w.perm[pos];
         const uint32_t d = w.sblock[pos];
         const uint32_t s = w.bscene[d];
@@ -769,6 +827,7 @@ static inline int grid_for(uint64_t n, int threads, int cap)
     uint64_t g = (n + threads - 1) / threads;
     if (g < 1) g = 1;
     if (g > (uint64_t)cap) g = cap;
+    g = (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD;   // whole XCD rounds (xcd_range)
     return (int)g;
 }
 
@@ -779,12 +838,12 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (ev) (void)hipEventRecord(ev[0], st);
     k_clear<<<grid_for(std::max(w.h_total, w.b_total), T, cap), T, 0, st>>>(w);
     k_keys_touch<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
-    k_touch_nbrs<<<grid_for(w.h_total, T, cap), T, 0, st>>>(w);
+    k_touch_nbrs<<<grid_for((uint64_t)w.h_total * 8, T, cap), T, 0, st>>>(w);
     const int hscan = std::min((int)((w.h_total + SCAN_TILE - 1) / SCAN_TILE), cap);
     k_scan_reduce<MODE_SLOTS><<<hscan, 256, 0, st>>>(w);
     k_scan_tiles<MODE_SLOTS><<<1, 256, 0, st>>>(w);
     k_scan_down<MODE_SLOTS><<<hscan, 256, 0, st>>>(w);
-    k_block_info<<<grid_for(w.b_total, T, cap), T, 0, st>>>(w);
+    k_block_info<<<grid_for((uint64_t)w.b_total * 16, T, cap), T, 0, st>>>(w);
     k_count<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     const int bscan = std::min((int)((w.b_total + SCAN_TILE - 1) / SCAN_TILE), cap);
     k_scan_reduce<MODE_COUNTS><<<bscan, 256, 0, st>>>(w);

@@ -40,7 +40,7 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
-    "nxm_scene_len", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
@@ -105,6 +105,10 @@ def library() -> ct.CDLL:
     L.nxm_scene_set_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_len.argtypes = [P]
     L.nxm_scene_len.restype = ct.c_size_t
+    L.nxm_scene_device_bytes.argtypes = [P]
+    L.nxm_scene_device_bytes.restype = ct.c_size_t
+    L.nxm_scene_set_graph.argtypes = [P, ct.c_int]
+    L.nxm_scene_set_graph.restype = ct.c_int
     L.nxm_scene_read_positions.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_sort_size.argtypes = [P, ct.POINTER(ct.c_size_t)]
@@ -218,6 +222,10 @@ class MpmState:
     def __len__(self) -> int:
         return int(self.lib.nxm_scene_len(self.handle))
 
+    def device_bytes(self) -> int:
+        """Device memory the scene occupies while resident in a world or batch."""
+        return int(self.lib.nxm_scene_device_bytes(self.handle))
+
     def set_simulation_params(self, gravity, dt: float) -> None:
         p = _SimParams((ct.c_float * 3)(*gravity), dt)
         _check(self.lib.nxm_scene_set_simulation_params(self.handle, ct.byref(p)))
@@ -234,6 +242,10 @@ class MpmState:
 
     def set_debug_grid(self, enabled: bool = True) -> None:
         _check(self.lib.nxm_scene_set_debug_grid(self.handle, int(enabled)))
+
+    def set_graph(self, enabled: bool = True) -> None:
+        """Standalone steps replay one captured HIP graph per substep."""
+        _check(self.lib.nxm_scene_set_graph(self.handle, int(enabled)))
 
     def add_particles(self, scene: "_scenes.Scene") -> None:
         arrs = dict(

@@ -315,25 +315,39 @@ def batch_scenes(count: int = 256, seed: int = SEED_BASE + 4, *, n_range=(2_000,
 
 
 def served_requests(duration_s: float, seed: int = SEED_BASE + 5, *, rate=200.0, n_range=(1_000, 500_000),
-                    horizon=(50, 500)):
-    """Workload 5: Poisson arrivals (rate per second) of scenes as in workload 4 with
-    N log-uniform in n_range and step horizons uniform in horizon.  Yields
-    (arrival_time_s, Scene) in arrival order."""
+                    horizon=(50, 500)) -> list["RequestSpec"]:
+    """Workload 5: Poisson arrivals (rate per second) over duration_s of scenes as in
+    workload 4, with N log-uniform in n_range and step horizons uniform in horizon.
+    Returns lightweight specs in arrival order; ``spec.build()`` makes the Scene."""
     rng = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
-    i = 0
     out = []
     while True:
         t += rng.exponential(1.0 / rate)
         if t > duration_s:
             break
+        i = len(out)
         srng = np.random.Generator(np.random.PCG64([seed, i]))
         n = int(math.floor(math.exp(srng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
         mat = int(srng.integers(0, 3))
-        h = 1.0 / int(srng.integers(64, 129))
-        sc = material_box(srng, n, mat, h)
-        sc.substeps = int(srng.integers(horizon[0], horizon[1] + 1))
-        sc.name = f"request {i}"
-        out.append((t, sc))
-        i += 1
+        k = int(srng.integers(64, 129))
+        steps = int(srng.integers(horizon[0], horizon[1] + 1))
+        out.append(RequestSpec(t, i, seed, n, mat, k, steps))
     return out
+
+
+class RequestSpec:
+    """One served request: arrival time, scene parameters and horizon (substeps)."""
+
+    __slots__ = ("arrival", "index", "seed", "n", "material", "inv_h", "horizon")
+
+    def __init__(self, arrival, index, seed, n, material, inv_h, horizon):
+        self.arrival, self.index, self.seed = float(arrival), int(index), int(seed)
+        self.n, self.material, self.inv_h, self.horizon = int(n), int(material), int(inv_h), int(horizon)
+
+    def build(self) -> Scene:
+        rng = np.random.Generator(np.random.PCG64([self.seed, self.index, 1]))
+        sc = material_box(rng, self.n, self.material, 1.0 / self.inv_h)
+        sc.substeps = self.horizon
+        sc.name = f"request {self.index}: {self.n} particles, material {self.material}, h=1/{self.inv_h}"
+        return sc
