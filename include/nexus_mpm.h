@@ -179,6 +179,12 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa
  * non-CPIC (use_cpic = false). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
 size_t nxm_scene_len(const nxm_scene *scene);
+/* CFL-based upper bound on the substep length of the scene's current state:
+ * WgTimestepBounds::compute_bounds (src_mpm/solver/timestep_bound.rs:25-54), kernel
+ * src_mpm_shaders/solver/timestep_bound.rs:61-111.  Same fixed-point result (1e-12 s
+ * units, minimum over enabled particles, 4.294967 ms when none is enabled).  Works for
+ * standalone and batched scenes; synchronizes. */
+nxm_status nxm_scene_timestep_bound(nxm_scene *scene, float *max_dt);
 /* Standalone steps of this scene replay one captured HIP graph (default off); as
  * nxm_batch_set_graph. */
 nxm_status nxm_scene_set_graph(nxm_scene *scene, int enabled);

@@ -7,9 +7,11 @@
 // one scene; a batch is a World holding many.  Scenes enter and leave a World between
 // steps; every step re-packs the live scenes' particles contiguously in slot order.
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -78,7 +80,17 @@ using namespace nxm;
 struct nxm_ctx {
     int device = 0;
     int num_cus = 256;
+    bool g2p_tile = false;   // G2P kernel variant (NXM_G2P=tile selects the LDS-tile kernel)
+    // Scenes and batches created from this context; nxm_ctx_destroy defers the delete
+    // until the last of them is destroyed.
+    std::atomic<int> refs{0};
+    std::atomic<bool> closed{false};
 };
+
+static void ctx_release(nxm_ctx *ctx)
+{
+    if (ctx && ctx->refs.fetch_sub(1) == 1 && ctx->closed.load()) delete ctx;
+}
 
 namespace nxm {
 
@@ -342,11 +354,19 @@ class World {
         lc.p_hint = p_in_;
         lc.persistent_blocks = ctx_->num_cus * 8;
         lc.p2g_blocks = ctx_->num_cus * 32;
+        lc.g2p_tile = ctx_->g2p_tile;
         hipEvent_t *evs = times ? ev_ : nullptr;
         if (!times && profiling_) {
             nxm_status st = ring_events(&evs);
             if (st != NXM_OK) return st;
         }
+        if (!clean_) {
+            // Hash regions, counters and scan state must start empty; each substep
+            // leaves them so for the next.
+            launch_clear(w, lc, stream_);
+            HIP_TRY(hipGetLastError());
+        }
+        clean_ = false;
         if (graph_on_ && !evs) {
             // Replays the substep's 15 launches as one graph; recaptured when the launch
             // shapes or any buffer address change.  The graph holds kernels only.
@@ -370,6 +390,7 @@ class World {
             launch_substep(w, lc, stream_, evs);
         }
         HIP_TRY(hipGetLastError());
+        clean_ = true;
         // The output arena holds every live particle, scenes in slot order.
         cur_ ^= 1;
         uint32_t at = 0;
@@ -439,6 +460,27 @@ class World {
         }
         if (steps) *steps = (uint32_t)ring_used_;
         ring_used_ = 0;
+        return NXM_OK;
+    }
+
+    // ---- CFL timestep bound of every scene of the world (in 1e-12 s units) ----
+    nxm_status timestep_bound(nxm_scene *sc, uint32_t *units)
+    {
+        HIP_TRY(hipSetDevice(ctx_->device));
+        if (dirty_) {
+            nxm_status st = rebuild_tables();
+            if (st != NXM_OK) return st;
+        }
+        *units = 0xFFFFFFFFu;
+        if (p_in_ == 0) return NXM_OK;
+        DevBuf<uint32_t> out;
+        HIP_TRY(out.reserve(slots_.size()));
+        HIP_TRY(hipMemsetD32Async(out.p, 0xFFFFFFFF, slots_.size(), stream_));
+        launch_timestep_bound(view(), p_in_, out.p, ctx_->num_cus * 8, stream_);
+        HIP_TRY(hipGetLastError());
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        HIP_TRY(hipMemcpy(units, out.p + sc->slot, sizeof(uint32_t), hipMemcpyDeviceToHost));
         return NXM_OK;
     }
 
@@ -551,7 +593,7 @@ class World {
         w.hkeys = hkeys_.p; w.hvals = hvals_.p; w.hmark = hmark_.p; w.bprim = bprim_.p; w.bmask = bmask_.p;
         w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
         w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
-        w.g = g_.p; w.scan_tmp = scan_tmp_.p;
+        w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
@@ -651,7 +693,8 @@ class World {
             HIP_TRY(stage_.reserve((size_t)4 * TILE_NODES * nb));
             HIP_TRY(gridv_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
-        HIP_TRY(scan_tmp_.reserve(std::max<size_t>(b_total_, h_total_) / 1024 + 2));
+        scan_stride_ = std::max<uint32_t>(b_total_, h_total_) / 1024u + 2u;
+        HIP_TRY(scan_state_.reserve(2 * (size_t)scan_stride_));
         if (debug_) HIP_TRY(dbg_mom_.reserve((size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(b_total_, 1)));
         HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hvals_.reserve(std::max<uint32_t>(h_total_, 1)));
@@ -664,25 +707,29 @@ class World {
             HIP_TRY(hipMemcpy(region_scene_.p, region_scene.data(), n_regions_ * 4, hipMemcpyHostToDevice));
         }
         dirty_ = false;
+        clean_ = false;
         ++gen_;
         return NXM_OK;
     }
 
     nxm_status collect_errors()
     {
-        if (slots_.empty() || !ctr_.p) return NXM_OK;
-        std::vector<SceneCtr> c(slots_.size());
+        // Counters exist for the slots of the last table rebuild; a scene added since
+        // then has not run yet.
+        const size_t nslots = std::min(slots_.size(), ctr_.n);
+        if (nslots == 0 || !ctr_.p) return NXM_OK;
+        std::vector<SceneCtr> c(nslots);
         HIP_TRY(hipMemcpy(c.data(), ctr_.p, c.size() * sizeof(SceneCtr), hipMemcpyDeviceToHost));
         nxm_status first = NXM_OK;
         bool any = false;
-        for (uint32_t i = 0; i < slots_.size(); ++i) {
+        for (uint32_t i = 0; i < nslots; ++i) {
             if (!c[i].err) continue;
             any = true;
             if (slots_[i].scene) slots_[i].scene->error |= c[i].err;
         }
         if (any) {
             HIP_TRY(hipMemset(ctr_.p, 0, c.size() * sizeof(SceneCtr)));
-            for (uint32_t i = 0; i < slots_.size(); ++i) {
+            for (uint32_t i = 0; i < nslots; ++i) {
                 if (!c[i].err) continue;
                 if (first == NXM_OK) {
                     if (c[i].err & ERR_GRID_CAPACITY)
@@ -743,7 +790,7 @@ class World {
     hipEvent_t ev_[5] = {};
     std::vector<Slot> slots_;
     std::vector<SceneDev> layout_;
-    bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false, profiling_ = false;
+    bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false, profiling_ = false, clean_ = false;
     std::vector<hipEvent_t> ring_;
     size_t ring_used_ = 0;
     int cur_ = 0;
@@ -751,7 +798,9 @@ class World {
     size_t ctr_zeroed_ = 0;
     DevBuf<float> arenas_[2];
     DevBuf<uint32_t> pkey_, pdense_, prank_, perm_, sblock_;
-    DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_, scan_tmp_;
+    DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_;
+    DevBuf<unsigned long long> scan_state_;
+    uint32_t scan_stride_ = 0;
     DevBuf<uint32_t> region_start_, region_scene_;
     uint32_t n_regions_ = 0, last_b_start_ = 0;
     DevBuf<int4> bvid_;
@@ -921,17 +970,25 @@ nxm_status nxm_ctx_create(int device, nxm_ctx **out)
     auto *c = new nxm_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    const char *g2p = std::getenv("NXM_G2P");
+    c->g2p_tile = g2p && std::strcmp(g2p, "tile") == 0;
     *out = c;
     return NXM_OK;
 }
 
-void nxm_ctx_destroy(nxm_ctx *ctx) { delete ctx; }
+void nxm_ctx_destroy(nxm_ctx *ctx)
+{
+    if (!ctx) return;
+    ctx->closed.store(true);
+    if (ctx->refs.load() == 0) delete ctx;
+}
 
 nxm_status nxm_scene_create(nxm_ctx *ctx, const nxm_capacities *caps, nxm_scene **out)
 {
     if (!ctx || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
     auto *sc = new nxm_scene();
     sc->ctx = ctx;
+    ctx->refs.fetch_add(1);
     if (caps && caps->grid_size) sc->grid_cap = next_pow2(caps->grid_size);
     // The scene stays on the host until it is stepped or joins a batch.
     *out = sc;
@@ -947,7 +1004,9 @@ void nxm_scene_destroy(nxm_scene *sc)
         if (sc->world) (void)sc->world->remove(sc);
     }
     sc->own.reset();
+    nxm_ctx *ctx = sc->ctx;
     delete sc;
+    ctx_release(ctx);
 }
 
 nxm_status nxm_scene_set_simulation_params(nxm_scene *sc, const nxm_sim_params *p)
@@ -1074,6 +1133,20 @@ nxm_status nxm_scene_read_particles_soa(nxm_scene *sc, const nxm_particles_soa *
     return latched(sc, NXM_OK);
 }
 
+nxm_status nxm_scene_timestep_bound(nxm_scene *sc, float *max_dt)
+{
+    if (!sc || !max_dt) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!sc->batch) {
+        nxm_status st = ensure_own_world(sc);
+        if (st != NXM_OK) return st;
+    }
+    uint32_t units = 0xFFFFFFFFu;
+    nxm_status st = sc->world->timestep_bound(sc, &units);
+    if (st != NXM_OK) return st;
+    *max_dt = (float)units / 1.0e12f;
+    return latched(sc, NXM_OK);
+}
+
 nxm_status nxm_scene_sort_size(nxm_scene *sc, size_t *nb)
 {
     if (!sc || !nb) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
@@ -1155,9 +1228,10 @@ nxm_status nxm_batch_create(nxm_ctx *ctx, nxm_batch **out)
     if (!ctx || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
     auto *b = new nxm_batch();
     b->ctx = ctx;
+    ctx->refs.fetch_add(1);
     b->world = std::make_unique<World>(ctx);
     nxm_status st = b->world->init();
-    if (st != NXM_OK) { delete b; return st; }
+    if (st != NXM_OK) { delete b; ctx_release(ctx); return st; }
     *out = b;
     return NXM_OK;
 }
@@ -1167,7 +1241,9 @@ void nxm_batch_destroy(nxm_batch *b)
     if (!b) return;
     std::vector<nxm_scene *> m = b->members;
     for (nxm_scene *sc : m) (void)nxm_batch_remove_scene(b, sc);
+    nxm_ctx *ctx = b->ctx;
     delete b;
+    ctx_release(ctx);
 }
 
 nxm_status nxm_batch_add_scene(nxm_batch *b, nxm_scene *sc)

@@ -33,7 +33,8 @@ struct WorldDev {
     const ClassDev *classes;
     const ColliderDev *cols;
     Globals *g;
-    uint32_t *scan_tmp;
+    unsigned long long *scan_state;   // single-pass scan tile status, 2 x scan_stride words
+    uint32_t scan_stride;
     float *dbg_mom;                // optional per-node momentum after P2G (3 floats)
     uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
 };
@@ -41,10 +42,13 @@ struct WorldDev {
 struct LaunchConfig {
     uint64_t p_hint;               // upper bound of particles (grid sizing only)
     int persistent_blocks;         // cap on workgroups of grid-stride kernels
-    int p2g_blocks;                // workgroups of the P2G work queue
+    int p2g_blocks;                // workgroups of the per-block kernels
+    bool g2p_tile;                 // G2P through a per-block LDS node tile
 };
 
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev);
+void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st);
+void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st);
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,
                       hipStream_t st);
 

@@ -1,12 +1,14 @@
 // mpm_kernels.hip — gfx950 kernels of the MLS-MPM substep.
 //
 // One substep over every active scene of a world (device data in WorldDev):
-//   k_clear         clear hash regions, block tables and counters
-//   k_keys_touch    per particle: block key; wave-deduplicated insertion of primary blocks
-//   k_touch_nbrs    per primary block: insert its 7 +1 neighbours
-//   k_block_info    per block: dense ids of the 7 +1 and 7 -1 neighbours
-//   k_count         per particle: dense block id, wave-aggregated block counts and ranks
-//   k_scan_*        exclusive scan of block counts -> first particle of each block
+//   k_clear         clear hash regions, block tables and counters (first substep and after
+//                   the tables change; afterwards k_keys_touch and k_g2p_update keep them)
+//   k_keys_touch    per particle: block key; per run of equal keys in a wave, insertion
+//                   of the block and its 7 +1 neighbours
+//   k_scan<SLOTS>   single-pass scan of occupied hash slots -> dense block ids
+//   k_block_count   per block: dense ids of the 7 +1 and 7 -1 neighbours, collider mask;
+//                   per particle: dense block id, wave-aggregated block counts and ranks
+//   k_scan<COUNTS>  single-pass scan of block counts -> first particle of each block
 //   k_scatter       per particle: sorted position (block-major permutation)
 //   k_p2g           per block: LDS-staged scatter of its particles into a 6^3 node tile
 //   k_grid_update   per node: sum of the tiles covering it, momentum -> velocity, gravity,
@@ -117,18 +119,26 @@ __device__ inline int4 unpack_vid(uint32_t key)
 
 __global__ void k_clear(WorldDev w)
 {
-    uint32_t n = max(w.h_total, w.b_total);
+    uint32_t n = max(max(w.h_total, w.b_total), 2u * w.scan_stride);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < w.h_total) { w.hkeys[i] = EMPTY_KEY; w.hmark[i] = 0u; }
         if (i < w.b_total) w.bnp[i] = 0u;
-        if (i == 0) { w.g->p_out = 0u; w.g->n_blocks = 0u; }
+        if (i < 2u * w.scan_stride) w.scan_state[i] = 0ull;
+        if (i == 0) { w.g->p_out = 0u; w.g->n_blocks = 0u; w.g->ticket[0] = 0u; w.g->ticket[1] = 0u; }
     }
 }
 
+// Hash keys and marks are left empty by the previous substep's G2P kernel (k_clear
+// does it before the first substep and after the tables change); the counters used
+// later in this substep are cleared here.
 __global__ void k_keys_touch(WorldDev w)
 {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < w.b_total; i += gsz) w.bnp[i] = 0u;
+    for (uint32_t i = gid; i < 2u * w.scan_stride; i += gsz) w.scan_state[i] = 0ull;
+    if (gid == 0) { w.g->p_out = 0u; w.g->n_blocks = 0u; w.g->ticket[0] = 0u; w.g->ticket[1] = 0u; }
     // The virtual overflow slot closing each active scene's region.
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.s_count; s += gridDim.x * blockDim.x) {
+    for (uint32_t s = gid; s < w.s_count; s += gsz) {
         const SceneDev sc = w.scenes[s];
         if (sc.active) w.hkeys[sc.h_base + sc.h_cap] = OVERFLOW_KEY;
     }
@@ -160,35 +170,29 @@ __global__ void k_keys_touch(WorldDev w)
             }
             w.pkey[i] = key;
         }
-        // Run heads: adjacent lanes with the same (scene, key) insert once.
+        // Run heads: adjacent lanes with the same (scene, key) insert once.  For each
+        // run, lanes 0..7 insert the block and its 7 +1 neighbours (the active set is
+        // P + {0,1}^3, src_mpm/grid/grid.rs:193-200); lane 0 marks the block primary.
         uint32_t pk = __shfl_up(key, 1, WAVE);
         uint32_t ps = __shfl_up(s, 1, WAVE);
         bool pv = __shfl_up((int)valid, 1, WAVE) != 0;
         bool head = valid && key != OVERFLOW_KEY && (l == 0 || !pv || pk != key || ps != s);
-        if (head) {
-            uint32_t slot;
-            hash_insert(w.hkeys + sc.h_base, sc.h_cap - 1u, key, slot);
-            if (slot != NONE) w.hmark[sc.h_base + slot] = 1u;
-            else atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
+        uint64_t heads = __ballot(head);
+        while (heads) {
+            const uint32_t hl = (uint32_t)__ffsll((unsigned long long)heads) - 1u;
+            heads &= heads - 1ull;
+            const uint32_t hk = __shfl(key, (int)hl, WAVE);
+            const uint32_t hs = __shfl(s, (int)hl, WAVE);
+            if (l < 8) {
+                const SceneDev hsc = w.scenes[hs];
+                const int4 v = unpack_vid(hk);
+                uint32_t slot;
+                hash_insert(w.hkeys + hsc.h_base, hsc.h_cap - 1u,
+                            pack_vid(v.x + ASSOC[l][0], v.y + ASSOC[l][1], v.z + ASSOC[l][2]), slot);
+                if (slot == NONE) atomicOr(&w.ctr[hs].err, ERR_GRID_CAPACITY);
+                else if (l == 0) w.hmark[hsc.h_base + slot] = 1u;
+            }
         }
-    }
-}
-
-// Inserts the 7 +1 neighbours of every primary block (hash slots marked by the
-// touch): 8 threads per slot, thread 1..7 inserting one neighbour each.
-__global__ void k_touch_nbrs(WorldDev w)
-{
-    const uint64_t n = (uint64_t)w.h_total * 8;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t j = (uint32_t)(t >> 3), i = (uint32_t)(t & 7);
-        if (i == 0 || w.hmark[j] != 1u) continue;
-        const uint32_t key = w.hkeys[j];
-        const uint32_t s = scene_of_slot(w, j);
-        const SceneDev sc = w.scenes[s];
-        const int4 v = unpack_vid(key);
-        uint32_t slot;
-        hash_insert(w.hkeys + sc.h_base, sc.h_cap - 1u, pack_vid(v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]), slot);
-        if (slot == NONE) atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
     }
 }
 
@@ -243,78 +247,91 @@ __device__ inline uint32_t block_excl_scan_256(uint32_t v, uint32_t *sh, uint32_
     return off + incl - v;
 }
 
+// Output of one scanned element: its exclusive prefix r and its value v.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_scan_reduce(WorldDev w)
+__device__ inline void scan_out(const WorldDev &w, uint32_t i, uint32_t r, uint32_t v)
 {
-    __shared__ uint32_t sh[4];
-    const uint32_t len = scan_len<MODE>(w);
-    const uint32_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        uint32_t v = 0;
-        for (int k = 0; k < 4; ++k) {
-            uint32_t i = t * SCAN_TILE + k * 256 + threadIdx.x;
-            if (i < len) v += scan_in<MODE>(w, i);
+    if (MODE == MODE_COUNTS) {
+        w.bfirst[i] = r;
+        return;
+    }
+    // Dense block id of slot i: its rank among occupied slots.
+    w.hvals[i] = r;
+    if (v) {
+        const uint32_t key = w.hkeys[i];
+        const uint32_t s = scene_of_slot(w, i);
+        if (r < w.b_total) {
+            w.bscene[r] = s;
+            w.bvid[r] = key == OVERFLOW_KEY ? make_int4(OVERFLOW_VID, 0, 0, 0) : unpack_vid(key);
+            w.bprim[r] = w.hmark[i];
+        } else {
+            w.hvals[i] = NONE;
+            atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
         }
-        uint32_t total = block_reduce_256(v, sh);
-        if (threadIdx.x == 0) w.scan_tmp[t] = total;
     }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(256) k_scan_tiles(WorldDev w)
-{
-    __shared__ uint32_t sh[4];
-    const uint32_t ntiles = (scan_len<MODE>(w) + SCAN_TILE - 1) / SCAN_TILE;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += 256) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < ntiles ? w.scan_tmp[i] : 0u;
-        uint32_t total;
-        uint32_t ex = block_excl_scan_256(v, sh, total);
-        if (i < ntiles) w.scan_tmp[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) {
-        if (MODE == MODE_SLOTS) w.g->n_blocks = min(carry, w.b_total);
-        else w.g->p_out = carry;
-    }
-}
+// Single-pass exclusive scan with decoupled look-back (Merrill and Garland, "Single-pass
+// Parallel Prefix Scan with Decoupled Look-back", 2016).  Workgroups take tiles in ticket
+// order, publish the tile's aggregate, then add the published values of the preceding
+// tiles until one carries an inclusive prefix.  Status and value share one 64-bit word
+// (bits 32-33: 1 aggregate, 2 inclusive prefix), so a reader sees both or neither.  A
+// tile waits only for tiles with smaller tickets; those belong to workgroups already
+// running, which publish their aggregate before waiting themselves, so every wait ends.
+constexpr unsigned long long ST_AGG = 1ull << 32, ST_PREFIX = 2ull << 32;
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_scan_down(WorldDev w)
+__global__ void __launch_bounds__(256) k_scan(WorldDev w)
 {
     __shared__ uint32_t sh[4];
-    const uint32_t len = scan_len<MODE>(w);
+    __shared__ uint32_t s_tile, s_prefix;
+    const uint32_t len = MODE == MODE_SLOTS ? w.h_total : min(w.g->n_blocks, w.b_total);
     const uint32_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        uint32_t carry = w.scan_tmp[t];
+    unsigned long long *state = w.scan_state + (MODE == MODE_SLOTS ? 0u : w.scan_stride);
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&w.g->ticket[MODE], 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= ntiles) break;
+        uint32_t v[4], sum = 0;
+        const uint32_t i0 = t * SCAN_TILE + 4 * threadIdx.x;
+        #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t i = t * SCAN_TILE + k * 256 + threadIdx.x;
-            const uint32_t v = i < len ? scan_in<MODE>(w, i) : 0u;
-            uint32_t total;
-            const uint32_t ex = block_excl_scan_256(v, sh, total);
-            if (i < len) {
-                const uint32_t r = carry + ex;
-                if (MODE == MODE_COUNTS) {
-                    w.bfirst[i] = r;
-                } else {
-                    // Dense block id of slot i: its rank among occupied slots.
-                    w.hvals[i] = r;
-                    if (v) {
-                        const uint32_t key = w.hkeys[i];
-                        const uint32_t s = scene_of_slot(w, i);
-                        if (r < w.b_total) {
-                            w.bscene[r] = s;
-                            w.bvid[r] = key == OVERFLOW_KEY ? make_int4(OVERFLOW_VID, 0, 0, 0) : unpack_vid(key);
-                            w.bprim[r] = w.hmark[i];
-                        } else {
-                            w.hvals[i] = NONE;
-                            atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
-                        }
-                    }
+            v[k] = i0 + k < len ? scan_in<MODE>(w, i0 + k) : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan_256(sum, sh, total);
+        if (threadIdx.x == 0) {
+            uint32_t prefix = 0;
+            if (t == 0) {
+                __hip_atomic_store(&state[0], ST_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&state[t], ST_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t j = t - 1;
+                for (;;) {
+                    const unsigned long long st = __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long flag = st & (3ull << 32);
+                    if (flag == 0ull) continue;            // not yet published
+                    prefix += (uint32_t)st;
+                    if (flag == ST_PREFIX) break;
+                    --j;                                   // j > 0: tile 0 publishes a prefix
                 }
+                __hip_atomic_store(&state[t], ST_PREFIX | (unsigned long long)(prefix + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
-            carry += total;
+            s_prefix = prefix;
+            if (t == ntiles - 1) {
+                if (MODE == MODE_SLOTS) w.g->n_blocks = min(prefix + total, w.b_total);
+                else w.g->p_out = prefix + total;
+            }
+        }
+        __syncthreads();
+        uint32_t r = s_prefix + ex;
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < len) scan_out<MODE>(w, i0 + k, r, v[k]);
+            r += v[k];
         }
     }
 }
@@ -322,7 +339,7 @@ __global__ void __launch_bounds__(256) k_scan_down(WorldDev w)
 // Per block, 16 threads: threads 0..7 look up the 8 blocks vid - {0,1}^3 (grid update
 // sources), threads 8..15 the 8 blocks vid + {0,1}^3 (G2P tile, primaries only); thread j
 // also tests collider j against the block (bounding spheres) for the collider mask.
-__global__ void k_block_info(WorldDev w)
+__device__ inline void block_info_part(const WorldDev &w)
 {
     // Per scene: its dense block range; the last block of the range is the overflow block.
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.s_count; s += gridDim.x * blockDim.x) {
@@ -368,7 +385,7 @@ __global__ void k_block_info(WorldDev w)
     }
 }
 
-__global__ void k_count(WorldDev w)
+__device__ inline void count_part(const WorldDev &w)
 {
     const uint32_t n = w.g->p_in;
     const uint32_t nr = (n + WAVE - 1) / WAVE * WAVE;
@@ -399,6 +416,13 @@ __global__ void k_count(WorldDev w)
         base = __shfl(base, (int)h, WAVE);
         if (valid) w.prank[i] = base + (l - h);
     }
+}
+
+// Block tables and particle counts, both needing only the dense block ids.
+__global__ void k_block_count(WorldDev w)
+{
+    count_part(w);
+    block_info_part(w);
 }
 
 __global__ void k_scatter(WorldDev w)
@@ -451,11 +475,19 @@ __device__ inline int bound_exponent(float b)
     return e;
 }
 
+// round(v * scale) as a two's-complement 64-bit integer, exact for |v * scale| < 2^63:
+// r is an integer-valued float; hi = floor(r / 2^32) and lo = r - hi 2^32 in [0, 2^32)
+// are both exact (lo is a multiple of r's ulp and below 2^32, so it has at most 24
+// significant bits), and r = hi 2^32 + lo.  Non-finite inputs give garbage that the
+// caller's non-finite mask overrides.
 __device__ inline unsigned long long to_fixed(float v, float scale)
 {
-    float x = v * scale;
-    x = fminf(fmaxf(x, -4.6e18f), 4.6e18f);
-    return (unsigned long long)__float2ll_rn(x);
+    const float r = __builtin_rintf(v * scale);
+    const float hi = floorf(r * 0x1p-32f);
+    const float lo = __builtin_fmaf(hi, -0x1p32f, r);
+    const uint32_t uhi = (uint32_t)(int32_t)hi;
+    const uint32_t ulo = (uint32_t)lo;
+    return ((unsigned long long)uhi << 32) | ulo;
 }
 
 __device__ inline float wave_max(float v)
@@ -521,6 +553,23 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                     C.c0 = mk3(w.in.plane(PC0)[slot], w.in.plane(PC1)[slot], w.in.plane(PC2)[slot]);
                     C.c1 = mk3(w.in.plane(PC3)[slot], w.in.plane(PC4)[slot], w.in.plane(PC5)[slot]);
                     C.c2 = mk3(w.in.plane(PC6)[slot], w.in.plane(PC7)[slot], w.in.plane(PC8)[slot]);
+                    // With finite inputs and a finite block bound far below FLT_MAX every
+                    // contribution is finite (|contribution| <= bound), so only suspect
+                    // particles check each contribution for the non-finite mask.
+                    float amax = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+                    amax = fmaxf(amax, fmaxf(fabsf(mom.x), fmaxf(fabsf(mom.y), fabsf(mom.z))));
+                    amax = fmaxf(amax, fabsf(m));
+                    #pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const f3 col = k == 0 ? C.c0 : (k == 1 ? C.c1 : C.c2);
+                        amax = fmaxf(amax, fmaxf(fabsf(col.x), fmaxf(fabsf(col.y), fabsf(col.z))));
+                    }
+                    // fmaxf drops NaN operands: test them separately.
+                    const bool any_nan = px != px || py != py || pz != pz || mom.x != mom.x || mom.y != mom.y ||
+                                         mom.z != mom.z || m != m || C.c0.x != C.c0.x || C.c0.y != C.c0.y ||
+                                         C.c0.z != C.c0.z || C.c1.x != C.c1.x || C.c1.y != C.c1.y ||
+                                         C.c1.z != C.c1.z || C.c2.x != C.c2.x || C.c2.y != C.c2.y || C.c2.z != C.c2.z;
+                    const bool suspect = any_nan || !(amax < 0x1p100f) || !(bmom < 0x1p100f) || !(bm < 0x1p100f);
                     // Base node of the stencil and its offset from the particle.
                     const float cx = roundf(px / h) - 1.0f, cy = roundf(py / h) - 1.0f, cz = roundf(pz / h) - 1.0f;
                     const int bx = (int)cx - 4 * v.x, by = (int)cy - 4 * v.y, bz = (int)cz - 4 * v.z;
@@ -550,7 +599,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                                 atomicAdd(&tile[TILE_NODES + t], to_fixed(vc.y, smom));
                                 atomicAdd(&tile[2 * TILE_NODES + t], to_fixed(vc.z, smom));
                                 atomicAdd(&tile[3 * TILE_NODES + t], to_fixed(mw, sm));
-                                if (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw)) {
+                                if (suspect && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
                                     const int q[4] = {t, TILE_NODES + t, 2 * TILE_NODES + t, 3 * TILE_NODES + t};
                                     #pragma unroll
                                     for (int k = 0; k < 4; ++k) atomicOr(&bad[q[k] >> 5], 1u << (q[k] & 31));
@@ -647,13 +696,48 @@ __device__ __constant__ const int8_t NBH[27][3] = {
     {2, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {1, 0, 0}, {1, 1, 0},
     {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
 
-__global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
+// Node velocities of the particle's 6^3 stencil tile, read from the grid (the 8 blocks
+// vid + {0,1}^3; missing blocks read as zero) ...
+struct GatherGrid {
+    const uint32_t *bup;
+    const float4 *gridv;
+    uint32_t nb[8];
+    __device__ void prepare(uint32_t d)
+    {
+        #pragma unroll
+        for (int o = 0; o < 8; ++o) nb[o] = bup[8 * d + o];
+    }
+    __device__ f3 at(int tx, int ty, int tz) const
+    {
+        const uint32_t blk = nb[(tx >> 2) * 4 + (ty >> 2) * 2 + (tz >> 2)];
+        if (blk == NONE) return mk3(0, 0, 0);
+        const float4 g = gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
+        return mk3(g.x, g.y, g.z);
+    }
+};
+
+// ... or from the block's tile staged in LDS by its wave.
+struct GatherTile {
+    const float4 *tile;
+    __device__ void prepare(uint32_t) {}
+    __device__ f3 at(int tx, int ty, int tz) const
+    {
+        const float4 g = tile[tx + TILE * (ty + TILE * tz)];
+        return mk3(g.x, g.y, g.z);
+    }
+};
+
+// Particles of the overflow block read no nodes.
+struct GatherNone {
+    __device__ void prepare(uint32_t) {}
+    __device__ f3 at(int, int, int) const { return mk3(0, 0, 0); }
+};
+
+template <class Gather>
+__device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, uint32_t d, Gather &gather)
 {
-    uint32_t pbegin, pend;
-    xcd_range(w.g->p_out, pbegin, pend);
-    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
+    {
         const uint32_t slot = w.perm[pos];
-        const uint32_t d = w.sblock[pos];
         const uint32_t s = w.bscene[d];
         const SceneDev sc = w.scenes[s];
         const int4 bv = w.bvid[d];
@@ -697,20 +781,11 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
                     wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
                     wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
                 }
-                uint32_t nb[8];
-                #pragma unroll
-                for (int o = 0; o < 8; ++o) nb[o] = w.bup[8 * d + o];
+                gather.prepare(d);
                 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
                     const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
-                    const int tx = bx + sx, ty = by + sy, tz = bz + sz;
-                    const int ox = tx >> 2, oy = ty >> 2, oz = tz >> 2;
-                    const uint32_t blk = nb[ox * 4 + oy * 2 + oz];
-                    f3 cv = mk3(0, 0, 0);
-                    if (blk != NONE) {
-                        const float4 g = w.gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
-                        cv = mk3(g.x, g.y, g.z);
-                    }
+                    const f3 cv = gather.at(bx + sx, by + sy, bz + sz);
                     const f3 dpt = mk3(ref.x + (float)sx * h, ref.y + (float)sy * h, ref.z + (float)sz * h);
                     const float wgt = wx[sx] * wy[sy] * wz[sz];
                     const float wi = wgt * inv_d;
@@ -787,6 +862,138 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
     }
 }
 
+// The hash regions are not read after the block tables are built: the G2P kernels leave
+// them empty for the next substep.
+__device__ inline void clear_hash(const WorldDev &w)
+{
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < w.h_total; i += gsz) {
+        w.hkeys[i] = EMPTY_KEY;
+        w.hmark[i] = 0u;
+    }
+}
+
+// One thread per sorted particle, nodes read from the grid.
+__global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
+{
+    uint32_t pbegin, pend;
+    xcd_range(w.g->p_out, pbegin, pend);
+    GatherGrid gather{w.bup, w.gridv, {}};
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) g2p_particle(w, pos, w.sblock[pos], gather);
+    clear_hash(w);
+}
+
+// One wave per block: the wave stages the block's 6^3 node tile (velocities of the
+// blocks vid + {0,1}^3, zero where absent) in LDS, then each lane updates one of the
+// block's particles reading its 27 nodes from the tile.  Same arithmetic and order as
+// k_g2p_update.
+__global__ void __launch_bounds__(64) k_g2p_tile(WorldDev w)
+{
+    __shared__ float4 tile[TILE_NODES];
+    uint32_t dbegin, dend;
+    xcd_range(w.g->n_blocks, dbegin, dend);
+    const uint32_t l = threadIdx.x;
+    GatherTile tg{tile};
+    GatherNone none;
+    for (uint32_t d = dbegin; d < dend; ++d) {
+        const uint32_t np = w.bnp[d];
+        if (np == 0u) continue;
+        const int4 v = w.bvid[d];
+        const uint32_t first = w.bfirst[d];
+        if (v.x == OVERFLOW_VID) {
+            for (uint32_t c = l; c < np; c += WAVE) g2p_particle(w, first + c, d, none);
+            continue;
+        }
+        __syncthreads();
+        for (int t = l; t < TILE_NODES; t += WAVE) {
+            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+            const uint32_t blk = w.bup[8 * d + (tx >> 2) * 4 + (ty >> 2) * 2 + (tz >> 2)];
+            tile[t] = blk == NONE ? make_float4(0, 0, 0, 0)
+                                  : w.gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
+        }
+        __syncthreads();
+        for (uint32_t c = l; c < np; c += WAVE) g2p_particle(w, first + c, d, tg);
+    }
+    clear_hash(w);
+}
+
+// ------------------------------------------------------------------------------------
+// CFL timestep bound (timestep_bound.rs:61-111, models/default.rs:268-330,
+// models/utils.rs:157-172) over the input arena: per enabled particle the smaller of the
+// model's sound-speed bound and the velocity bound, as 1e-12 s units (saturating),
+// min-reduced per scene.  Particles of one scene are contiguous, so a wave first
+// reduces over the lanes that share a scene and issues one atomic per scene present.
+__device__ inline uint32_t secs_to_int(float dt)
+{
+    const float x = dt * 1.0e12f;
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967295.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+__device__ inline float particle_dt_bound(const ClassDev &cls, float h, float mass, float vol0, const f3 &v,
+                                          const m3 &F, const m3 &A)
+{
+    const uint32_t tag = cls.tag;
+    float bulk, shear, cfl;
+    if (tag <= 1u) { bulk = cls.w[0] + 2.0f * cls.w[1] / 3.0f; shear = cls.w[1]; cfl = cls.w[2]; }
+    else if (tag <= 3u) { bulk = cls.w[10] + 2.0f * cls.w[11] / 3.0f; shear = cls.w[11]; cfl = cls.w[12]; }
+    else if (tag == 4u) { bulk = cls.w[0] * cls.w[1]; shear = 0.0f; cfl = cls.w[3]; }
+    else { bulk = shear = cfl = 0.0f; }
+    float dt = 0.0f;
+    if (tag <= 4u) {
+        const float density0 = mass / vol0;
+        const float density = density0 / fmaxf(det(F), 1.0e-6f);
+        const float c = sqrtf((bulk + shear * 4.0f / 3.0f) / density);
+        dt = cfl * h / fmaxf(length(v), c);
+    }
+    const float na2 = dot(A.c0, A.c0) + dot(A.c1, A.c1) + dot(A.c2, A.c2);
+    const float norm_b = (h * h) / 4.0f * sqrtf(na2) / mass;
+    const float apic_v = norm_b * 6.0f * sqrtf(3.0f) / h;
+    return fminf(dt, h / (length(v) + apic_v));
+}
+
+__global__ void __launch_bounds__(256) k_timestep_bound(WorldDev w, uint32_t n, uint32_t *out)
+{
+    const uint32_t nr = (n + WAVE - 1) / WAVE * WAVE;
+    const uint32_t l = lane_id();
+    uint32_t cur_s = NONE, cur_min = 0xFFFFFFFFu;     // per lane: running minimum of one scene
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
+        if (i >= n) continue;
+        const uint32_t fl = w.in.uplane(PFLAGS)[i];
+        const uint32_t s = flags_scene(fl);
+        const SceneDev sc = w.scenes[s];
+        if (!sc.active || !(fl & FLAG_ENABLED)) continue;
+        m3 F, A;
+        F.c0 = mk3(w.in.plane(PF0)[i], w.in.plane(PF1)[i], w.in.plane(PF2)[i]);
+        F.c1 = mk3(w.in.plane(PF3)[i], w.in.plane(PF4)[i], w.in.plane(PF5)[i]);
+        F.c2 = mk3(w.in.plane(PF6)[i], w.in.plane(PF7)[i], w.in.plane(PF8)[i]);
+        A.c0 = mk3(w.in.plane(PC0)[i], w.in.plane(PC1)[i], w.in.plane(PC2)[i]);
+        A.c1 = mk3(w.in.plane(PC3)[i], w.in.plane(PC4)[i], w.in.plane(PC5)[i]);
+        A.c2 = mk3(w.in.plane(PC6)[i], w.in.plane(PC7)[i], w.in.plane(PC8)[i]);
+        const f3 v = mk3(w.in.plane(PVX)[i], w.in.plane(PVY)[i], w.in.plane(PVZ)[i]);
+        const ClassDev &cls = w.classes[sc.class_base + flags_class(fl)];
+        const uint32_t cand = secs_to_int(particle_dt_bound(cls, sc.h, w.in.plane(PMASS)[i], w.in.plane(PVOL0)[i], v, F, A));
+        if (s != cur_s) {
+            if (cur_s != NONE) atomicMin(&out[cur_s], cur_min);
+            cur_s = s;
+            cur_min = cand;
+        } else {
+            cur_min = min(cur_min, cand);
+        }
+    }
+    // Wave reduction per scene: one atomic per distinct scene held by the wave's lanes.
+    uint64_t pending = __ballot(cur_s != NONE);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t s = __shfl(cur_s, (int)leader, WAVE);
+        uint32_t v = (cur_s == s) ? cur_min : 0xFFFFFFFFu;
+        for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, WAVE));
+        if (l == leader) atomicMin(&out[s], v);
+        pending &= ~__ballot(cur_s == s);
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Canonical sort readback (not on the hot path): per sorted particle its primary
 // block and local cell, and the ten bucket counts of every block including the
@@ -836,27 +1043,32 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     const int T = 256;
     const int cap = lc.persistent_blocks;
     if (ev) (void)hipEventRecord(ev[0], st);
-    k_clear<<<grid_for(std::max(w.h_total, w.b_total), T, cap), T, 0, st>>>(w);
     k_keys_touch<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
-    k_touch_nbrs<<<grid_for((uint64_t)w.h_total * 8, T, cap), T, 0, st>>>(w);
     const int hscan = std::min((int)((w.h_total + SCAN_TILE - 1) / SCAN_TILE), cap);
-    k_scan_reduce<MODE_SLOTS><<<hscan, 256, 0, st>>>(w);
-    k_scan_tiles<MODE_SLOTS><<<1, 256, 0, st>>>(w);
-    k_scan_down<MODE_SLOTS><<<hscan, 256, 0, st>>>(w);
-    k_block_info<<<grid_for((uint64_t)w.b_total * 16, T, cap), T, 0, st>>>(w);
-    k_count<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    k_scan<MODE_SLOTS><<<std::max(hscan, 1), 256, 0, st>>>(w);
+    k_block_count<<<grid_for(std::max<uint64_t>(lc.p_hint, (uint64_t)w.b_total * 16), T, cap), T, 0, st>>>(w);
     const int bscan = std::min((int)((w.b_total + SCAN_TILE - 1) / SCAN_TILE), cap);
-    k_scan_reduce<MODE_COUNTS><<<bscan, 256, 0, st>>>(w);
-    k_scan_tiles<MODE_COUNTS><<<1, 256, 0, st>>>(w);
-    k_scan_down<MODE_COUNTS><<<bscan, 256, 0, st>>>(w);
+    k_scan<MODE_COUNTS><<<std::max(bscan, 1), 256, 0, st>>>(w);
     k_scatter<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[1], st);
     k_p2g<<<lc.p2g_blocks, 64, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[2], st);
     k_grid_update<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_g2p_update<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    if (lc.g2p_tile) k_g2p_tile<<<lc.p2g_blocks, 64, 0, st>>>(w);
+    else k_g2p_update<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[4], st);
+}
+
+void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st)
+{
+    const int T = 256;
+    k_clear<<<grid_for(std::max(std::max(w.h_total, w.b_total), 2u * w.scan_stride), T, lc.persistent_blocks), T, 0, st>>>(w);
+}
+
+void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st)
+{
+    k_timestep_bound<<<grid_for(n, 256, max_blocks), 256, 0, st>>>(w, n, out);
 }
 
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,

@@ -40,7 +40,7 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
-    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
@@ -109,6 +109,8 @@ def library() -> ct.CDLL:
     L.nxm_scene_device_bytes.restype = ct.c_size_t
     L.nxm_scene_set_graph.argtypes = [P, ct.c_int]
     L.nxm_scene_set_graph.restype = ct.c_int
+    L.nxm_scene_timestep_bound.argtypes = [P, ct.POINTER(ct.c_float)]
+    L.nxm_scene_timestep_bound.restype = ct.c_int
     L.nxm_scene_read_positions.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_sort_size.argtypes = [P, ct.POINTER(ct.c_size_t)]
@@ -247,6 +249,12 @@ class MpmState:
         """Standalone steps replay one captured HIP graph per substep."""
         _check(self.lib.nxm_scene_set_graph(self.handle, int(enabled)))
 
+    def timestep_bound(self) -> float:
+        """CFL bound on the substep length of the current state (WgTimestepBounds)."""
+        dt = ct.c_float(0.0)
+        _check(self.lib.nxm_scene_timestep_bound(self.handle, ct.byref(dt)))
+        return float(dt.value)
+
     def add_particles(self, scene: "_scenes.Scene") -> None:
         arrs = dict(
             position=np.ascontiguousarray(scene.x, np.float32), velocity=np.ascontiguousarray(scene.v, np.float32),
@@ -332,6 +340,7 @@ class Batch:
 
     def __init__(self, pipeline: MpmPipeline):
         self.lib = pipeline.lib
+        self.pipeline = pipeline
         h = ct.c_void_p()
         _check(self.lib.nxm_batch_create(pipeline.handle, ct.byref(h)))
         self.handle = h

@@ -1010,3 +1010,61 @@ int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid, uin
     for (int64_t p = 0; p < n; ++p) particle_block(vload(x + 3 * p), (REAL)cell_width, vid + 3 * p, local + 3 * p);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* CFL timestep bound (src_mpm_shaders/solver/timestep_bound.rs:61-111): per enabled
+ * particle the smaller of the model's sound-speed bound (models/default.rs:268-330,
+ * models/utils.rs:157-172: cfl h / max(|v|, sqrt((K + 4/3 G) / (rho0 / max(det F, 1e-6)))),
+ * K = lambda + 2 mu / 3 and G = mu for the elastic and sand tags (utils.rs:28-36), K = bulk
+ * gamma and G = 0 for the fluid (fluid.rs:88-106)) and the velocity bound
+ * h / (|v| + (h^2/4) |A|_F / m * 6 sqrt(3) / h); the minimum over particles of
+ * (u32)(dt * 1e12) (saturating, as Rust's `as`), u32::MAX when none is enabled. */
+static uint32_t secs_to_int(REAL dt)
+{
+    REAL x = dt * (REAL)1.0e12f;
+    if (!(x > (REAL)0)) return 0u;               /* NaN and negatives saturate to 0 */
+    if (x >= (REAL)4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+uint32_t orc_timestep_bound(int64_t n, const REAL *v, const REAL *C, const REAL *F, const REAL *mass,
+                            const REAL *vol0, const uint32_t *enabled, const uint32_t *tag, const float *params,
+                            float cell_width)
+{
+    const REAL h = (REAL)cell_width;
+    uint32_t best = 0xFFFFFFFFu;
+    for (int64_t p = 0; p < n; ++p) {
+        if (!enabled[p]) continue;
+        const float *w = params + 24 * p;
+        REAL bulk, shear, cfl;
+        switch (tag[p]) {
+        case 0: case 1: bulk = (REAL)w[0] + K(2.0) * (REAL)w[1] / K(3.0); shear = (REAL)w[1]; cfl = (REAL)w[2]; break;
+        case 2: case 3: bulk = (REAL)w[10] + K(2.0) * (REAL)w[11] / K(3.0); shear = (REAL)w[11]; cfl = (REAL)w[12]; break;
+        case 4: bulk = (REAL)w[0] * (REAL)w[1]; shear = K(0.0); cfl = (REAL)w[3]; break;
+        default: bulk = shear = cfl = K(0.0); break;
+        }
+        const m3 Fm = mload(F + 9 * p);
+        const v3 vel = vload(v + 3 * p);
+        REAL dt;
+        if (tag[p] <= 4u) {
+            const REAL density0 = mass[p] / vol0[p];
+            const REAL det = mdet(Fm);
+            const REAL density = density0 / RFMAX(det, (REAL)1.0e-6f);
+            const REAL c = RSQRT((bulk + shear * K(4.0) / K(3.0)) / density);
+            const REAL speed = RFMAX(vlen(vel), c);
+            dt = cfl * h / speed;
+        } else {
+            dt = K(0.0);
+        }
+        const m3 A = mload(C + 9 * p);
+        const REAL na2 = vdot(A.c0, A.c0) + vdot(A.c1, A.c1) + vdot(A.c2, A.c2);
+        const REAL d = (h * h) / K(4.0);
+        const REAL norm_b = d * RSQRT(na2) / mass[p];
+        const REAL apic_v = norm_b * K(6.0) * RSQRT(K(3.0)) / h;
+        const REAL vv = vlen(vel) + apic_v;
+        dt = RFMIN(dt, h / vv);
+        const uint32_t cand = secs_to_int(dt);
+        if (cand < best) best = cand;
+    }
+    return best;
+}

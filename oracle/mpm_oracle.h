@@ -81,6 +81,11 @@ int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim,
                      const REAL *node_pos, REAL *vel);
 int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid,
                    uint32_t *local);
+/* CFL timestep bound over the enabled particles, in units of 1e-12 s (the reference's
+ * GpuTimestepBounds::computed_max_dt_as_uint); u32::MAX when none is enabled. */
+uint32_t orc_timestep_bound(int64_t n, const REAL *v, const REAL *C, const REAL *F, const REAL *mass,
+                            const REAL *vol0, const uint32_t *enabled, const uint32_t *tag, const float *params,
+                            float cell_width);
 
 #ifdef __cplusplus
 }

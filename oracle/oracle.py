@@ -47,6 +47,8 @@ def lib(bits: int = 32) -> ct.CDLL:
     L.orc_project_cuboid.argtypes = [P] * 4
     L.orc_grid_node_bc.argtypes = [P, ct.c_int, P, P, P]
     L.orc_block_keys.argtypes = [ct.c_int64, P, ct.c_float, P, P]
+    L.orc_timestep_bound.argtypes = [ct.c_int64] + [P] * 8 + [ct.c_float]
+    L.orc_timestep_bound.restype = ct.c_uint32
     _LIBS[bits] = L
     return L
 
@@ -220,3 +222,12 @@ def block_keys(x: np.ndarray, cell_width: float, bits: int = 32):
     loc = np.zeros((n, 3), np.uint32)
     lib(bits).orc_block_keys(n, _p(xc), ct.c_float(cell_width), _p(vid), _p(loc))
     return vid, loc
+
+
+def timestep_bound(state: "OracleState") -> tuple[int, float]:
+    """CFL timestep bound of the oracle's current particle state: the reference's
+    fixed-point value (1e-12 s units) and the seconds it stands for."""
+    s = state
+    u = lib(s.bits).orc_timestep_bound(s.n, _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.enabled),
+                                       _p(s.tag), _p(s.params), ct.c_float(float(s.sim[4])))
+    return int(u), float(np.float32(u) / np.float32(1.0e12))

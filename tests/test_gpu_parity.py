@@ -138,7 +138,8 @@ def test_graph_replay_matches_eager(pipeline):
     graphed.close()
 
     extra = scenes.elastic_cube(8)
-    ref = [run_gpu(pipeline, s, 4).read_particles() for s in (sc, extra)]
+    ref1 = run_gpu(pipeline, sc, 6).read_particles()
+    ref2 = run_gpu(pipeline, extra, 2).read_particles()
     batch = mpm.Batch(pipeline)
     batch.set_graph(True)
     s1 = mpm.MpmState.from_scene(pipeline, sc)
@@ -147,10 +148,53 @@ def test_graph_replay_matches_eager(pipeline):
     s2 = mpm.MpmState.from_scene(pipeline, extra)
     batch.add(s2)                      # new tables: recapture
     batch.run(2)
-    got2 = s2.read_particles()
+    got2 = s2.read_particles()         # 2 substeps in the batch
     batch.run(2)
-    got1 = s1.read_particles()
+    got1 = s1.read_particles()         # 6 substeps
     for f in ("x", "v", "C", "F"):
-        assert np.array_equal(got1[f], run_gpu(pipeline, sc, 6).read_particles()[f]), f
-        assert np.array_equal(got2[f], ref[1][f]), f
+        assert np.array_equal(got1[f], ref1[f]), f
+        assert np.array_equal(got2[f], ref2[f]), f
     batch.close()
+
+
+@pytest.mark.parametrize("make", [lambda: scenes.elastic_cube(12), lambda: small_sand(8000, seed=31),
+                                  lambda: small_fluid(12000, seed=32)], ids=["cube", "sand", "fluid"])
+def test_g2p_kernels_identical(monkeypatch, make):
+    """The two G2P kernels (nodes gathered from the grid, or from a per-block LDS tile)
+    perform the same arithmetic in the same order: bit-identical results."""
+    sc = make()
+    out = {}
+    for variant in ("grid", "tile"):
+        monkeypatch.setenv("NXM_G2P", variant)
+        pipe = mpm.MpmPipeline(0)
+        st = run_gpu(pipe, sc, 4)
+        out[variant] = st.read_particles()
+        st.close()
+        pipe.close()
+    for f in ("x", "v", "C", "F", "enabled"):
+        assert np.array_equal(out["grid"][f], out["tile"][f]), f
+
+
+@pytest.mark.parametrize("make", [lambda: scenes.elastic_cube(12), lambda: small_sand(8000, seed=41),
+                                  lambda: small_fluid(12000, seed=42)], ids=["cube", "sand", "fluid"])
+def test_timestep_bound_matches_oracle(pipeline, make):
+    """The CFL bound of the GPU state equals the oracle's FP32 bound of the same state
+    (the reference's 1e-12 s fixed point, within FP32 rounding), alone and in a batch."""
+    import oracle as O
+    sc = make()
+    st = run_gpu(pipeline, sc, 3)
+    p = st.read_particles()
+    o = O.Oracle(sc, 32)
+    for f in ("v", "C", "F", "mass", "enabled"):
+        getattr(o.s, f)[...] = p[f]
+    _, secs = O.timestep_bound(o.s)
+    got = st.timestep_bound()
+    assert abs(got - secs) <= 1e-5 * secs, (got, secs)
+    other = mpm.MpmState.from_scene(pipeline, scenes.elastic_cube(6))
+    batch = mpm.Batch(pipeline)
+    batch.add(other)
+    batch.add(st)
+    assert st.timestep_bound() == got
+    batch.close()
+    other.close()
+    st.close()

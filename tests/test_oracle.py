@@ -385,3 +385,73 @@ def test_fp32_mirror_tracks_fp64():
         b.step(1)
         assert np.max(np.abs(a.s.x - b.s.x)) < 1e-6
         assert np.max(np.abs(a.s.F - b.s.F)) < 1e-5
+
+
+# ---------------------------------------------------------------------------- CFL bound
+
+def _one_particle(model, density=1000.0, radius=0.01, velocity=(0.0, 0.0, 0.0), h=1.0 / 64):
+    parts = scenes.particles(np.array([[0.5, 0.5, 0.5]]), radius, density, model,
+                             velocity=np.array([velocity], np.float64))
+    return scenes.make_scene(parts, h, 1e-4)
+
+
+@pytest.mark.parametrize("young,poisson", [(1e5, 0.3), (3.5e5, 0.25), (2e6, 0.45)])
+def test_timestep_bound_is_p_wave_cfl(young, poisson):
+    """At rest the bound is cfl h / c with c the P-wave speed sqrt(M / rho), where the
+    P-wave modulus is the textbook M = E (1 - nu) / ((1 + nu)(1 - 2 nu)) (not the
+    lambda/mu route the model takes).  Neo-Hookean (cfl word 0.5) and sand (elastic
+    backbone) agree; the result is in 1e-12 s units."""
+    h = 1.0 / 64
+    M = young * (1 - poisson) / ((1 + poisson) * (1 - 2 * poisson))
+    for model, rho in ((scenes.model_neo_hookean(young, poisson), 1000.0),
+                       (scenes.model_sand(young, poisson, friction_deg=30.0), 2200.0)):
+        sc = _one_particle(model, density=rho, h=h)
+        o = O.Oracle(sc, 64)
+        cfl = 0.5
+        expect = cfl * h / math.sqrt(M / rho)
+        u, secs = O.timestep_bound(o.s)
+        assert abs(u - expect * 1e12) <= 1e-6 * expect * 1e12 + 1, (u, expect * 1e12)
+
+
+def test_timestep_bound_fluid_tait_sound_speed_and_velocity():
+    """Fluid: the Tait law p = K ((rho/rho0)^gamma - 1) has dp/drho = K gamma / rho0 at
+    rest, so c = sqrt(K gamma / rho0).  A particle faster than c bounds dt by cfl h / |v|
+    (cfl = 0.5 < 1, so the sound-speed term is the smaller one); compression (det F < 1)
+    raises the density and lowers c."""
+    h = 1.0 / 128
+    K, gamma, rho = 1e4, 7.0, 1000.0
+    c = math.sqrt(K * gamma / rho)
+    sc = _one_particle(scenes.model_fluid(K, gamma), density=rho, h=h)
+    o = O.Oracle(sc, 64)
+    assert abs(O.timestep_bound(o.s)[0] - 0.5 * h / c * 1e12) <= 1e-6 * 0.5 * h / c * 1e12 + 1
+    v = 3.0 * c
+    sc = _one_particle(scenes.model_fluid(K, gamma), density=rho, h=h, velocity=(v, 0.0, 0.0))
+    o = O.Oracle(sc, 64)
+    assert abs(O.timestep_bound(o.s)[0] - 0.5 * h / v * 1e12) <= 1e-6 * 0.5 * h / v * 1e12 + 1
+    sc = _one_particle(scenes.model_fluid(K, gamma), density=rho, h=h)
+    o = O.Oracle(sc, 64)
+    J = 0.5
+    o.s.F[0] = np.diag([J, 1.0, 1.0]).reshape(-1)
+    expect = 0.5 * h / math.sqrt(K * gamma / (rho / J))
+    assert abs(O.timestep_bound(o.s)[0] - expect * 1e12) <= 1e-6 * expect * 1e12 + 1
+
+
+def test_timestep_bound_minimum_and_disabled():
+    """The bound is the minimum over enabled particles; disabled particles are skipped
+    and an empty set leaves the reset value u32::MAX."""
+    h = 1.0 / 64
+    a = _one_particle(scenes.model_neo_hookean(1e5, 0.3), h=h)
+    b = _one_particle(scenes.model_neo_hookean(4e5, 0.3), h=h)
+    both = scenes.make_scene(scenes.concat([
+        {k: getattr(a, k) for k in ("x", "v", "F", "C", "mass", "vol0", "radius", "damping", "enabled", "fixed",
+                                    "tag", "model")},
+        {k: getattr(b, k) for k in ("x", "v", "F", "C", "mass", "vol0", "radius", "damping", "enabled", "fixed",
+                                    "tag", "model")}]), h, 1e-4)
+    ua = O.timestep_bound(O.Oracle(a, 64).s)[0]
+    ub = O.timestep_bound(O.Oracle(b, 64).s)[0]
+    o = O.Oracle(both, 64)
+    assert O.timestep_bound(o.s)[0] == min(ua, ub) == ub      # stiffer: faster sound, smaller dt
+    o.s.enabled[1] = 0
+    assert O.timestep_bound(o.s)[0] == ua
+    o.s.enabled[:] = 0
+    assert O.timestep_bound(o.s)[0] == 0xFFFFFFFF
