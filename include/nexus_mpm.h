@@ -179,6 +179,12 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa
  * non-CPIC (use_cpic = false). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
 size_t nxm_scene_len(const nxm_scene *scene);
+/* GpuParticles::swap_remove (src_mpm/solver/particle.rs:575-631): removes the particles
+ * with the given ids (duplicates and ids >= len ignored); processing ids in descending
+ * order, the current last particle moves into each freed id.  The moves are reported as
+ * (moved_from[k], moved_to[k]), k < *n_moved (arrays sized n, may be NULL).  Synchronizes. */
+nxm_status nxm_scene_swap_remove(nxm_scene *scene, const uint32_t *ids, size_t n, uint32_t *moved_from,
+                                 uint32_t *moved_to, size_t *n_moved);
 /* CFL-based upper bound on the substep length of the scene's current state:
  * WgTimestepBounds::compute_bounds (src_mpm/solver/timestep_bound.rs:25-54), kernel
  * src_mpm_shaders/solver/timestep_bound.rs:61-111.  Same fixed-point result (1e-12 s

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -80,7 +81,6 @@ using namespace nxm;
 struct nxm_ctx {
     int device = 0;
     int num_cus = 256;
-    bool g2p_tile = false;   // G2P kernel variant (NXM_G2P=tile selects the LDS-tile kernel)
     // Scenes and batches created from this context; nxm_ctx_destroy defers the delete
     // until the last of them is destroyed.
     std::atomic<int> refs{0};
@@ -117,6 +117,17 @@ struct HostParticles {
         group.resize(m); tag.resize(m); words.resize(24 * m); cls.resize(m);
     }
 };
+
+// Copies every per-particle array entry of particle `from` to `to`.
+static void move_particle(HostParticles &hp, size_t from, size_t to)
+{
+    auto mv = [&](auto &vec, size_t k) {
+        for (size_t c = 0; c < k; ++c) vec[k * to + c] = vec[k * from + c];
+    };
+    mv(hp.x, 3); mv(hp.v, 3); mv(hp.F, 9); mv(hp.C, 9); mv(hp.mass, 1); mv(hp.vol0, 1); mv(hp.pstate, 3);
+    mv(hp.enabled, 1); mv(hp.fixed, 1); mv(hp.radius, 1); mv(hp.phase, 1); mv(hp.bfric, 1); mv(hp.damping, 1);
+    mv(hp.group, 1); mv(hp.tag, 1); mv(hp.words, 24); mv(hp.cls, 1);
+}
 
 struct SceneRange {
     uint32_t start, count;
@@ -354,7 +365,6 @@ class World {
         lc.p_hint = p_in_;
         lc.persistent_blocks = ctx_->num_cus * 8;
         lc.p2g_blocks = ctx_->num_cus * 32;
-        lc.g2p_tile = ctx_->g2p_tile;
         hipEvent_t *evs = times ? ev_ : nullptr;
         if (!times && profiling_) {
             nxm_status st = ring_events(&evs);
@@ -970,8 +980,6 @@ nxm_status nxm_ctx_create(int device, nxm_ctx **out)
     auto *c = new nxm_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    const char *g2p = std::getenv("NXM_G2P");
-    c->g2p_tile = g2p && std::strcmp(g2p, "tile") == 0;
     *out = c;
     return NXM_OK;
 }
@@ -1131,6 +1139,38 @@ nxm_status nxm_scene_read_particles_soa(nxm_scene *sc, const nxm_particles_soa *
         }
     }
     return latched(sc, NXM_OK);
+}
+
+nxm_status nxm_scene_swap_remove(nxm_scene *sc, const uint32_t *ids, size_t n, uint32_t *moved_from,
+                                 uint32_t *moved_to, size_t *n_moved)
+{
+    if (!sc || (!ids && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    World *w = sc->world;
+    if (w) {
+        // Bring the state to the host and leave the world; the scene re-enters it below.
+        nxm_status st = w->remove(sc);
+        if (st != NXM_OK) return st;
+    }
+    std::vector<uint32_t> targets(ids, ids + n);
+    std::sort(targets.begin(), targets.end(), std::greater<uint32_t>());
+    targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
+    HostParticles &hp = sc->hp;
+    size_t len = hp.n, moved = 0;
+    for (uint32_t id : targets) {
+        if (id >= len) continue;
+        const size_t last = len - 1;
+        if (id != last) {
+            move_particle(hp, last, id);
+            if (moved_from) moved_from[moved] = (uint32_t)last;
+            if (moved_to) moved_to[moved] = id;
+            ++moved;
+        }
+        --len;
+    }
+    hp.resize(len);
+    if (n_moved) *n_moved = moved;
+    if (w) return w->add(sc);
+    return NXM_OK;
 }
 
 nxm_status nxm_scene_timestep_bound(nxm_scene *sc, float *max_dt)

@@ -43,7 +43,6 @@ struct LaunchConfig {
     uint64_t p_hint;               // upper bound of particles (grid sizing only)
     int persistent_blocks;         // cap on workgroups of grid-stride kernels
     int p2g_blocks;                // workgroups of the per-block kernels
-    bool g2p_tile;                 // G2P through a per-block LDS node tile
 };
 
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev);

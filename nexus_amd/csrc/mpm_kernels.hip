@@ -52,9 +52,9 @@ __device__ inline void xcd_range(uint32_t n, uint32_t &begin, uint32_t &end)
 {
     const uint32_t g = gridDim.x, b = blockIdx.x;
     const uint32_t logical = (g % NUM_XCD == 0) ? (b % NUM_XCD) * (g / NUM_XCD) + b / NUM_XCD : b;
-    const uint32_t per = (n + g - 1) / g;
-    begin = min(n, logical * per);
-    end = min(n, begin + per);
+    // Balanced split: every workgroup gets floor(n / g) or ceil(n / g) items.
+    begin = (uint32_t)(((uint64_t)logical * n) / g);
+    end = (uint32_t)(((uint64_t)(logical + 1) * n) / g);
 }
 
 // ------------------------------------------------------------------------------------
@@ -454,11 +454,32 @@ __global__ void k_scatter(WorldDev w)
 // contributions down to 2^-62 np of the largest.  Integer sums do not depend on the
 // order of the adds, so the tile is bit-reproducible whatever the particle order.
 
-__device__ inline float fixed_scale(int e, uint32_t np)
+// 64-bit fixed-point accumulator words of the P2G tile.
+struct Fixed {
+    using word = unsigned long long;
+    static constexpr int BITS = 62;
+    // round(v * scale) as a two's-complement 64-bit integer, exact for |v * scale| < 2^63:
+    // r is an integer-valued float; hi = floor(r / 2^32) and lo = r - hi 2^32 in [0, 2^32)
+    // are both exact (lo is a multiple of r's ulp and below 2^32, so it has at most 24
+    // significant bits), and r = hi 2^32 + lo.  Non-finite inputs give garbage that the
+    // caller's non-finite mask overrides.
+    __device__ static word to(float v, float scale)
+    {
+        const float r = __builtin_rintf(v * scale);
+        const float hi = floorf(r * 0x1p-32f);
+        const float lo = __builtin_fmaf(hi, -0x1p32f, r);
+        const uint32_t uhi = (uint32_t)(int32_t)hi;
+        const uint32_t ulo = (uint32_t)lo;
+        return ((unsigned long long)uhi << 32) | ulo;
+    }
+    __device__ static float back(word x, float scale) { return (float)((double)(long long)x / (double)scale); }
+};
+
+__device__ inline float fixed_scale(int bits, int e, uint32_t np)
 {
     int lg = 0;
     while ((1u << lg) < np && lg < 31) ++lg;
-    int k = 62 - e - lg;
+    int k = bits - e - lg;
     k = k > 126 ? 126 : (k < -126 ? -126 : k);
     return ldexpf(1.0f, k);
 }
@@ -475,20 +496,6 @@ __device__ inline int bound_exponent(float b)
     return e;
 }
 
-// round(v * scale) as a two's-complement 64-bit integer, exact for |v * scale| < 2^63:
-// r is an integer-valued float; hi = floor(r / 2^32) and lo = r - hi 2^32 in [0, 2^32)
-// are both exact (lo is a multiple of r's ulp and below 2^32, so it has at most 24
-// significant bits), and r = hi 2^32 + lo.  Non-finite inputs give garbage that the
-// caller's non-finite mask overrides.
-__device__ inline unsigned long long to_fixed(float v, float scale)
-{
-    const float r = __builtin_rintf(v * scale);
-    const float hi = floorf(r * 0x1p-32f);
-    const float lo = __builtin_fmaf(hi, -0x1p32f, r);
-    const uint32_t uhi = (uint32_t)(int32_t)hi;
-    const uint32_t ulo = (uint32_t)lo;
-    return ((unsigned long long)uhi << 32) | ulo;
-}
 
 __device__ inline float wave_max(float v)
 {
@@ -498,7 +505,8 @@ __device__ inline float wave_max(float v)
 
 __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 {
-    __shared__ unsigned long long tile[4 * TILE_NODES];
+    using FX = Fixed;
+    __shared__ typename FX::word tile[4 * TILE_NODES];
     __shared__ uint32_t bad[(4 * TILE_NODES + 31) / 32];   // non-finite contributions
     const uint32_t l = threadIdx.x;
     const float *X = w.in.plane(PX), *Y = w.in.plane(PY), *Z = w.in.plane(PZ);
@@ -534,9 +542,9 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         }
         bmom = wave_max(bmom);
         bm = wave_max(bm);
-        const float smom = fixed_scale(bound_exponent(bmom), np);
-        const float sm = fixed_scale(bound_exponent(bm), np);
-        for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0ull;
+        const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
+        const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
+        for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
         if (l < (4 * TILE_NODES + 31) / 32) bad[l] = 0u;
         __syncthreads();
         // Pass 2: scatter.
@@ -584,21 +592,36 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                         wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
                         wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
                     }
+                    // C dpt + m v at node (sx, sy, sz) = (C d0 + m v) + sx (h C.c0) + sy (h C.c1)
+                    // + sz (h C.c2): three adds per node instead of a matrix product.  The
+                    // fixed-point scales are powers of two, so folding them into the weight
+                    // changes no bits.
+                    const f3 base = mul(C, d0) + mom;
+                    const f3 hx = C.c0 * h, hy = C.c1 * h, hz = C.c2 * h;
+                    float wxy[3][3];
+                    #pragma unroll
+                    for (int sy = 0; sy < 3; ++sy) {
+                        #pragma unroll
+                        for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
+                    }
+                    const float msm = m * sm;
                     #pragma unroll
                     for (int sz = 0; sz < 3 && inside; ++sz) {
+                        const f3 az = sz == 0 ? base : (sz == 1 ? base + hz : base + hz * 2.0f);
                         #pragma unroll
                         for (int sy = 0; sy < 3; ++sy) {
+                            const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
                             #pragma unroll
                             for (int sx = 0; sx < 3; ++sx) {
-                                const float wgt = wx[sx] * wy[sy] * wz[sz];
-                                const f3 dpt = mk3(d0.x + (float)sx * h, d0.y + (float)sy * h, d0.z + (float)sz * h);
-                                const f3 vc = (mul(C, dpt) + mom) * wgt;
-                                const float mw = m * wgt;
+                                const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
+                                const float wgt = wxy[sy][sx] * wz[sz];
+                                const f3 vc = a * (wgt * smom);
+                                const float mw = msm * wgt;
                                 const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
-                                atomicAdd(&tile[t], to_fixed(vc.x, smom));
-                                atomicAdd(&tile[TILE_NODES + t], to_fixed(vc.y, smom));
-                                atomicAdd(&tile[2 * TILE_NODES + t], to_fixed(vc.z, smom));
-                                atomicAdd(&tile[3 * TILE_NODES + t], to_fixed(mw, sm));
+                                atomicAdd(&tile[t], FX::to(vc.x, 1.0f));
+                                atomicAdd(&tile[TILE_NODES + t], FX::to(vc.y, 1.0f));
+                                atomicAdd(&tile[2 * TILE_NODES + t], FX::to(vc.z, 1.0f));
+                                atomicAdd(&tile[3 * TILE_NODES + t], FX::to(mw, 1.0f));
                                 if (suspect && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
                                     const int q[4] = {t, TILE_NODES + t, 2 * TILE_NODES + t, 3 * TILE_NODES + t};
                                     #pragma unroll
@@ -615,7 +638,6 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         // (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that block's
         // node order, so the consumer reads each region as one run of float4.
         float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
-        const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;
         for (int t = l; t < TILE_NODES; t += WAVE) {
             const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
             const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
@@ -626,7 +648,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int k = c * TILE_NODES + t;
-                float val = (float)((double)(long long)tile[k] * (c < 3 ? inv_smom : inv_sm));
+                float val = FX::back(tile[k], c < 3 ? smom : sm);
                 if (bad[k >> 5] & (1u << (k & 31))) val = __builtin_nanf("");
                 v4[c] = val;
             }
@@ -697,7 +719,7 @@ __device__ __constant__ const int8_t NBH[27][3] = {
     {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
 
 // Node velocities of the particle's 6^3 stencil tile, read from the grid (the 8 blocks
-// vid + {0,1}^3; missing blocks read as zero) ...
+// vid + {0,1}^3; missing blocks read as zero).
 struct GatherGrid {
     const uint32_t *bup;
     const float4 *gridv;
@@ -714,23 +736,6 @@ struct GatherGrid {
         const float4 g = gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
         return mk3(g.x, g.y, g.z);
     }
-};
-
-// ... or from the block's tile staged in LDS by its wave.
-struct GatherTile {
-    const float4 *tile;
-    __device__ void prepare(uint32_t) {}
-    __device__ f3 at(int tx, int ty, int tz) const
-    {
-        const float4 g = tile[tx + TILE * (ty + TILE * tz)];
-        return mk3(g.x, g.y, g.z);
-    }
-};
-
-// Particles of the overflow block read no nodes.
-struct GatherNone {
-    __device__ void prepare(uint32_t) {}
-    __device__ f3 at(int, int, int) const { return mk3(0, 0, 0); }
 };
 
 template <class Gather>
@@ -883,39 +888,7 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
     clear_hash(w);
 }
 
-// One wave per block: the wave stages the block's 6^3 node tile (velocities of the
-// blocks vid + {0,1}^3, zero where absent) in LDS, then each lane updates one of the
-// block's particles reading its 27 nodes from the tile.  Same arithmetic and order as
-// k_g2p_update.
-__global__ void __launch_bounds__(64) k_g2p_tile(WorldDev w)
-{
-    __shared__ float4 tile[TILE_NODES];
-    uint32_t dbegin, dend;
-    xcd_range(w.g->n_blocks, dbegin, dend);
-    const uint32_t l = threadIdx.x;
-    GatherTile tg{tile};
-    GatherNone none;
-    for (uint32_t d = dbegin; d < dend; ++d) {
-        const uint32_t np = w.bnp[d];
-        if (np == 0u) continue;
-        const int4 v = w.bvid[d];
-        const uint32_t first = w.bfirst[d];
-        if (v.x == OVERFLOW_VID) {
-            for (uint32_t c = l; c < np; c += WAVE) g2p_particle(w, first + c, d, none);
-            continue;
-        }
-        __syncthreads();
-        for (int t = l; t < TILE_NODES; t += WAVE) {
-            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
-            const uint32_t blk = w.bup[8 * d + (tx >> 2) * 4 + (ty >> 2) * 2 + (tz >> 2)];
-            tile[t] = blk == NONE ? make_float4(0, 0, 0, 0)
-                                  : w.gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
-        }
-        __syncthreads();
-        for (uint32_t c = l; c < np; c += WAVE) g2p_particle(w, first + c, d, tg);
-    }
-    clear_hash(w);
-}
+
 
 // ------------------------------------------------------------------------------------
 // CFL timestep bound (timestep_bound.rs:61-111, models/default.rs:268-330,
@@ -1055,8 +1028,7 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (ev) (void)hipEventRecord(ev[2], st);
     k_grid_update<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[3], st);
-    if (lc.g2p_tile) k_g2p_tile<<<lc.p2g_blocks, 64, 0, st>>>(w);
-    else k_g2p_update<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    k_g2p_update<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
-    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
@@ -111,6 +111,8 @@ def library() -> ct.CDLL:
     L.nxm_scene_set_graph.restype = ct.c_int
     L.nxm_scene_timestep_bound.argtypes = [P, ct.POINTER(ct.c_float)]
     L.nxm_scene_timestep_bound.restype = ct.c_int
+    L.nxm_scene_swap_remove.argtypes = [P, P, ct.c_size_t, P, P, ct.POINTER(ct.c_size_t)]
+    L.nxm_scene_swap_remove.restype = ct.c_int
     L.nxm_scene_read_positions.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_sort_size.argtypes = [P, ct.POINTER(ct.c_size_t)]
@@ -248,6 +250,17 @@ class MpmState:
     def set_graph(self, enabled: bool = True) -> None:
         """Standalone steps replay one captured HIP graph per substep."""
         _check(self.lib.nxm_scene_set_graph(self.handle, int(enabled)))
+
+    def swap_remove(self, ids) -> list[tuple[int, int]]:
+        """GpuParticles::swap_remove: removes particles by id; the last particle moves
+        into each freed id.  Returns the (from, to) moves."""
+        a = np.ascontiguousarray(ids, np.uint32)
+        f = np.zeros(max(a.size, 1), np.uint32)
+        t = np.zeros(max(a.size, 1), np.uint32)
+        k = ct.c_size_t(0)
+        _check(self.lib.nxm_scene_swap_remove(self.handle, _ptr(a) if a.size else None, a.size, _ptr(f), _ptr(t),
+                                              ct.byref(k)))
+        return [(int(f[i]), int(t[i])) for i in range(k.value)]
 
     def timestep_bound(self) -> float:
         """CFL bound on the substep length of the current state (WgTimestepBounds)."""

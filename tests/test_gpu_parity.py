@@ -157,24 +157,6 @@ def test_graph_replay_matches_eager(pipeline):
     batch.close()
 
 
-@pytest.mark.parametrize("make", [lambda: scenes.elastic_cube(12), lambda: small_sand(8000, seed=31),
-                                  lambda: small_fluid(12000, seed=32)], ids=["cube", "sand", "fluid"])
-def test_g2p_kernels_identical(monkeypatch, make):
-    """The two G2P kernels (nodes gathered from the grid, or from a per-block LDS tile)
-    perform the same arithmetic in the same order: bit-identical results."""
-    sc = make()
-    out = {}
-    for variant in ("grid", "tile"):
-        monkeypatch.setenv("NXM_G2P", variant)
-        pipe = mpm.MpmPipeline(0)
-        st = run_gpu(pipe, sc, 4)
-        out[variant] = st.read_particles()
-        st.close()
-        pipe.close()
-    for f in ("x", "v", "C", "F", "enabled"):
-        assert np.array_equal(out["grid"][f], out["tile"][f]), f
-
-
 @pytest.mark.parametrize("make", [lambda: scenes.elastic_cube(12), lambda: small_sand(8000, seed=41),
                                   lambda: small_fluid(12000, seed=42)], ids=["cube", "sand", "fluid"])
 def test_timestep_bound_matches_oracle(pipeline, make):
@@ -198,3 +180,44 @@ def test_timestep_bound_matches_oracle(pipeline, make):
     batch.close()
     other.close()
     st.close()
+
+
+def _swap_remove_host(arrs: dict, ids):
+    """GpuParticles::swap_remove on host arrays: descending ids, last moves into each."""
+    n = arrs["x"].shape[0]
+    moves = []
+    for i in sorted(set(int(k) for k in ids), reverse=True):
+        if i >= n:
+            continue
+        last = n - 1
+        if i != last:
+            for k in arrs:
+                arrs[k][i] = arrs[k][last]
+            moves.append((last, i))
+        n -= 1
+    return {k: v[:n].copy() for k, v in arrs.items()}, moves
+
+
+def test_swap_remove_then_step(pipeline):
+    """Removing particles mid-run equals building the reduced scene from the same state:
+    the moves follow swap_remove, and the next substeps agree bit for bit."""
+    sc = small_sand(6000, seed=51)
+    st = run_gpu(pipeline, sc, 2)
+    p = st.read_particles()
+    ids = [5, 17, 5, 4321, sc.n - 1, sc.n + 10, 0]
+    keys = ("x", "v", "C", "F", "mass", "vol0", "radius", "damping", "enabled", "fixed", "tag", "model")
+    arrs = {k: np.array(p[k]) for k in keys}       # model words carry the plastic state
+    reduced, moves = _swap_remove_host(arrs, ids)
+    got_moves = st.swap_remove(ids)
+    assert got_moves == moves
+    assert len(st) == sc.n - 5
+    ref_scene = scenes.Scene(cell_width=sc.cell_width, dt=sc.dt, gravity=sc.gravity, colliders=sc.colliders,
+                             **reduced)
+    ref = run_gpu(pipeline, ref_scene, 2)
+    for _ in range(2):
+        pipeline.step(st)
+    a, b = st.read_particles(), ref.read_particles()
+    for f in ("x", "v", "C", "F", "enabled"):
+        assert np.array_equal(a[f], b[f]), f
+    st.close()
+    ref.close()
