@@ -18,7 +18,7 @@ struct WorldDev {
     uint32_t *bnp;                 // per block: primary particles
     uint32_t *bfirst;              // per block: first sorted position
     uint32_t *bup;                 // per block: 8 ids of vid + {0,1}^3 (reference order)
-    uint32_t *bdn;                 // per block: 8 ids of vid - {0,1}^3
+    uint32_t *bdn;                 // per block: 8 ids of vid - {0,1}^3 (NONE unless primary)
     float *stage;                  // per block: P2G tile, 4 x 216 floats
     float4 *gridv;                 // per node: velocity after the grid update, mass in w
     uint32_t *hkeys, *hvals;       // hash regions; hvals: dense block id of each occupied slot

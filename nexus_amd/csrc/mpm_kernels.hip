@@ -16,6 +16,8 @@
 //   k_g2p_update    per sorted particle: 27-node gather + deformation / constitutive update,
 //                   written in sorted order to the output arena
 // The reference runs the same stages as ~21 dispatches (src_mpm/pipeline.rs:385-526).
+#include <type_traits>
+
 #include "mpm_kernels.h"
 
 namespace nxm {
@@ -199,7 +201,7 @@ __global__ void k_keys_touch(WorldDev w)
 // Exclusive scans, 1024 elements per tile.  MODE_SLOTS counts occupied hash slots
 // (dense block ids); MODE_COUNTS sums the particles of each block (first sorted
 // position of each block).
-constexpr int SCAN_TILE = 1024;
+constexpr int SCAN_TILE = 2048;
 enum ScanMode { MODE_SLOTS = 0, MODE_COUNTS = 1 };
 
 template <int MODE>
@@ -283,6 +285,7 @@ constexpr unsigned long long ST_AGG = 1ull << 32, ST_PREFIX = 2ull << 32;
 template <int MODE>
 __global__ void __launch_bounds__(256) k_scan(WorldDev w)
 {
+    constexpr int PER = SCAN_TILE / 256;
     __shared__ uint32_t sh[4];
     __shared__ uint32_t s_tile, s_prefix;
     const uint32_t len = MODE == MODE_SLOTS ? w.h_total : min(w.g->n_blocks, w.b_total);
@@ -293,43 +296,57 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= ntiles) break;
-        uint32_t v[4], sum = 0;
-        const uint32_t i0 = t * SCAN_TILE + 4 * threadIdx.x;
+        uint32_t v[PER], sum = 0;
+        const uint32_t i0 = t * SCAN_TILE + PER * threadIdx.x;
         #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PER; ++k) {
             v[k] = i0 + k < len ? scan_in<MODE>(w, i0 + k) : 0u;
             sum += v[k];
         }
         uint32_t total;
         const uint32_t ex = block_excl_scan_256(sum, sh, total);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < WAVE) {
+            // Look-back by the first wave, 64 predecessors at a time: lane l reads tile
+            // hi - l; the nearest tile with an inclusive prefix ends the walk once every
+            // tile up to it has published.
+            const uint32_t l = threadIdx.x;
             uint32_t prefix = 0;
             if (t == 0) {
-                __hip_atomic_store(&state[0], ST_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (l == 0) __hip_atomic_store(&state[0], ST_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                __hip_atomic_store(&state[t], ST_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t j = t - 1;
+                if (l == 0) __hip_atomic_store(&state[t], ST_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int64_t hi = (int64_t)t - 1;
                 for (;;) {
-                    const unsigned long long st = __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const int64_t j = hi - (int64_t)l;
+                    // Below tile 0 (never needed: tile 0 carries a prefix) read as prefix 0.
+                    const unsigned long long st =
+                        j >= 0 ? __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ST_PREFIX;
                     const unsigned long long flag = st & (3ull << 32);
-                    if (flag == 0ull) continue;            // not yet published
-                    prefix += (uint32_t)st;
-                    if (flag == ST_PREFIX) break;
-                    --j;                                   // j > 0: tile 0 publishes a prefix
+                    const uint64_t pmask = __ballot(flag == ST_PREFIX);
+                    const uint32_t stop = pmask ? (uint32_t)__ffsll((unsigned long long)pmask) - 1u : 64u;
+                    const uint64_t need = stop >= 63u ? ~0ull : ((2ull << stop) - 1ull);
+                    if (__ballot(flag == 0ull) & need) continue;   // a tile in range has not published yet
+                    uint32_t val = l <= stop ? (uint32_t)st : 0u;
+                    for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, WAVE);
+                    prefix += val;
+                    if (stop < 64u) break;
+                    hi -= 64;                                         // 64 aggregates: walk further back
                 }
-                __hip_atomic_store(&state[t], ST_PREFIX | (unsigned long long)(prefix + total), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (l == 0)
+                    __hip_atomic_store(&state[t], ST_PREFIX | (unsigned long long)(prefix + total), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_prefix = prefix;
-            if (t == ntiles - 1) {
-                if (MODE == MODE_SLOTS) w.g->n_blocks = min(prefix + total, w.b_total);
-                else w.g->p_out = prefix + total;
+            if (l == 0) {
+                s_prefix = prefix;
+                if (t == ntiles - 1) {
+                    if (MODE == MODE_SLOTS) w.g->n_blocks = min(prefix + total, w.b_total);
+                    else w.g->p_out = prefix + total;
+                }
             }
         }
         __syncthreads();
         uint32_t r = s_prefix + ex;
-        #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PER; ++k) {
             if (i0 + k < len) scan_out<MODE>(w, i0 + k, r, v[k]);
             r += v[k];
         }
@@ -358,7 +375,9 @@ __device__ inline void block_info_part(const WorldDev &w)
         if (v.x == OVERFLOW_VID) continue;
         const SceneDev sc = w.scenes[w.bscene[d]];
         if (j < 8) {
-            w.bdn[8 * d + j] = j == 0 ? d : find_block(w, sc, v.x - ASSOC[j][0], v.y - ASSOC[j][1], v.z - ASSOC[j][2]);
+            // Sources of the grid update: only primary blocks have a P2G staging tile.
+            const uint32_t src = j == 0 ? d : find_block(w, sc, v.x - ASSOC[j][0], v.y - ASSOC[j][1], v.z - ASSOC[j][2]);
+            w.bdn[8 * d + j] = (src != NONE && w.bprim[src]) ? src : NONE;
         } else if (w.bprim[d]) {
             const uint32_t i = j - 8;
             w.bup[8 * d + i] = i == 0 ? d : find_block(w, sc, v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]);
@@ -605,30 +624,40 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                         for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
                     }
                     const float msm = m * sm;
-                    #pragma unroll
-                    for (int sz = 0; sz < 3 && inside; ++sz) {
-                        const f3 az = sz == 0 ? base : (sz == 1 ? base + hz : base + hz * 2.0f);
+                    // The non-finite checks run only for suspect lanes (a separate copy of
+                    // the loop, so the common path carries none of them).
+                    auto scatter = [&](auto check) {
                         #pragma unroll
-                        for (int sy = 0; sy < 3; ++sy) {
-                            const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
+                        for (int sz = 0; sz < 3; ++sz) {
+                            const f3 az = sz == 0 ? base : (sz == 1 ? base + hz : base + hz * 2.0f);
                             #pragma unroll
-                            for (int sx = 0; sx < 3; ++sx) {
-                                const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
-                                const float wgt = wxy[sy][sx] * wz[sz];
-                                const f3 vc = a * (wgt * smom);
-                                const float mw = msm * wgt;
-                                const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
-                                atomicAdd(&tile[t], FX::to(vc.x, 1.0f));
-                                atomicAdd(&tile[TILE_NODES + t], FX::to(vc.y, 1.0f));
-                                atomicAdd(&tile[2 * TILE_NODES + t], FX::to(vc.z, 1.0f));
-                                atomicAdd(&tile[3 * TILE_NODES + t], FX::to(mw, 1.0f));
-                                if (suspect && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
-                                    const int q[4] = {t, TILE_NODES + t, 2 * TILE_NODES + t, 3 * TILE_NODES + t};
-                                    #pragma unroll
-                                    for (int k = 0; k < 4; ++k) atomicOr(&bad[q[k] >> 5], 1u << (q[k] & 31));
+                            for (int sy = 0; sy < 3; ++sy) {
+                                const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
+                                #pragma unroll
+                                for (int sx = 0; sx < 3; ++sx) {
+                                    const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
+                                    const float wgt = wxy[sy][sx] * wz[sz];
+                                    const f3 vc = a * (wgt * smom);
+                                    const float mw = msm * wgt;
+                                    const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
+                                    atomicAdd(&tile[t], FX::to(vc.x, 1.0f));
+                                    atomicAdd(&tile[TILE_NODES + t], FX::to(vc.y, 1.0f));
+                                    atomicAdd(&tile[2 * TILE_NODES + t], FX::to(vc.z, 1.0f));
+                                    atomicAdd(&tile[3 * TILE_NODES + t], FX::to(mw, 1.0f));
+                                    if constexpr (decltype(check)::value) {
+                                        if (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw)) {
+                                            const int q[4] = {t, TILE_NODES + t, 2 * TILE_NODES + t, 3 * TILE_NODES + t};
+                                            #pragma unroll
+                                            for (int k = 0; k < 4; ++k) atomicOr(&bad[q[k] >> 5], 1u << (q[k] & 31));
+                                        }
+                                    }
                                 }
                             }
                         }
+                    };
+                    if (inside) {
+                        if (suspect) scatter(std::true_type{});
+                        else scatter(std::false_type{});
                     }
                 }
             }
@@ -678,7 +707,7 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             const int ex = ASSOC[e][0], ey = ASSOC[e][1], ez = ASSOC[e][2];
             if ((ex && i > 1) || (ey && j > 1) || (ez && k > 1)) continue;
             const uint32_t src = w.bdn[8 * d + e];
-            if (src == NONE || w.bnp[src] == 0u) continue;
+            if (src == NONE) continue;
             const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
             const float4 q = reinterpret_cast<const float4 *>(w.stage)[(size_t)src * TILE_NODES + REGION_OFFSET[e] + i + nx * (j + ny * k)];
             mx += q.x;
