@@ -183,6 +183,8 @@ class World {
     nxm_status init()
     {
         HIP_TRY(hipSetDevice(ctx_->device));
+        p2g_groups_ = (uint32_t)ctx_->num_cus * 32u;
+        HIP_TRY(p2g_start_.reserve(p2g_groups_ + 1));
         HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         for (auto &e : ev_) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(g_.reserve(1));
@@ -364,7 +366,7 @@ class World {
         LaunchConfig lc{};
         lc.p_hint = p_in_;
         lc.persistent_blocks = ctx_->num_cus * 8;
-        lc.p2g_blocks = ctx_->num_cus * 32;
+        lc.p2g_blocks = (int)p2g_groups_;
         hipEvent_t *evs = times ? ev_ : nullptr;
         if (!times && profiling_) {
             nxm_status st = ring_events(&evs);
@@ -604,6 +606,7 @@ class World {
         w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
         w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
+        w.p2g_start = p2g_start_.p; w.p2g_groups = p2g_groups_;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
@@ -810,6 +813,8 @@ class World {
     DevBuf<uint32_t> pkey_, pdense_, prank_, perm_, sblock_;
     DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_;
     DevBuf<unsigned long long> scan_state_;
+    DevBuf<uint32_t> p2g_start_;
+    uint32_t p2g_groups_ = 0;
     uint32_t scan_stride_ = 0;
     DevBuf<uint32_t> region_start_, region_scene_;
     uint32_t n_regions_ = 0, last_b_start_ = 0;

@@ -444,6 +444,29 @@ __global__ void k_block_count(WorldDev w)
     block_info_part(w);
 }
 
+// P2G work split: workgroup g of the P2G grid takes blocks [p2g_start[g], p2g_start[g+1]),
+// so that every workgroup gets about the same cost, counted as its particles plus
+// P2G_BLOCK_COST per block (the tile clear and flush).  cost(d) = first particle of block
+// d + P2G_BLOCK_COST d is non-decreasing; p2g_start[g] = min{d : cost(d) G >= g T}, T the
+// total cost, G the number of workgroups.
+constexpr uint64_t P2G_BLOCK_COST = 16;
+
+__device__ inline void p2g_split(const WorldDev &w)
+{
+    const uint32_t nb = w.g->n_blocks, G = w.p2g_groups;
+    if (nb == 0u || G == 0u) return;
+    const uint64_t T = (uint64_t)w.g->p_out + P2G_BLOCK_COST * nb;
+    for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nb; d += gridDim.x * blockDim.x) {
+        const uint64_t a = ((uint64_t)w.bfirst[d] + P2G_BLOCK_COST * d) * G;
+        const uint64_t g_hi = a / T;                                  // last g with start d
+        uint64_t g_lo = 0;                                            // first g with start d
+        if (d > 0) g_lo = (((uint64_t)w.bfirst[d - 1] + P2G_BLOCK_COST * (d - 1)) * G) / T + 1;
+        for (uint64_t g = g_lo; g <= g_hi && g < G; ++g) w.p2g_start[g] = d;
+        if (d == nb - 1)
+            for (uint64_t g = g_hi + 1; g <= G; ++g) w.p2g_start[g] = nb;
+    }
+}
+
 __global__ void k_scatter(WorldDev w)
 {
     const uint32_t n = w.g->p_in;
@@ -456,6 +479,7 @@ __global__ void k_scatter(WorldDev w)
         w.perm[pos] = i;
         w.sblock[pos] = d;
     }
+    p2g_split(w);
 }
 
 // ------------------------------------------------------------------------------------
@@ -522,6 +546,51 @@ __device__ inline float wave_max(float v)
     return v;
 }
 
+// Scatter of one particle's 27 contributions into the block's fixed-point tile.  Lanes
+// start the stencil at different z slabs (lane mod 3): particles of one cell in one wave
+// then add to different nodes at the same instruction instead of serializing on one LDS
+// address; the integer sums do not depend on the order.  CHECK adds the non-finite mask
+// (only for suspect particles, so the common path carries none of the tests).
+template <bool CHECK>
+__device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int rot,
+                                            const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wxy)[3][3],
+                                            float wz0, float wz1, float wz2, float smom, float msm)
+{
+    #pragma unroll
+    for (int kz = 0; kz < 3; ++kz) {
+        const int sz = kz + rot >= 3 ? kz + rot - 3 : kz + rot;
+        // base + sz hz, exactly as base + hz or base + 2 hz (sz hz is exact).
+        const float fsz = (float)sz;
+        const f3 az = mk3(__builtin_fmaf(hz.x, fsz, base.x), __builtin_fmaf(hz.y, fsz, base.y),
+                          __builtin_fmaf(hz.z, fsz, base.z));
+        float wzs = wz0;
+        wzs = sz == 1 ? wz1 : wzs;
+        wzs = sz == 2 ? wz2 : wzs;
+        #pragma unroll
+        for (int sy = 0; sy < 3; ++sy) {
+            const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
+            #pragma unroll
+            for (int sx = 0; sx < 3; ++sx) {
+                const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
+                const float wgt = wxy[sy][sx] * wzs;
+                const f3 vc = a * (wgt * smom);
+                const float mw = msm * wgt;
+                const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
+                atomicAdd(&tile[t], Fixed::to(vc.x, 1.0f));
+                atomicAdd(&tile[TILE_NODES + t], Fixed::to(vc.y, 1.0f));
+                atomicAdd(&tile[2 * TILE_NODES + t], Fixed::to(vc.z, 1.0f));
+                atomicAdd(&tile[3 * TILE_NODES + t], Fixed::to(mw, 1.0f));
+                if (CHECK && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
+                    atomicOr(&bad[t >> 5], 1u << (t & 31));
+                    atomicOr(&bad[(TILE_NODES + t) >> 5], 1u << ((TILE_NODES + t) & 31));
+                    atomicOr(&bad[(2 * TILE_NODES + t) >> 5], 1u << ((2 * TILE_NODES + t) & 31));
+                    atomicOr(&bad[(3 * TILE_NODES + t) >> 5], 1u << ((3 * TILE_NODES + t) & 31));
+                }
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 {
     using FX = Fixed;
@@ -532,8 +601,10 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
     const float *VX = w.in.plane(PVX), *VY = w.in.plane(PVY), *VZ = w.in.plane(PVZ);
     const float *M = w.in.plane(PMASS);
     const uint32_t *FL = w.in.uplane(PFLAGS);
-    uint32_t dbegin, dend;
-    xcd_range(w.g->n_blocks, dbegin, dend);
+    // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t g = (G % NUM_XCD == 0) ? (b % NUM_XCD) * (G / NUM_XCD) + b / NUM_XCD : b;
+    const uint32_t dbegin = w.p2g_start[g], dend = w.p2g_start[g + 1];
     for (uint32_t d = dbegin; d < dend; ++d) {
         const uint32_t s = w.bscene[d];
         const uint32_t np = w.bnp[d];
@@ -624,40 +695,11 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                         for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
                     }
                     const float msm = m * sm;
-                    // The non-finite checks run only for suspect lanes (a separate copy of
-                    // the loop, so the common path carries none of them).
-                    auto scatter = [&](auto check) {
-                        #pragma unroll
-                        for (int sz = 0; sz < 3; ++sz) {
-                            const f3 az = sz == 0 ? base : (sz == 1 ? base + hz : base + hz * 2.0f);
-                            #pragma unroll
-                            for (int sy = 0; sy < 3; ++sy) {
-                                const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
-                                #pragma unroll
-                                for (int sx = 0; sx < 3; ++sx) {
-                                    const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
-                                    const float wgt = wxy[sy][sx] * wz[sz];
-                                    const f3 vc = a * (wgt * smom);
-                                    const float mw = msm * wgt;
-                                    const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
-                                    atomicAdd(&tile[t], FX::to(vc.x, 1.0f));
-                                    atomicAdd(&tile[TILE_NODES + t], FX::to(vc.y, 1.0f));
-                                    atomicAdd(&tile[2 * TILE_NODES + t], FX::to(vc.z, 1.0f));
-                                    atomicAdd(&tile[3 * TILE_NODES + t], FX::to(mw, 1.0f));
-                                    if constexpr (decltype(check)::value) {
-                                        if (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw)) {
-                                            const int q[4] = {t, TILE_NODES + t, 2 * TILE_NODES + t, 3 * TILE_NODES + t};
-                                            #pragma unroll
-                                            for (int k = 0; k < 4; ++k) atomicOr(&bad[q[k] >> 5], 1u << (q[k] & 31));
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                    };
+                    const int rot = (int)(l % 3u);
+                    const float wz0 = wz[0], wz1 = wz[1], wz2 = wz[2];
                     if (inside) {
-                        if (suspect) scatter(std::true_type{});
-                        else scatter(std::false_type{});
+                        if (suspect) p2g_scatter<true>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz0, wz1, wz2, smom, msm);
+                        else p2g_scatter<false>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz0, wz1, wz2, smom, msm);
                     }
                 }
             }
