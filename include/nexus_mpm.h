@@ -143,6 +143,15 @@ typedef struct {
     float total_ms;
 } nxm_stage_times;
 
+/* Launch shapes of a substep (no reference counterpart: wgpu picks none of these).
+ * Results do not depend on them (fixed-point P2G, per-particle and per-node stages), so
+ * they can be tuned while a scene runs. */
+typedef struct {
+    uint32_t p2g_groups;      /* workgroups (one wave each) of the P2G kernel */
+    uint32_t p2g_block_cost;  /* per-block cost, in particles, of the P2G work split */
+    uint32_t grid_groups;     /* workgroup cap of the grid-stride kernels (256 threads each) */
+} nxm_launch_config;
+
 /* Canonical form of the block sort (the part of the reference's sort that does not
  * depend on atomic ordering): per active block its virtual id, the number of
  * primary particles, of particles including spilled extras, the ten bucket counts
@@ -247,6 +256,18 @@ size_t nxm_batch_len(const nxm_batch *batch);
  * uncaptured.  No reference counterpart (the reference records one compute pass per
  * step, src_mpm/pipeline.rs:385-526). */
 nxm_status nxm_batch_set_graph(nxm_batch *batch, int enabled);
+/* Online autotuner: the next substeps of the batch try candidate launch shapes
+ * (coordinate descent over p2g_groups, p2g_block_cost and grid_groups, each candidate
+ * timed with HIP events over a few substeps) and keep the fastest.  The substeps run
+ * normally meanwhile; retunes when the particle count changes by more than 2x.  No
+ * reference counterpart. */
+nxm_status nxm_batch_set_autotune(nxm_batch *batch, int enabled);
+nxm_status nxm_scene_set_autotune(nxm_scene *scene, int enabled);
+/* Current launch shapes (the tuner's choice once it has finished) and a manual setting
+ * (which stops the tuner).  No reference counterpart. */
+nxm_status nxm_batch_get_launch_config(nxm_batch *batch, nxm_launch_config *out);
+nxm_status nxm_batch_set_launch_config(nxm_batch *batch, const nxm_launch_config *cfg);
+nxm_status nxm_scene_get_launch_config(nxm_scene *scene, nxm_launch_config *out);
 nxm_status nxm_batch_profile_begin(nxm_batch *batch);
 nxm_status nxm_batch_profile_end(nxm_batch *batch, nxm_stage_times *avg, uint32_t *steps);
 

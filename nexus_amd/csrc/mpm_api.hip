@@ -175,6 +175,7 @@ class World {
             (void)hipStreamSynchronize(stream_);
             drop_graphs();
             for (auto &e : ev_) (void)hipEventDestroy(e);
+            for (auto &e : tune_ev_) (void)hipEventDestroy(e);
             for (auto &e : ring_) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream_);
         }
@@ -183,8 +184,8 @@ class World {
     nxm_status init()
     {
         HIP_TRY(hipSetDevice(ctx_->device));
-        p2g_groups_ = (uint32_t)ctx_->num_cus * 32u;
-        HIP_TRY(p2g_start_.reserve(p2g_groups_ + 1));
+        cfg_ = default_config();
+        HIP_TRY(p2g_start_.reserve(max_p2g_groups() + 1));
         HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         for (auto &e : ev_) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(g_.reserve(1));
@@ -362,15 +363,26 @@ class World {
             pin_dirty_ = false;
         }
         if (p_in_ == 0 || b_total_ == 0 || n_regions_ == 0) return NXM_OK;
+        if (tune_on_ && !tuning_ && tuned_p_ && (p_in_ > 2 * tuned_p_ || 2 * p_in_ < tuned_p_)) start_tuning();
+        const nxm_launch_config cfg = tuning_ ? tune_candidate() : cfg_;
         WorldDev w = view();
+        w.p2g_groups = cfg.p2g_groups;
+        w.p2g_block_cost = cfg.p2g_block_cost;
         LaunchConfig lc{};
         lc.p_hint = p_in_;
-        lc.persistent_blocks = ctx_->num_cus * 8;
-        lc.p2g_blocks = (int)p2g_groups_;
+        lc.persistent_blocks = (int)cfg.grid_groups;
+        lc.p2g_blocks = (int)cfg.p2g_groups;
         hipEvent_t *evs = times ? ev_ : nullptr;
         if (!times && profiling_) {
             nxm_status st = ring_events(&evs);
             if (st != NXM_OK) return st;
+        }
+        hipEvent_t tev[2] = {nullptr, nullptr};
+        const bool tuning_step = tuning_ && !evs;
+        if (tuning_step) {
+            nxm_status st = tune_events(tev);
+            if (st != NXM_OK) return st;
+            if (tev[0]) HIP_TRY(hipEventRecord(tev[0], stream_));
         }
         if (!clean_) {
             // Hash regions, counters and scan state must start empty; each substep
@@ -379,10 +391,10 @@ class World {
             HIP_TRY(hipGetLastError());
         }
         clean_ = false;
-        if (graph_on_ && !evs) {
-            // Replays the substep's 15 launches as one graph; recaptured when the launch
+        if (graph_on_ && !evs && !tuning_) {
+            // Replays the substep's launches as one graph; recaptured when the launch
             // shapes or any buffer address change.  The graph holds kernels only.
-            const GraphKey k{p_in_, b_total_, h_total_, gen_};
+            const GraphKey k{p_in_, b_total_, h_total_, gen_, cfg.p2g_groups, cfg.p2g_block_cost, cfg.grid_groups};
             if (!gexec_[cur_] || !(gkey_[cur_] == k)) {
                 if (gexec_[cur_]) { HIP_TRY(hipGraphExecDestroy(gexec_[cur_])); gexec_[cur_] = nullptr; }
                 hipGraph_t graph = nullptr;
@@ -403,6 +415,11 @@ class World {
         }
         HIP_TRY(hipGetLastError());
         clean_ = true;
+        if (tuning_step) {
+            if (tev[1]) HIP_TRY(hipEventRecord(tev[1], stream_));
+            nxm_status st = tune_advance();
+            if (st != NXM_OK) return st;
+        }
         // The output arena holds every live particle, scenes in slot order.
         cur_ ^= 1;
         uint32_t at = 0;
@@ -606,7 +623,7 @@ class World {
         w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
         w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
-        w.p2g_start = p2g_start_.p; w.p2g_groups = p2g_groups_;
+        w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
@@ -780,11 +797,122 @@ class World {
     struct GraphKey {
         uint32_t p_in, b_total, h_total;
         uint64_t gen;
+        uint32_t p2g_groups, p2g_block_cost, grid_groups;
         bool operator==(const GraphKey &o) const
         {
-            return p_in == o.p_in && b_total == o.b_total && h_total == o.h_total && gen == o.gen;
+            return p_in == o.p_in && b_total == o.b_total && h_total == o.h_total && gen == o.gen &&
+                   p2g_groups == o.p2g_groups && p2g_block_cost == o.p2g_block_cost && grid_groups == o.grid_groups;
         }
     };
+
+    // ---- online autotuner ----
+    // Coordinate descent: stage 0 varies p2g_groups, stage 1 p2g_block_cost, stage 2
+    // grid_groups, each from the best so far; a candidate runs TUNE_REPS timed substeps
+    // (after one untimed substep) and scores its fastest.
+    static constexpr int TUNE_REPS = 3;
+
+    nxm_launch_config default_config() const
+    {
+        const uint32_t cu = (uint32_t)ctx_->num_cus;
+        return nxm_launch_config{cu * 32u, 16u, cu * 8u};
+    }
+    uint32_t max_p2g_groups() const { return (uint32_t)ctx_->num_cus * 64u; }
+
+    std::vector<nxm_launch_config> stage_candidates(int stage, const nxm_launch_config &b) const
+    {
+        const uint32_t cu = (uint32_t)ctx_->num_cus;
+        std::vector<nxm_launch_config> c;
+        if (stage == 0) for (uint32_t k : {16u, 32u, 64u}) c.push_back({cu * k, b.p2g_block_cost, b.grid_groups});
+        if (stage == 1) for (uint32_t k : {4u, 16u, 64u}) c.push_back({b.p2g_groups, k, b.grid_groups});
+        if (stage == 2) for (uint32_t k : {4u, 8u, 16u}) c.push_back({b.p2g_groups, b.p2g_block_cost, cu * k});
+        return c;
+    }
+
+    void start_tuning()
+    {
+        tuning_ = true;
+        tune_stage_ = 0;
+        tune_best_ = cfg_;
+        tune_cands_ = stage_candidates(0, tune_best_);
+        tune_idx_ = 0;
+        tune_rep_ = -1;   // one untimed substep first
+        tune_times_.assign(tune_cands_.size(), 1e30f);
+        tune_used_ = 0;
+    }
+
+    nxm_launch_config tune_candidate() const { return tune_cands_[tune_idx_]; }
+
+    nxm_status tune_events(hipEvent_t *tev)
+    {
+        if (tune_rep_ < 0) return NXM_OK;   // untimed warm-up substep
+        const size_t need = 2 * (tune_used_ + 1);
+        while (tune_ev_.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            tune_ev_.push_back(e);
+        }
+        tev[0] = tune_ev_[2 * tune_used_];
+        tev[1] = tune_ev_[2 * tune_used_ + 1];
+        tune_slot_.resize(tune_used_ + 1);
+        tune_slot_[tune_used_] = tune_idx_;
+        ++tune_used_;
+        return NXM_OK;
+    }
+
+    // Called after each tuning substep; at the end of a stage reads the events (one
+    // synchronization per stage) and moves to the next stage or finishes.
+    nxm_status tune_advance()
+    {
+        if (++tune_rep_ < TUNE_REPS) return NXM_OK;
+        tune_rep_ = 0;
+        if (++tune_idx_ < tune_cands_.size()) return NXM_OK;
+        HIP_TRY(hipStreamSynchronize(stream_));
+        for (size_t k = 0; k < tune_used_; ++k) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, tune_ev_[2 * k], tune_ev_[2 * k + 1]));
+            tune_times_[tune_slot_[k]] = std::min(tune_times_[tune_slot_[k]], ms);
+        }
+        size_t best = 0;
+        for (size_t k = 1; k < tune_cands_.size(); ++k)
+            if (tune_times_[k] < tune_times_[best]) best = k;
+        tune_best_ = tune_cands_[best];
+        tune_used_ = 0;
+        if (++tune_stage_ < 3) {
+            tune_cands_ = stage_candidates(tune_stage_, tune_best_);
+            tune_idx_ = 0;
+            tune_rep_ = -1;
+            tune_times_.assign(tune_cands_.size(), 1e30f);
+            return NXM_OK;
+        }
+        cfg_ = tune_best_;
+        tuning_ = false;
+        tuned_p_ = p_in_;
+        return NXM_OK;
+    }
+
+  public:
+    nxm_status set_autotune(bool on)
+    {
+        tune_on_ = on;
+        if (on) start_tuning();
+        else tuning_ = false;
+        return NXM_OK;
+    }
+
+    nxm_launch_config launch_config() const { return cfg_; }
+
+    nxm_status set_launch_config(const nxm_launch_config &c)
+    {
+        if (c.p2g_groups == 0 || c.p2g_groups > max_p2g_groups() || c.p2g_groups % 8 != 0)
+            return fail(NXM_ERR_INVALID_ARGUMENT, "p2g_groups must be a positive multiple of 8 up to 64 per CU");
+        if (c.grid_groups == 0) return fail(NXM_ERR_INVALID_ARGUMENT, "grid_groups must be positive");
+        tune_on_ = false;
+        tuning_ = false;
+        cfg_ = c;
+        return NXM_OK;
+    }
+
+  private:
 
     void drop_graphs()
     {
@@ -796,6 +924,16 @@ class World {
 
     nxm_ctx *ctx_;
     hipStream_t stream_ = nullptr;
+    nxm_launch_config cfg_{};
+    bool tune_on_ = false, tuning_ = false;
+    int tune_stage_ = 0, tune_rep_ = 0;
+    size_t tune_idx_ = 0, tune_used_ = 0;
+    uint32_t tuned_p_ = 0;
+    nxm_launch_config tune_best_{};
+    std::vector<nxm_launch_config> tune_cands_;
+    std::vector<float> tune_times_;
+    std::vector<size_t> tune_slot_;
+    std::vector<hipEvent_t> tune_ev_;
     bool graph_on_ = false;
     uint64_t gen_ = 0;                 // bumped whenever a buffer address or table changes
     hipGraphExec_t gexec_[2] = {};     // one per arena parity
@@ -814,7 +952,6 @@ class World {
     DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_;
     DevBuf<unsigned long long> scan_state_;
     DevBuf<uint32_t> p2g_start_;
-    uint32_t p2g_groups_ = 0;
     uint32_t scan_stride_ = 0;
     DevBuf<uint32_t> region_start_, region_scene_;
     uint32_t n_regions_ = 0, last_b_start_ = 0;
@@ -1346,6 +1483,42 @@ nxm_status nxm_batch_set_graph(nxm_batch *b, int enabled)
 {
     if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
     return b->world->set_graph(enabled != 0);
+}
+
+nxm_status nxm_batch_set_autotune(nxm_batch *b, int enabled)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->world->set_autotune(enabled != 0);
+}
+
+nxm_status nxm_scene_set_autotune(nxm_scene *sc, int enabled)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (sc->batch) return fail(NXM_ERR_BUSY, "scene belongs to a batch");
+    nxm_status st = ensure_own_world(sc);
+    if (st != NXM_OK) return st;
+    return sc->own->set_autotune(enabled != 0);
+}
+
+nxm_status nxm_batch_get_launch_config(nxm_batch *b, nxm_launch_config *out)
+{
+    if (!b || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = b->world->launch_config();
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_set_launch_config(nxm_batch *b, const nxm_launch_config *cfg)
+{
+    if (!b || !cfg) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    return b->world->set_launch_config(*cfg);
+}
+
+nxm_status nxm_scene_get_launch_config(nxm_scene *sc, nxm_launch_config *out)
+{
+    if (!sc || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    *out = sc->world->launch_config();
+    return NXM_OK;
 }
 
 nxm_status nxm_scene_set_graph(nxm_scene *sc, int enabled)

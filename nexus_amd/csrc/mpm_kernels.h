@@ -35,6 +35,7 @@ struct WorldDev {
     Globals *g;
     uint32_t *p2g_start;           // P2G workgroup g: blocks [p2g_start[g], p2g_start[g+1])
     uint32_t p2g_groups;           // workgroups of the P2G launch
+    uint32_t p2g_block_cost;       // per-block cost of the P2G split, in particles
     unsigned long long *scan_state;   // single-pass scan tile status, 2 x scan_stride words
     uint32_t scan_stride;
     float *dbg_mom;                // optional per-node momentum after P2G (3 floats)

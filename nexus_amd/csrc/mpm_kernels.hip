@@ -446,21 +446,21 @@ __global__ void k_block_count(WorldDev w)
 
 // P2G work split: workgroup g of the P2G grid takes blocks [p2g_start[g], p2g_start[g+1]),
 // so that every workgroup gets about the same cost, counted as its particles plus
-// P2G_BLOCK_COST per block (the tile clear and flush).  cost(d) = first particle of block
-// d + P2G_BLOCK_COST d is non-decreasing; p2g_start[g] = min{d : cost(d) G >= g T}, T the
-// total cost, G the number of workgroups.
-constexpr uint64_t P2G_BLOCK_COST = 16;
+// w.p2g_block_cost per block (the tile clear and flush).  cost(d) = first particle of
+// block d + p2g_block_cost d is non-decreasing; p2g_start[g] = min{d : cost(d) G >= g T},
+// T the total cost, G the number of workgroups.
 
 __device__ inline void p2g_split(const WorldDev &w)
 {
     const uint32_t nb = w.g->n_blocks, G = w.p2g_groups;
     if (nb == 0u || G == 0u) return;
-    const uint64_t T = (uint64_t)w.g->p_out + P2G_BLOCK_COST * nb;
+    const uint64_t K = w.p2g_block_cost;
+    const uint64_t T = (uint64_t)w.g->p_out + K * nb;
     for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nb; d += gridDim.x * blockDim.x) {
-        const uint64_t a = ((uint64_t)w.bfirst[d] + P2G_BLOCK_COST * d) * G;
+        const uint64_t a = ((uint64_t)w.bfirst[d] + K * d) * G;
         const uint64_t g_hi = a / T;                                  // last g with start d
         uint64_t g_lo = 0;                                            // first g with start d
-        if (d > 0) g_lo = (((uint64_t)w.bfirst[d - 1] + P2G_BLOCK_COST * (d - 1)) * G) / T + 1;
+        if (d > 0) g_lo = (((uint64_t)w.bfirst[d - 1] + K * (d - 1)) * G) / T + 1;
         for (uint64_t g = g_lo; g <= g_hi && g < G; ++g) w.p2g_start[g] = d;
         if (d == nb - 1)
             for (uint64_t g = g_hi + 1; g <= G; ++g) w.p2g_start[g] = nb;

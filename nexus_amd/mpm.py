@@ -40,7 +40,8 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
-    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
+    "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
@@ -67,6 +68,14 @@ class _SoA(ct.Structure):
     _fields_ = [(name, ct.c_void_p) for name in (
         "position", "velocity", "def_grad", "affine", "init_volume", "init_radius", "mass", "damping", "phase",
         "boundary_friction", "group_id", "enabled", "fixed", "model_tag", "model_data")]
+
+
+class LaunchConfig(ct.Structure):
+    """Launch shapes of a substep (nxm_launch_config)."""
+    _fields_ = [("p2g_groups", ct.c_uint32), ("p2g_block_cost", ct.c_uint32), ("grid_groups", ct.c_uint32)]
+
+    def as_tuple(self):
+        return (int(self.p2g_groups), int(self.p2g_block_cost), int(self.grid_groups))
 
 
 class _StageTimes(ct.Structure):
@@ -113,6 +122,12 @@ def library() -> ct.CDLL:
     L.nxm_scene_timestep_bound.restype = ct.c_int
     L.nxm_scene_swap_remove.argtypes = [P, P, ct.c_size_t, P, P, ct.POINTER(ct.c_size_t)]
     L.nxm_scene_swap_remove.restype = ct.c_int
+    for f in ("nxm_batch_set_autotune", "nxm_scene_set_autotune"):
+        getattr(L, f).argtypes = [P, ct.c_int]
+        getattr(L, f).restype = ct.c_int
+    for f in ("nxm_batch_get_launch_config", "nxm_scene_get_launch_config", "nxm_batch_set_launch_config"):
+        getattr(L, f).argtypes = [P, ct.POINTER(LaunchConfig)]
+        getattr(L, f).restype = ct.c_int
     L.nxm_scene_read_positions.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_sort_size.argtypes = [P, ct.POINTER(ct.c_size_t)]
@@ -262,6 +277,15 @@ class MpmState:
                                               ct.byref(k)))
         return [(int(f[i]), int(t[i])) for i in range(k.value)]
 
+    def set_autotune(self, enabled: bool = True) -> None:
+        """The next standalone substeps try launch shapes and keep the fastest."""
+        _check(self.lib.nxm_scene_set_autotune(self.handle, int(enabled)))
+
+    def launch_config(self) -> tuple:
+        c = LaunchConfig()
+        _check(self.lib.nxm_scene_get_launch_config(self.handle, ct.byref(c)))
+        return c.as_tuple()
+
     def timestep_bound(self) -> float:
         """CFL bound on the substep length of the current state (WgTimestepBounds)."""
         dt = ct.c_float(0.0)
@@ -398,6 +422,19 @@ class Batch:
 
     def set_graph(self, enabled: bool) -> None:
         _check(self.lib.nxm_batch_set_graph(self.handle, int(enabled)))
+
+    def set_autotune(self, enabled: bool = True) -> None:
+        """The next substeps try launch shapes (coordinate descent) and keep the fastest."""
+        _check(self.lib.nxm_batch_set_autotune(self.handle, int(enabled)))
+
+    def launch_config(self) -> tuple:
+        c = LaunchConfig()
+        _check(self.lib.nxm_batch_get_launch_config(self.handle, ct.byref(c)))
+        return c.as_tuple()
+
+    def set_launch_config(self, p2g_groups: int, p2g_block_cost: int, grid_groups: int) -> None:
+        c = LaunchConfig(p2g_groups, p2g_block_cost, grid_groups)
+        _check(self.lib.nxm_batch_set_launch_config(self.handle, ct.byref(c)))
 
     def profile_begin(self) -> None:
         _check(self.lib.nxm_batch_profile_begin(self.handle))

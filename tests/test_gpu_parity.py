@@ -221,3 +221,37 @@ def test_swap_remove_then_step(pipeline):
         assert np.array_equal(a[f], b[f]), f
     st.close()
     ref.close()
+
+
+def test_launch_shapes_do_not_change_results(pipeline):
+    """P2G sums in exact fixed point and the other stages work per particle or per node,
+    so any launch shape gives the same bits: fixed extreme shapes and an autotuned run
+    (which switches shapes between substeps) all agree."""
+    mix = [scenes.elastic_cube(10), small_sand(5000, seed=61), small_fluid(7000, seed=62)]
+    cu = None
+    results = []
+    for mode in ("default", "small", "large", "autotune"):
+        batch = mpm.Batch(pipeline)
+        states = [mpm.MpmState.from_scene(pipeline, sc) for sc in mix]
+        for s in states:
+            batch.add(s)
+        if cu is None:
+            cu = batch.launch_config()[0] // 32
+        if mode == "small":
+            batch.set_launch_config(cu * 16, 4, cu * 4)
+        elif mode == "large":
+            batch.set_launch_config(cu * 64, 64, cu * 16)
+        elif mode == "autotune":
+            batch.set_autotune(True)
+        batch.run(32)
+        results.append([s.read_particles() for s in states])
+        if mode == "autotune":
+            g, k, gg = batch.launch_config()
+            assert g in (cu * 16, cu * 32, cu * 64) and k in (4, 16, 64) and gg in (cu * 4, cu * 8, cu * 16)
+        batch.close()
+        for s in states:
+            s.close()
+    for other in results[1:]:
+        for a, b in zip(results[0], other):
+            for f in ("x", "v", "C", "F", "enabled"):
+                assert np.array_equal(a[f], b[f]), f
