@@ -194,17 +194,25 @@ __device__ inline float trace(const m3 &a) { return a.c0.x + a.c1.y + a.c2.z; }
 // cyclic pairs (0,1), (0,2), (1,2), at most 8 sweeps with early exit once every
 // column pair is orthogonal to FP32 precision.  Returns F = U diag(s) V^T with U, V
 // rotations; s is not sorted, and its smallest entry carries the sign of det F.
+// The rotations use the hardware reciprocal, reciprocal square root and square root
+// (about 1 ulp): the SVD needs its rotations orthogonal to FP32 rounding, not correctly
+// rounded quotients.  Convergence is tested on (gamma^2 / (alpha beta)), the squared
+// cosine between the columns, against (1e-7)^2.
+__device__ inline float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ inline void jacobi_pair(f3 &ap, f3 &aq, f3 &vp, f3 &vq, float &off)
 {
     float alpha = dot(ap, ap), beta = dot(aq, aq), gamma = dot(ap, aq);
-    float scale = sqrtf(alpha * beta);
-    float rel = scale > 0.0f ? fabsf(gamma) / scale : 0.0f;
-    off = fmaxf(off, rel);
-    if (!(rel > 1.0e-7f)) return;
-    float zeta = (beta - alpha) / (2.0f * gamma);
+    float ab = alpha * beta;
+    float rel2 = ab > 0.0f ? (gamma * gamma) * rcp_hw(ab) : 0.0f;
+    off = fmaxf(off, rel2);
+    if (!(rel2 > 1.0e-14f)) return;
+    float zeta = (beta - alpha) * rcp_hw(2.0f * gamma);
     float az = fabsf(zeta);
-    float t = az > 1.0e15f ? 0.5f / zeta : copysignf(1.0f, zeta) / (az + sqrtf(1.0f + zeta * zeta));
-    float c = 1.0f / sqrtf(1.0f + t * t);
+    float t = az > 1.0e15f ? 0.5f * rcp_hw(zeta) : copysignf(rcp_hw(az + sqrt_hw(1.0f + zeta * zeta)), zeta);
+    float c = rsq_hw(1.0f + t * t);
     float s = c * t;
     f3 a0 = ap, a1 = aq, w0 = vp, w1 = vq;
     ap = a0 * c - a1 * s;
@@ -229,14 +237,14 @@ __device__ inline void svd3(const m3 &F, m3 &U, f3 &s, m3 &V)
         jacobi_pair(a0, a1, w0, w1, off);
         jacobi_pair(a0, a2, w0, w2, off);
         jacobi_pair(a1, a2, w1, w2, off);
-        if (!(off > 1.0e-7f)) break;
+        if (!(off > 1.0e-14f)) break;
     }
-    float s0 = length(a0), s1 = length(a1), s2 = length(a2);
+    float s0 = sqrt_hw(dot(a0, a0)), s1 = sqrt_hw(dot(a1, a1)), s2 = sqrt_hw(dot(a2, a2));
     float big = fmaxf(s0, fmaxf(s1, s2));
     float tiny = big * 1.0e-30f;
-    f3 u0 = s0 > tiny ? a0 / s0 : mk3(0, 0, 0);
-    f3 u1 = s1 > tiny ? a1 / s1 : mk3(0, 0, 0);
-    f3 u2 = s2 > tiny ? a2 / s2 : mk3(0, 0, 0);
+    f3 u0 = s0 > tiny ? a0 * rcp_hw(s0) : mk3(0, 0, 0);
+    f3 u1 = s1 > tiny ? a1 * rcp_hw(s1) : mk3(0, 0, 0);
+    f3 u2 = s2 > tiny ? a2 * rcp_hw(s2) : mk3(0, 0, 0);
     // Complete U where columns vanished (rank-deficient F).
     int nz = (s0 > tiny) + (s1 > tiny) + (s2 > tiny);
     if (nz < 3) {

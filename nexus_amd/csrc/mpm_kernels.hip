@@ -794,15 +794,19 @@ __device__ __constant__ const int8_t NBH[27][3] = {
 struct GatherGrid {
     const uint32_t *bup;
     const float4 *gridv;
-    uint32_t nb[8];
+    uint32_t n0, n1, n2, n3, n4, n5, n6, n7;   // vid + {0,1}^3 in ASSOC order (4 ox + 2 oy + oz)
     __device__ void prepare(uint32_t d)
     {
-        #pragma unroll
-        for (int o = 0; o < 8; ++o) nb[o] = bup[8 * d + o];
+        const uint32_t *p = bup + 8 * (size_t)d;
+        n0 = p[0]; n1 = p[1]; n2 = p[2]; n3 = p[3]; n4 = p[4]; n5 = p[5]; n6 = p[6]; n7 = p[7];
     }
     __device__ f3 at(int tx, int ty, int tz) const
     {
-        const uint32_t blk = nb[(tx >> 2) * 4 + (ty >> 2) * 2 + (tz >> 2)];
+        // block of the node by selects on the three offset bits (no indexed register array)
+        const bool ox = tx >= 4, oy = ty >= 4, oz = tz >= 4;
+        const uint32_t z0 = oz ? n1 : n0, z1 = oz ? n3 : n2, z2 = oz ? n5 : n4, z3 = oz ? n7 : n6;
+        const uint32_t y0 = oy ? z1 : z0, y1 = oy ? z3 : z2;
+        const uint32_t blk = ox ? y1 : y0;
         if (blk == NONE) return mk3(0, 0, 0);
         const float4 g = gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
         return mk3(g.x, g.y, g.z);
@@ -858,19 +862,26 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                     wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
                 }
                 gather.prepare(d);
+                // With dpt = ref + s h (s in {0,1,2}^3): sum w v dpt^T = V ref^T + h [Gx Gy Gz],
+                // V = sum w v and Gj = sum w v s_j, so each node costs a product and a few
+                // adds instead of an outer product; the velocity gradient and its trace
+                // (the divergence) follow after the loop.
+                f3 gx = mk3(0, 0, 0), gy = mk3(0, 0, 0), gz = mk3(0, 0, 0);
                 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
                     const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
                     const f3 cv = gather.at(bx + sx, by + sy, bz + sz);
-                    const f3 dpt = mk3(ref.x + (float)sx * h, ref.y + (float)sy * h, ref.z + (float)sz * h);
                     const float wgt = wx[sx] * wy[sy] * wz[sz];
-                    const float wi = wgt * inv_d;
-                    vel = vel + cv * wgt;
-                    gradv.c0 = gradv.c0 + cv * (dpt.x * wi);
-                    gradv.c1 = gradv.c1 + cv * (dpt.y * wi);
-                    gradv.c2 = gradv.c2 + cv * (dpt.z * wi);
-                    div = div + wi * dot(cv, dpt);
+                    const f3 cw = cv * wgt;
+                    vel = vel + cw;
+                    if (sx == 1) gx = gx + cw; else if (sx == 2) gx = gx + cw * 2.0f;
+                    if (sy == 1) gy = gy + cw; else if (sy == 2) gy = gy + cw * 2.0f;
+                    if (sz == 1) gz = gz + cw; else if (sz == 2) gz = gz + cw * 2.0f;
                 }
+                gradv.c0 = (vel * ref.x + gx * h) * inv_d;
+                gradv.c1 = (vel * ref.y + gy * h) * inv_d;
+                gradv.c2 = (vel * ref.z + gz * h) * inv_d;
+                div = gradv.c0.x + gradv.c1.y + gradv.c2.z;
             }
             // Particle update.
             const float vl = length(vel);
@@ -954,8 +965,8 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
-    GatherGrid gather{w.bup, w.gridv, {}};
-    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) g2p_particle(w, pos, w.sblock[pos], gather);
+    GatherGrid grid{w.bup, w.gridv, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) g2p_particle(w, pos, w.sblock[pos], grid);
     clear_hash(w);
 }
 
