@@ -179,20 +179,30 @@ __global__ void k_keys_touch(WorldDev w)
         uint32_t ps = __shfl_up(s, 1, WAVE);
         bool pv = __shfl_up((int)valid, 1, WAVE) != 0;
         bool head = valid && key != OVERFLOW_KEY && (l == 0 || !pv || pk != key || ps != s);
+        // Up to 8 runs per round: lane l inserts neighbour (l mod 8) of run (l / 8), so a
+        // wave's insertions overlap instead of waiting for each other's atomics.
         uint64_t heads = __ballot(head);
         while (heads) {
-            const uint32_t hl = (uint32_t)__ffsll((unsigned long long)heads) - 1u;
-            heads &= heads - 1ull;
-            const uint32_t hk = __shfl(key, (int)hl, WAVE);
-            const uint32_t hs = __shfl(s, (int)hl, WAVE);
-            if (l < 8) {
+            uint32_t hsrc[8];
+            #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                hsrc[k] = heads ? (uint32_t)__ffsll((unsigned long long)heads) - 1u : 64u;
+                heads = heads ? heads & (heads - 1ull) : 0ull;
+            }
+            const uint32_t r = l >> 3, j = l & 7u;
+            uint32_t src = hsrc[0];
+            #pragma unroll
+            for (int k = 1; k < 8; ++k) src = r == (uint32_t)k ? hsrc[k] : src;
+            const uint32_t hk = __shfl(key, (int)(src & 63u), WAVE);
+            const uint32_t hs = __shfl(s, (int)(src & 63u), WAVE);
+            if (src < 64u) {
                 const SceneDev hsc = w.scenes[hs];
                 const int4 v = unpack_vid(hk);
                 uint32_t slot;
                 hash_insert(w.hkeys + hsc.h_base, hsc.h_cap - 1u,
-                            pack_vid(v.x + ASSOC[l][0], v.y + ASSOC[l][1], v.z + ASSOC[l][2]), slot);
+                            pack_vid(v.x + ASSOC[j][0], v.y + ASSOC[j][1], v.z + ASSOC[j][2]), slot);
                 if (slot == NONE) atomicOr(&w.ctr[hs].err, ERR_GRID_CAPACITY);
-                else if (l == 0) w.hmark[hsc.h_base + slot] = 1u;
+                else if (j == 0) w.hmark[hsc.h_base + slot] = 1u;
             }
         }
     }
