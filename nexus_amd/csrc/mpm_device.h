@@ -114,9 +114,20 @@ __device__ inline int32_t sat_i32(float f)
     return (int32_t)f;
 }
 
-__device__ inline void cell_of(float x, float h, int32_t &blk, uint32_t &local)
+// roundf(x / h) with the IEEE quotient, cheaply: x * inv_h is within 1.5 ulp of x / h,
+// so the two round to the same integer unless the product lies within a few ulps of a
+// half-integer; only then is the division done.  (roundf rounds half away from zero.)
+__device__ inline float round_div(float x, float h, float inv_h)
 {
-    float cell = roundf(x / h) - 1.0f;            // round half away from zero
+    const float q = x * inv_h;
+    const float f = fabsf(q - truncf(q));
+    if (fabsf(f - 0.5f) > fabsf(q) * 0x1p-21f) return roundf(q);
+    return roundf(x / h);
+}
+
+__device__ inline void cell_of(float x, float h, float inv_h, int32_t &blk, uint32_t &local)
+{
+    float cell = round_div(x, h, inv_h) - 1.0f;
     float b = floorf(cell / 4.0f);
     blk = sat_i32(b);
     float l = cell - b * 4.0f;
@@ -317,12 +328,15 @@ __device__ inline m3 linear_stress_svd(float lambda, float mu, const m3 &U, f3 s
     return st;
 }
 
-// DruckerPragerPlasticity::alpha (drucker_prager.rs:64-68); dp = {ha, hb, hc, hd, lambda, mu, cohesion}
+// The Drucker-Prager return map runs on the hardware transcendentals (v_exp_f32,
+// v_log_f32, v_sin_f32, about 1-2 ulp) and reciprocals: the reference leaves their
+// precision to the shader compiler, and the parity bar on F and the plastic state is
+// FP32 rounding, not correctly rounded libm results.
 __device__ inline float dp_alpha(const float *dp, float q)
 {
-    float angle = dp[0] + (dp[1] * q - dp[3]) * expf(-dp[2] * q);
-    float sa = sinf(angle);
-    return sqrtf(2.0f / 3.0f) * (2.0f * sa) / (3.0f - sa);
+    float angle = dp[0] + (dp[1] * q - dp[3]) * __expf(-dp[2] * q);
+    float sa = __sinf(angle);
+    return 0.81649658f * (2.0f * sa) * rcp_hw(3.0f - sa);   // sqrt(2/3)
 }
 
 // DruckerPragerPlasticity::project (drucker_prager.rs:224-267 with :121-167).  Updates
@@ -333,31 +347,31 @@ __device__ inline void dp_project(const float *dp, float &pdet, float &q, float 
     svd3(F, U, s, V);
     if (dp[4] == 0.0f) return;
     float alpha = dp_alpha(dp, q);
-    const float d = 3.0f;
-    float lg = lvg / d;
-    f3 strain = mk3(logf(s.x) + lg, logf(s.y) + lg, logf(s.z) + lg);
+    const float d = 3.0f, inv_d3 = 1.0f / 3.0f;
+    float lg = lvg * inv_d3;
+    f3 strain = mk3(__logf(s.x) + lg, __logf(s.y) + lg, __logf(s.z) + lg);
     float tr = strain.x + strain.y + strain.z;
-    float m = tr / d;
+    float m = tr * inv_d3;
     f3 dev = mk3(strain.x - m, strain.y - m, strain.z - m);
     float shifted = tr - dp[6];
     f3 ns;
     float dq;
     if (shifted > 0.0f || (dev.x == 0.0f && dev.y == 0.0f && dev.z == 0.0f)) {
-        float e = expf(dp[6] / d);
+        float e = __expf(dp[6] * inv_d3);
         ns = mk3(e, e, e);
         dq = length(strain);
     } else {
         float dn = length(dev);
-        float gamma = dn + (d * dp[4] + 2.0f * dp[5]) / (2.0f * dp[5]) * shifted * alpha;
+        float gamma = dn + (d * dp[4] + 2.0f * dp[5]) * rcp_hw(2.0f * dp[5]) * shifted * alpha;
         if (gamma <= 0.0f) return;   // elastic: state and F unchanged
-        float k = gamma / dn;
-        ns = mk3(expf(strain.x - dev.x * k), expf(strain.y - dev.y * k), expf(strain.z - dev.z * k));
+        float k = gamma * rcp_hw(dn);
+        ns = mk3(__expf(strain.x - dev.x * k), __expf(strain.y - dev.y * k), __expf(strain.z - dev.z * k));
         dq = gamma;
     }
     float prev_det = s.x * s.y * s.z;
     float new_det = ns.x * ns.y * ns.z;
-    pdet = pdet * prev_det / new_det;
-    lvg = lvg + logf(prev_det) - logf(new_det);
+    pdet = pdet * prev_det * rcp_hw(new_det);
+    lvg = lvg + __logf(prev_det) - __logf(new_det);
     q = q + dq;
     s = ns;
     F = recompose(U, s, V);

@@ -161,9 +161,9 @@ __global__ void k_keys_touch(WorldDev w)
         if (valid) {
             int32_t bx, by, bz;
             uint32_t lx, ly, lz;
-            cell_of(X[i], sc.h, bx, lx);
-            cell_of(Y[i], sc.h, by, ly);
-            cell_of(Z[i], sc.h, bz, lz);
+            cell_of(X[i], sc.h, sc.inv_h, bx, lx);
+            cell_of(Y[i], sc.h, sc.inv_h, by, ly);
+            cell_of(Z[i], sc.h, sc.inv_h, bz, lz);
             if (vid_in_range(bx, by, bz)) {
                 key = pack_vid(bx, by, bz);
             } else {
@@ -525,7 +525,8 @@ struct Fixed {
         const uint32_t ulo = (uint32_t)lo;
         return ((unsigned long long)uhi << 32) | ulo;
     }
-    __device__ static float back(word x, float scale) { return (float)((double)(long long)x / (double)scale); }
+    // scale is a power of two: multiplying by its reciprocal is exact
+    __device__ static float back(word x, double inv_scale) { return (float)((double)(long long)x * inv_scale); }
 };
 
 __device__ inline float fixed_scale(int bits, int e, uint32_t np)
@@ -561,6 +562,8 @@ __device__ inline float wave_max(float v)
 // then add to different nodes at the same instruction instead of serializing on one LDS
 // address; the integer sums do not depend on the order.  CHECK adds the non-finite mask
 // (only for suspect particles, so the common path carries none of the tests).
+constexpr int BAD_ANY = (4 * TILE_NODES + 31) / 32;   // words of the non-finite mask
+
 template <bool CHECK>
 __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int rot,
                                             const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wxy)[3][3],
@@ -591,6 +594,7 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
                 atomicAdd(&tile[2 * TILE_NODES + t], Fixed::to(vc.z, 1.0f));
                 atomicAdd(&tile[3 * TILE_NODES + t], Fixed::to(mw, 1.0f));
                 if (CHECK && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
+                    atomicOr(&bad[BAD_ANY], 1u);
                     atomicOr(&bad[t >> 5], 1u << (t & 31));
                     atomicOr(&bad[(TILE_NODES + t) >> 5], 1u << ((TILE_NODES + t) & 31));
                     atomicOr(&bad[(2 * TILE_NODES + t) >> 5], 1u << ((2 * TILE_NODES + t) & 31));
@@ -605,7 +609,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 {
     using FX = Fixed;
     __shared__ typename FX::word tile[4 * TILE_NODES];
-    __shared__ uint32_t bad[(4 * TILE_NODES + 31) / 32];   // non-finite contributions
+    __shared__ uint32_t bad[BAD_ANY + 1];   // non-finite contributions; bad[BAD_ANY]: any at all
     const uint32_t l = threadIdx.x;
     const float *X = w.in.plane(PX), *Y = w.in.plane(PY), *Z = w.in.plane(PZ);
     const float *VX = w.in.plane(PVX), *VY = w.in.plane(PVY), *VZ = w.in.plane(PVZ);
@@ -645,7 +649,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
         const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
         for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
-        if (l < (4 * TILE_NODES + 31) / 32) bad[l] = 0u;
+        if (l <= BAD_ANY) bad[l] = 0u;
         __syncthreads();
         // Pass 2: scatter.
         for (uint32_t c = 0; c < np; c += WAVE) {
@@ -679,7 +683,9 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                                          C.c1.z != C.c1.z || C.c2.x != C.c2.x || C.c2.y != C.c2.y || C.c2.z != C.c2.z;
                     const bool suspect = any_nan || !(amax < 0x1p100f) || !(bmom < 0x1p100f) || !(bm < 0x1p100f);
                     // Base node of the stencil and its offset from the particle.
-                    const float cx = roundf(px / h) - 1.0f, cy = roundf(py / h) - 1.0f, cz = roundf(pz / h) - 1.0f;
+                    const float ih = sc.inv_h;
+                    const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
+                                cz = round_div(pz, h, ih) - 1.0f;
                     const int bx = (int)cx - 4 * v.x, by = (int)cy - 4 * v.y, bz = (int)cz - 4 * v.z;
                     // A particle's primary block holds its base cell: bx, by, bz are in
                     // [0, 3] for every finite position; anything else contributes nothing.
@@ -687,7 +693,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                     const f3 d0 = mk3(cx * h - px, cy * h - py, cz * h - pz);
                     float wx[3], wy[3], wz[3];
                     {
-                        const float fx = -d0.x / h, fy = -d0.y / h, fz = -d0.z / h;
+                        const float fx = -d0.x * ih, fy = -d0.y * ih, fz = -d0.z * ih;
                         wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
                         wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
                         wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
@@ -719,6 +725,8 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         // (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that block's
         // node order, so the consumer reads each region as one run of float4.
         float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
+        const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
+        const bool any_bad = bad[BAD_ANY] != 0u;
         for (int t = l; t < TILE_NODES; t += WAVE) {
             const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
             const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
@@ -727,11 +735,13 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
             float v4[4];
             #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int k = c * TILE_NODES + t;
-                float val = FX::back(tile[k], c < 3 ? smom : sm);
-                if (bad[k >> 5] & (1u << (k & 31))) val = __builtin_nanf("");
-                v4[c] = val;
+            for (int c = 0; c < 4; ++c) v4[c] = FX::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
+            if (any_bad) {
+                #pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int k = c * TILE_NODES + t;
+                    if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
+                }
             }
             st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
         }
@@ -859,14 +869,16 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
             vel = mk3(0, 0, 0);
             m3 gradv = mzero();
             float div = 0.0f;
-            const float cx = roundf(px / h) - 1.0f, cy = roundf(py / h) - 1.0f, cz = roundf(pz / h) - 1.0f;
+            const float ih = sc.inv_h;
+            const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
+                        cz = round_div(pz, h, ih) - 1.0f;
             const int bx = (int)cx - 4 * bv.x, by = (int)cy - 4 * bv.y, bz = (int)cz - 4 * bv.z;
             const bool inside = (unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u;
             if ((fl & FLAG_ENABLED) && inside) {
                 const f3 ref = mk3(cx * h - px, cy * h - py, cz * h - pz);
                 float wx[3], wy[3], wz[3];
                 {
-                    const float fx = -ref.x / h, fy = -ref.y / h, fz = -ref.z / h;
+                    const float fx = -ref.x * ih, fy = -ref.y * ih, fz = -ref.z * ih;
                     wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
                     wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
                     wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
@@ -895,8 +907,8 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
             }
             // Particle update.
             const float vl = length(vel);
-            if (vl > h / dt) vel = vel / vl * h / dt;
-            vel = vel / (1.0f + cls.damping * dt);
+            if (vl > sc.vlim) vel = vel / vl * h / dt;
+            vel = vel * rcp_hw(1.0f + cls.damping * dt);
             if (fl & FLAG_FIXED) vel = mk3(0, 0, 0);
             npos = mk3(px, py, pz) + vel * dt;
             m3 tau;
@@ -1074,9 +1086,9 @@ __global__ void k_canonical(WorldDev w, Arena prev, uint32_t *pcell_out, uint32_
         if (bv.x == OVERFLOW_VID) { pcell_out[pos] = NONE; continue; }
         int32_t b[3];
         uint32_t loc[3];
-        cell_of(prev.plane(PX)[slot], sc.h, b[0], loc[0]);
-        cell_of(prev.plane(PY)[slot], sc.h, b[1], loc[1]);
-        cell_of(prev.plane(PZ)[slot], sc.h, b[2], loc[2]);
+        cell_of(prev.plane(PX)[slot], sc.h, sc.inv_h, b[0], loc[0]);
+        cell_of(prev.plane(PY)[slot], sc.h, sc.inv_h, b[1], loc[1]);
+        cell_of(prev.plane(PZ)[slot], sc.h, sc.inv_h, b[2], loc[2]);
         pcell_out[pos] = loc[0] + 4 * loc[1] + 16 * loc[2];
         atomicAdd(&buckets[10 * d + loc[2]], 1u);
         for (int i = 1; i < 8; ++i) {
