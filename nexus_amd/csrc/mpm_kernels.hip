@@ -605,15 +605,133 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
     }
 }
 
+// Flush: after the block's scatter, node t of the 6^3 tile goes to the region of the block
+// it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
+// block's node order, so the consumer reads each region as one run of float4.  Ends with
+// the tile free for the next block.
+__device__ __forceinline__ void p2g_flush(const WorldDev &w, const unsigned long long *tile, const uint32_t *bad,
+                                          uint32_t d, float smom, float sm, uint32_t l)
+{
+    __syncthreads();
+    float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
+    const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
+    const bool any_bad = bad[BAD_ANY] != 0u;
+    for (int t = l; t < TILE_NODES; t += WAVE) {
+        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+        const int e = 4 * ex + 2 * ey + ez;
+        const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+        const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+        float v4[4];
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) v4[c] = Fixed::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
+        if (any_bad) {
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int k = c * TILE_NODES + t;
+                if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
+            }
+        }
+        st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
+    }
+    __syncthreads();
+}
+
+// One particle's P2G inputs.
+struct P2GIn {
+    float px, py, pz, m;
+    f3 v;
+    m3 C;
+};
+
+__device__ inline P2GIn p2g_load(const WorldDev &w, uint32_t slot)
+{
+    P2GIn p;
+    p.px = w.in.plane(PX)[slot]; p.py = w.in.plane(PY)[slot]; p.pz = w.in.plane(PZ)[slot];
+    p.m = w.in.plane(PMASS)[slot];
+    p.v = mk3(w.in.plane(PVX)[slot], w.in.plane(PVY)[slot], w.in.plane(PVZ)[slot]);
+    p.C.c0 = mk3(w.in.plane(PC0)[slot], w.in.plane(PC1)[slot], w.in.plane(PC2)[slot]);
+    p.C.c1 = mk3(w.in.plane(PC3)[slot], w.in.plane(PC4)[slot], w.in.plane(PC5)[slot]);
+    p.C.c2 = mk3(w.in.plane(PC6)[slot], w.in.plane(PC7)[slot], w.in.plane(PC8)[slot]);
+    return p;
+}
+
+// Bounds of |m v + C dpt| (|dpt_j| <= 1.5 h) and of m for one particle.
+__device__ inline void p2g_bound(const P2GIn &p, float h, float &bmom, float &bm)
+{
+    float cmax = fmaxf(fmaxf(fabsf(p.C.c0.x), fmaxf(fabsf(p.C.c0.y), fabsf(p.C.c0.z))),
+                       fmaxf(fmaxf(fabsf(p.C.c1.x), fmaxf(fabsf(p.C.c1.y), fabsf(p.C.c1.z))),
+                             fmaxf(fabsf(p.C.c2.x), fmaxf(fabsf(p.C.c2.y), fabsf(p.C.c2.z)))));
+    const float vmax = fmaxf(fabsf(p.v.x), fmaxf(fabsf(p.v.y), fabsf(p.v.z)));
+    bmom = fmaxf(bmom, fabsf(p.m) * vmax + 4.5f * h * cmax);
+    bm = fmaxf(bm, fabsf(p.m));
+}
+
+// Scatter of one enabled particle of block vid into the tile.
+__device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
+                                        const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t l)
+{
+    const float h = sc.h;
+    const float px = p.px, py = p.py, pz = p.pz, m = p.m;
+    const f3 mom = p.v * m;
+    const m3 &C = p.C;
+    // With finite inputs and a finite block bound far below FLT_MAX every contribution is
+    // finite (|contribution| <= bound), so only suspect particles check each contribution
+    // for the non-finite mask.
+    float amax = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+    amax = fmaxf(amax, fmaxf(fabsf(mom.x), fmaxf(fabsf(mom.y), fabsf(mom.z))));
+    amax = fmaxf(amax, fabsf(m));
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 col = k == 0 ? C.c0 : (k == 1 ? C.c1 : C.c2);
+        amax = fmaxf(amax, fmaxf(fabsf(col.x), fmaxf(fabsf(col.y), fabsf(col.z))));
+    }
+    // fmaxf drops NaN operands: test them separately.
+    const bool any_nan = px != px || py != py || pz != pz || mom.x != mom.x || mom.y != mom.y ||
+                         mom.z != mom.z || m != m || C.c0.x != C.c0.x || C.c0.y != C.c0.y ||
+                         C.c0.z != C.c0.z || C.c1.x != C.c1.x || C.c1.y != C.c1.y ||
+                         C.c1.z != C.c1.z || C.c2.x != C.c2.x || C.c2.y != C.c2.y || C.c2.z != C.c2.z;
+    const bool suspect = any_nan || !(amax < 0x1p100f) || !(bmom < 0x1p100f) || !(bm < 0x1p100f);
+    // Base node of the stencil and its offset from the particle.
+    const float ih = sc.inv_h;
+    const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f, cz = round_div(pz, h, ih) - 1.0f;
+    const int bx = (int)cx - 4 * v.x, by = (int)cy - 4 * v.y, bz = (int)cz - 4 * v.z;
+    // A particle's primary block holds its base cell: bx, by, bz are in [0, 3] for every
+    // finite position; anything else contributes nothing.
+    const bool inside = (unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u;
+    const f3 d0 = mk3(cx * h - px, cy * h - py, cz * h - pz);
+    float wx[3], wy[3], wz[3];
+    {
+        const float fx = -d0.x * ih, fy = -d0.y * ih, fz = -d0.z * ih;
+        wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
+        wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
+        wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
+    }
+    // C dpt + m v at node (sx, sy, sz) = (C d0 + m v) + sx (h C.c0) + sy (h C.c1) + sz (h C.c2):
+    // three adds per node instead of a matrix product.  The fixed-point scales are powers of
+    // two, so folding them into the weight changes no bits.
+    const f3 base = mul(C, d0) + mom;
+    const f3 hx = C.c0 * h, hy = C.c1 * h, hz = C.c2 * h;
+    float wxy[3][3];
+    #pragma unroll
+    for (int sy = 0; sy < 3; ++sy) {
+        #pragma unroll
+        for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
+    }
+    const float msm = m * sm;
+    const int rot = (int)(l % 3u);
+    if (inside) {
+        if (suspect) p2g_scatter<true>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm);
+        else p2g_scatter<false>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm);
+    }
+}
+
 __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 {
     using FX = Fixed;
     __shared__ typename FX::word tile[4 * TILE_NODES];
     __shared__ uint32_t bad[BAD_ANY + 1];   // non-finite contributions; bad[BAD_ANY]: any at all
     const uint32_t l = threadIdx.x;
-    const float *X = w.in.plane(PX), *Y = w.in.plane(PY), *Z = w.in.plane(PZ);
-    const float *VX = w.in.plane(PVX), *VY = w.in.plane(PVY), *VZ = w.in.plane(PVZ);
-    const float *M = w.in.plane(PMASS);
     const uint32_t *FL = w.in.uplane(PFLAGS);
     // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
     const uint32_t G = gridDim.x, b = blockIdx.x;
@@ -627,21 +745,36 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         const SceneDev sc = w.scenes[s];
         const float h = sc.h;
         const uint32_t first = w.bfirst[d];
-        // Pass 1: bound of |m v + C dpt| (|dpt_j| <= 1.5 h) and of m over the block.
         float bmom = 0.0f, bm = 0.0f;
+        if (np <= (uint32_t)WAVE) {
+            // One particle per lane: load it once, bound, then scatter from registers.
+            P2GIn p;
+            bool on = false;
+            if (l < np) {
+                const uint32_t slot = w.perm[first + l];
+                on = (FL[slot] & FLAG_ENABLED) != 0u;
+                if (on) {
+                    p = p2g_load(w, slot);
+                    p2g_bound(p, h, bmom, bm);
+                }
+            }
+            bmom = wave_max(bmom);
+            bm = wave_max(bm);
+            const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
+            const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
+            for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
+            if (l <= BAD_ANY) bad[l] = 0u;
+            __syncthreads();
+            if (on) p2g_one(tile, bad, p, v, sc, smom, sm, bmom, bm, l);
+            p2g_flush(w, tile, bad, d, smom, sm, l);
+            continue;
+        }
+        // Pass 1: bounds over the block; pass 2: scatter, 64 particles at a time.
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
-                if (FL[slot] & FLAG_ENABLED) {
-                    const float m = M[slot];
-                    float cmax = 0.0f;
-                    #pragma unroll
-                    for (int k = 0; k < 9; ++k) cmax = fmaxf(cmax, fabsf(w.in.plane(PC0 + k)[slot]));
-                    const float vmax = fmaxf(fabsf(VX[slot]), fmaxf(fabsf(VY[slot]), fabsf(VZ[slot])));
-                    bmom = fmaxf(bmom, fabsf(m) * vmax + 4.5f * h * cmax);
-                    bm = fmaxf(bm, fabsf(m));
-                }
+                if (FL[slot] & FLAG_ENABLED) p2g_bound(p2g_load(w, slot), h, bmom, bm);
             }
         }
         bmom = wave_max(bmom);
@@ -651,101 +784,14 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
         if (l <= BAD_ANY) bad[l] = 0u;
         __syncthreads();
-        // Pass 2: scatter.
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
-                const uint32_t fl = FL[slot];
-                if (fl & FLAG_ENABLED) {
-                    const float px = X[slot], py = Y[slot], pz = Z[slot];
-                    const float m = M[slot];
-                    const f3 mom = mk3(VX[slot], VY[slot], VZ[slot]) * m;
-                    m3 C;
-                    C.c0 = mk3(w.in.plane(PC0)[slot], w.in.plane(PC1)[slot], w.in.plane(PC2)[slot]);
-                    C.c1 = mk3(w.in.plane(PC3)[slot], w.in.plane(PC4)[slot], w.in.plane(PC5)[slot]);
-                    C.c2 = mk3(w.in.plane(PC6)[slot], w.in.plane(PC7)[slot], w.in.plane(PC8)[slot]);
-                    // With finite inputs and a finite block bound far below FLT_MAX every
-                    // contribution is finite (|contribution| <= bound), so only suspect
-                    // particles check each contribution for the non-finite mask.
-                    float amax = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
-                    amax = fmaxf(amax, fmaxf(fabsf(mom.x), fmaxf(fabsf(mom.y), fabsf(mom.z))));
-                    amax = fmaxf(amax, fabsf(m));
-                    #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const f3 col = k == 0 ? C.c0 : (k == 1 ? C.c1 : C.c2);
-                        amax = fmaxf(amax, fmaxf(fabsf(col.x), fmaxf(fabsf(col.y), fabsf(col.z))));
-                    }
-                    // fmaxf drops NaN operands: test them separately.
-                    const bool any_nan = px != px || py != py || pz != pz || mom.x != mom.x || mom.y != mom.y ||
-                                         mom.z != mom.z || m != m || C.c0.x != C.c0.x || C.c0.y != C.c0.y ||
-                                         C.c0.z != C.c0.z || C.c1.x != C.c1.x || C.c1.y != C.c1.y ||
-                                         C.c1.z != C.c1.z || C.c2.x != C.c2.x || C.c2.y != C.c2.y || C.c2.z != C.c2.z;
-                    const bool suspect = any_nan || !(amax < 0x1p100f) || !(bmom < 0x1p100f) || !(bm < 0x1p100f);
-                    // Base node of the stencil and its offset from the particle.
-                    const float ih = sc.inv_h;
-                    const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
-                                cz = round_div(pz, h, ih) - 1.0f;
-                    const int bx = (int)cx - 4 * v.x, by = (int)cy - 4 * v.y, bz = (int)cz - 4 * v.z;
-                    // A particle's primary block holds its base cell: bx, by, bz are in
-                    // [0, 3] for every finite position; anything else contributes nothing.
-                    const bool inside = (unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u;
-                    const f3 d0 = mk3(cx * h - px, cy * h - py, cz * h - pz);
-                    float wx[3], wy[3], wz[3];
-                    {
-                        const float fx = -d0.x * ih, fy = -d0.y * ih, fz = -d0.z * ih;
-                        wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
-                        wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
-                        wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
-                    }
-                    // C dpt + m v at node (sx, sy, sz) = (C d0 + m v) + sx (h C.c0) + sy (h C.c1)
-                    // + sz (h C.c2): three adds per node instead of a matrix product.  The
-                    // fixed-point scales are powers of two, so folding them into the weight
-                    // changes no bits.
-                    const f3 base = mul(C, d0) + mom;
-                    const f3 hx = C.c0 * h, hy = C.c1 * h, hz = C.c2 * h;
-                    float wxy[3][3];
-                    #pragma unroll
-                    for (int sy = 0; sy < 3; ++sy) {
-                        #pragma unroll
-                        for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
-                    }
-                    const float msm = m * sm;
-                    const int rot = (int)(l % 3u);
-                    const float wz0 = wz[0], wz1 = wz[1], wz2 = wz[2];
-                    if (inside) {
-                        if (suspect) p2g_scatter<true>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz0, wz1, wz2, smom, msm);
-                        else p2g_scatter<false>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz0, wz1, wz2, smom, msm);
-                    }
-                }
+                if (FL[slot] & FLAG_ENABLED) p2g_one(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l);
             }
         }
-        __syncthreads();
-        // Flush: node t of the 6^3 tile goes to the region of the block it belongs to
-        // (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that block's
-        // node order, so the consumer reads each region as one run of float4.
-        float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
-        const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
-        const bool any_bad = bad[BAD_ANY] != 0u;
-        for (int t = l; t < TILE_NODES; t += WAVE) {
-            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
-            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-            const int e = 4 * ex + 2 * ey + ez;
-            const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
-            const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
-            float v4[4];
-            #pragma unroll
-            for (int c = 0; c < 4; ++c) v4[c] = FX::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
-            if (any_bad) {
-                #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int k = c * TILE_NODES + t;
-                    if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
-                }
-            }
-            st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
-        }
-        __syncthreads();
+        p2g_flush(w, tile, bad, d, smom, sm, l);
     }
 }
 
