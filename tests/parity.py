@@ -42,11 +42,12 @@ def oracle_canonical_per_particle(o: O.Oracle, n: int):
     """Per particle (primary block vid, local cell) from the oracle's last sort."""
     b = o.blocks()
     vid_of = np.zeros((n, 3), np.int32)
-    for blk in range(b["vid"].shape[0]):
-        start = int(b["first"][blk])
-        cnt = int(b["np"][blk])
-        ids = b["sorted"][start:start + cnt]
-        vid_of[ids] = b["vid"][blk]
+    first = np.asarray(b["first"], np.int64)
+    cnt = np.asarray(b["np"], np.int64)
+    # the primaries of block k are sorted[first[k] : first[k] + np[k]]
+    blk_of = np.repeat(np.arange(cnt.size), cnt)
+    pos = np.repeat(first, cnt) + (np.arange(blk_of.size) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    vid_of[np.asarray(b["sorted"])[pos]] = np.asarray(b["vid"])[blk_of]
     return vid_of, b
 
 
@@ -58,14 +59,15 @@ def compare_sort(gpu_can: dict, o32: O.Oracle, x_before: np.ndarray, cell_width:
     vid_of, b = oracle_canonical_per_particle(o32, n)
     ovid, oloc = O.block_keys(np.asarray(x_before, np.float32), cell_width, 32)
     assert np.array_equal(vid_of, ovid)
-    o_map = {tuple(map(int, b["vid"][i])): i for i in range(b["vid"].shape[0])}
-    g_map = {tuple(map(int, gpu_can["vid"][i])): i for i in range(gpu_can["vid"].shape[0])}
-    assert set(o_map) == set(g_map), f"block sets differ: {len(set(o_map) ^ set(g_map))} blocks"
-    for key, oi in o_map.items():
-        gi = g_map[key]
-        assert int(b["np"][oi]) == int(gpu_can["np"][gi]), key
-        assert int(b["npe"][oi]) == int(gpu_can["npe"][gi]), key
-        assert np.array_equal(b["buckets"][oi], gpu_can["buckets"][gi]), key
+    # blocks matched by virtual id (vectorized: both lists ordered by vid)
+    ov, gv = np.asarray(b["vid"], np.int64), np.asarray(gpu_can["vid"], np.int64)
+    assert ov.shape == gv.shape, f"block counts differ: oracle {ov.shape[0]}, gpu {gv.shape[0]}"
+    oo = np.lexsort((ov[:, 2], ov[:, 1], ov[:, 0]))
+    go = np.lexsort((gv[:, 2], gv[:, 1], gv[:, 0]))
+    assert np.array_equal(ov[oo], gv[go]), "block sets differ"
+    assert np.array_equal(np.asarray(b["np"])[oo], np.asarray(gpu_can["np"])[go])
+    assert np.array_equal(np.asarray(b["npe"])[oo], np.asarray(gpu_can["npe"])[go])
+    assert np.array_equal(np.asarray(b["buckets"])[oo], np.asarray(gpu_can["buckets"])[go])
     gvid = gpu_can["vid"][gpu_can["particle_block"]]
     assert np.array_equal(gvid, ovid)
     cell = oloc[:, 0] + 4 * oloc[:, 1] + 16 * oloc[:, 2]
