@@ -245,5 +245,15 @@ class Server:
             stop.set()
             if worker is not None:
                 worker.join()
+            # On an error, release what was prepared or admitted but not finished.
+            for r in list(active) + list(queue) + list(ready):
+                if r.state is not None:
+                    if r in active:
+                        try:
+                            self.batch.remove(r.state)
+                        except mpm.MpmError:
+                            pass
+                    r.state.close()
+                    r.state = None
         rep.makespan_s = (time.perf_counter() - t0) - (first_arrival if wall else 0.0)
         return rep

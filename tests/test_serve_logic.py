@@ -120,3 +120,14 @@ def test_weighted_percentiles():
     rep = serve.ServeReport(0, 0, 0, 0, 1.0, 1.0, step_latency_ms=[1.0, 10.0], step_weights=[99, 1])
     assert rep._weighted_pct(0.5) == 1.0
     assert rep._weighted_pct(0.995) == 10.0
+
+
+def test_error_releases_states(fake, monkeypatch):
+    """A failing substep propagates, and every prepared or admitted scene is released."""
+    def boom(self):
+        raise RuntimeError("device error")
+    monkeypatch.setattr(FakeBatch, "step", boom)
+    reqs = [serve.Request(0, spec(10), 3), serve.Request(0, spec(20), 3)]
+    with pytest.raises(RuntimeError):
+        serve.Server(None, hbm_budget_bytes=1e12).run(reqs, clock="steps")
+    assert all(r.state is None for r in reqs)
