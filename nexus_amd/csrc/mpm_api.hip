@@ -145,6 +145,8 @@ struct nxm_scene {
     std::vector<ColliderDev> cols;
     std::vector<ClassDev> classes;
     std::map<std::array<uint32_t, 26>, uint32_t> class_index;
+    std::array<uint32_t, 26> last_key{};   // the most recently interned class
+    uint32_t last_class = 0xFFFFFFFFu;
     HostParticles hp;
     bool on_host = true;                // hp state authoritative
     World *world = nullptr;             // resident world (own or a batch's)
@@ -176,6 +178,10 @@ class World {
             drop_graphs();
             for (auto &e : ev_) (void)hipEventDestroy(e);
             for (auto &e : tune_ev_) (void)hipEventDestroy(e);
+            for (auto &c : pins_) {
+                (void)hipEventDestroy(c.ev);
+                (void)hipHostFree(c.p);
+            }
             for (auto &e : ring_) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream_);
         }
@@ -233,39 +239,61 @@ class World {
         return NXM_OK;
     }
 
-    // Appends particles [first, first + count) of sc's host arrays to the input arena.
+    // Appends particles [first, first + count) of sc's host arrays to the input arena of
+    // the next step: gathered into pinned staging and copied on the world's stream, after
+    // any step in flight (which reads the other arena), without waiting for it.
     nxm_status upload(nxm_scene *sc, size_t first, size_t count)
     {
         if (count == 0) return NXM_OK;
-        nxm_status st = sync();
-        if (st != NXM_OK) return st;
-        st = ensure_particles(p_in_ + (uint32_t)count);
-        if (st != NXM_OK) return st;
+        if (p_in_ + count > p_cap_) {
+            nxm_status st = sync();   // the arenas are reallocated: nothing may use them
+            if (st != NXM_OK) return st;
+            st = ensure_particles(p_in_ + (uint32_t)count);
+            if (st != NXM_OK) return st;
+        }
         const HostParticles &hp = sc->hp;
-        std::vector<float> plane(count);
-        Arena a = arena(cur_);
-        uint32_t at = p_in_;
-        auto put = [&](int p, const float *src, int stride, int comp) -> hipError_t {
-            for (size_t i = 0; i < count; ++i) plane[i] = src[(first + i) * stride + comp];
-            return hipMemcpy(a.plane(p) + at, plane.data(), count * sizeof(float), hipMemcpyHostToDevice);
+        constexpr int NUP = NUM_PLANES;
+        char *stage = nullptr;
+        size_t ci = 0;
+        nxm_status st = pinned(NUP * count * 4, &stage, &ci);
+        if (st != NXM_OK) return st;
+        float *f = reinterpret_cast<float *>(stage);
+        auto put = [&](int p, const float *src, int stride, int comp) {
+            float *dst = f + (size_t)p * count;
+            for (size_t i = 0; i < count; ++i) dst[i] = src[(first + i) * stride + comp];
         };
-        for (int c = 0; c < 3; ++c) HIP_TRY(put(PX + c, hp.x.data(), 3, c));
-        for (int c = 0; c < 3; ++c) HIP_TRY(put(PVX + c, hp.v.data(), 3, c));
-        for (int c = 0; c < 9; ++c) HIP_TRY(put(PC0 + c, hp.C.data(), 9, c));
-        for (int c = 0; c < 9; ++c) HIP_TRY(put(PF0 + c, hp.F.data(), 9, c));
-        HIP_TRY(put(PMASS, hp.mass.data(), 1, 0));
-        HIP_TRY(put(PVOL0, hp.vol0.data(), 1, 0));
-        for (int c = 0; c < 3; ++c) HIP_TRY(put(PS0 + c, hp.pstate.data(), 3, c));
-        std::vector<uint32_t> u(count);
-        for (size_t i = 0; i < count; ++i)
-            u[i] = make_flags(hp.enabled[first + i] != 0, hp.fixed[first + i] != 0, hp.cls[first + i], sc->slot);
-        HIP_TRY(hipMemcpy(a.uplane(PFLAGS) + at, u.data(), count * 4, hipMemcpyHostToDevice));
-        for (size_t i = 0; i < count; ++i) u[i] = (uint32_t)(first + i);
-        HIP_TRY(hipMemcpy(a.uplane(PID) + at, u.data(), count * 4, hipMemcpyHostToDevice));
+        for (int c = 0; c < 3; ++c) put(PX + c, hp.x.data(), 3, c);
+        for (int c = 0; c < 3; ++c) put(PVX + c, hp.v.data(), 3, c);
+        for (int c = 0; c < 9; ++c) put(PC0 + c, hp.C.data(), 9, c);
+        for (int c = 0; c < 9; ++c) put(PF0 + c, hp.F.data(), 9, c);
+        put(PMASS, hp.mass.data(), 1, 0);
+        put(PVOL0, hp.vol0.data(), 1, 0);
+        for (int c = 0; c < 3; ++c) put(PS0 + c, hp.pstate.data(), 3, c);
+        uint32_t *uf = reinterpret_cast<uint32_t *>(f + (size_t)PFLAGS * count);
+        uint32_t *ui = reinterpret_cast<uint32_t *>(f + (size_t)PID * count);
+        for (size_t i = 0; i < count; ++i) {
+            uf[i] = make_flags(hp.enabled[first + i] != 0, hp.fixed[first + i] != 0, hp.cls[first + i], sc->slot);
+            ui[i] = (uint32_t)(first + i);
+        }
+        Arena a = arena(cur_);
+        const uint32_t at = p_in_;
+        for (int p = 0; p < NUP; ++p)
+            HIP_TRY(hipMemcpyAsync(a.plane(p) + at, f + (size_t)p * count, count * 4, hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipEventRecord(pins_[ci].ev, stream_));
         sc->ranges.push_back(SceneRange{at, (uint32_t)count});
         p_in_ += (uint32_t)count;
         pin_dirty_ = true;
         return NXM_OK;
+    }
+
+    // Detaches a scene whose state is discarded (the scene is being destroyed): no copy.
+    void drop(nxm_scene *sc)
+    {
+        slots_[sc->slot].state = SLOT_RETIRING;
+        slots_[sc->slot].scene = nullptr;
+        sc->world = nullptr;
+        sc->ranges.clear();
+        dirty_ = true;
     }
 
     // Copies the scene's state back to its host arrays (id order) and detaches it.
@@ -356,10 +384,8 @@ class World {
             if (st != NXM_OK) return st;
         }
         if (pin_dirty_) {
-            Globals gh{};
-            gh.p_in = p_in_;
-            HIP_TRY(hipMemcpyAsync(g_.p, &gh, sizeof(Globals), hipMemcpyHostToDevice, stream_));
-            HIP_TRY(hipStreamSynchronize(stream_));
+            launch_set_particles(g_.p, p_in_, stream_);
+            HIP_TRY(hipGetLastError());
             pin_dirty_ = false;
         }
         if (p_in_ == 0 || b_total_ == 0 || n_regions_ == 0) return NXM_OK;
@@ -594,6 +620,49 @@ class World {
     }
 
   private:
+    // Pinned host staging, reused once the copies that read it have executed.
+    struct PinnedChunk {
+        char *p = nullptr;
+        size_t n = 0;
+        hipEvent_t ev = nullptr;
+    };
+
+    nxm_status pinned(size_t bytes, char **out, size_t *idx)
+    {
+        for (size_t k = 0; k < pins_.size(); ++k) {
+            if (pins_[k].n < bytes) continue;
+            const hipError_t q = hipEventQuery(pins_[k].ev);
+            if (q == hipErrorNotReady) continue;
+            HIP_TRY(q);
+            *out = pins_[k].p;
+            *idx = k;
+            return NXM_OK;
+        }
+        PinnedChunk c;
+        c.n = std::max<size_t>(bytes, (size_t)1 << 20);
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&c.p), c.n, hipHostMallocDefault));
+        hipError_t e = hipEventCreate(&c.ev);
+        if (e != hipSuccess) { (void)hipHostFree(c.p); return hip_fail(e, "hipEventCreate"); }
+        pins_.push_back(c);
+        *out = c.p;
+        *idx = pins_.size() - 1;
+        return NXM_OK;
+    }
+
+    // Copies host bytes to a device buffer on the world's stream through pinned staging.
+    nxm_status put_async(void *dst, const void *src, size_t bytes)
+    {
+        if (bytes == 0) return NXM_OK;
+        char *stage = nullptr;
+        size_t ci = 0;
+        nxm_status st = pinned(bytes, &stage, &ci);
+        if (st != NXM_OK) return st;
+        std::memcpy(stage, src, bytes);
+        HIP_TRY(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipEventRecord(pins_[ci].ev, stream_));
+        return NXM_OK;
+    }
+
     nxm_status ring_events(hipEvent_t **out)
     {
         if (5 * (ring_used_ + 1) > ring_.size()) {
@@ -661,8 +730,6 @@ class World {
     // in slot order and uploads the tables.
     nxm_status rebuild_tables()
     {
-        nxm_status st = sync();
-        if (st != NXM_OK) return st;
         layout_.assign(slots_.size(), SceneDev{});
         std::vector<ClassDev> cls;
         std::vector<ColliderDev> cols;
@@ -696,20 +763,33 @@ class World {
         // the slot count keeps every rank (and each scene's overflow block) in range
         // even when a scene exceeds its capacity (reported as an error).
         (void)b;
+        // Buffers a step in flight may still use are reallocated only after it finished;
+        // otherwise the new tables are copied behind it on the stream.
+        const uint32_t scan_stride = std::max<uint32_t>(hsum, hsum) / 1024u + 2u;
+        const uint32_t nreg = (uint32_t)region_start.size();
+        const bool grow = std::max<size_t>(slots_.size(), 1) > scenes_.n || std::max<size_t>(slots_.size(), 1) > ctr_.n ||
+                          std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(cols.size(), 1) > cols_.n ||
+                          hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
+                          std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
+                          (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n);
+        if (grow) {
+            nxm_status st = sync();
+            if (st != NXM_OK) return st;
+        }
         h_total_ = hsum;
         b_total_ = hsum;
         HIP_TRY(scenes_.reserve(std::max<size_t>(slots_.size(), 1)));
         HIP_TRY(ctr_.reserve(std::max<size_t>(slots_.size(), 1)));
         if (ctr_.n > ctr_zeroed_) {
-            HIP_TRY(hipMemset(ctr_.p, 0, ctr_.n * sizeof(SceneCtr)));
+            HIP_TRY(hipMemsetAsync(ctr_.p, 0, ctr_.n * sizeof(SceneCtr), stream_));
             ctr_zeroed_ = ctr_.n;
         }
         HIP_TRY(classes_.reserve(std::max<size_t>(cls.size(), 1)));
         HIP_TRY(cols_.reserve(std::max<size_t>(cols.size(), 1)));
-        if (!layout_.empty())
-            HIP_TRY(hipMemcpy(scenes_.p, layout_.data(), layout_.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
-        if (!cls.empty()) HIP_TRY(hipMemcpy(classes_.p, cls.data(), cls.size() * sizeof(ClassDev), hipMemcpyHostToDevice));
-        if (!cols.empty()) HIP_TRY(hipMemcpy(cols_.p, cols.data(), cols.size() * sizeof(ColliderDev), hipMemcpyHostToDevice));
+        nxm_status st = put_async(scenes_.p, layout_.data(), layout_.size() * sizeof(SceneDev));
+        if (st == NXM_OK) st = put_async(classes_.p, cls.data(), cls.size() * sizeof(ClassDev));
+        if (st == NXM_OK) st = put_async(cols_.p, cols.data(), cols.size() * sizeof(ColliderDev));
+        if (st != NXM_OK) return st;
         if (b_total_ > bscene_.n) {
             size_t nb = (size_t)b_total_ + b_total_ / 4;
             HIP_TRY(bscene_.reserve(nb));
@@ -723,7 +803,7 @@ class World {
             HIP_TRY(stage_.reserve((size_t)4 * TILE_NODES * nb));
             HIP_TRY(gridv_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
-        scan_stride_ = std::max<uint32_t>(b_total_, h_total_) / 1024u + 2u;
+        scan_stride_ = scan_stride;
         HIP_TRY(scan_state_.reserve(2 * (size_t)scan_stride_));
         if (debug_) HIP_TRY(dbg_mom_.reserve((size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(b_total_, 1)));
         HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
@@ -732,10 +812,9 @@ class World {
         n_regions_ = (uint32_t)region_start.size();
         HIP_TRY(region_start_.reserve(std::max<uint32_t>(n_regions_, 1)));
         HIP_TRY(region_scene_.reserve(std::max<uint32_t>(n_regions_, 1)));
-        if (n_regions_) {
-            HIP_TRY(hipMemcpy(region_start_.p, region_start.data(), n_regions_ * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(region_scene_.p, region_scene.data(), n_regions_ * 4, hipMemcpyHostToDevice));
-        }
+        st = put_async(region_start_.p, region_start.data(), n_regions_ * 4);
+        if (st == NXM_OK) st = put_async(region_scene_.p, region_scene.data(), n_regions_ * 4);
+        if (st != NXM_OK) return st;
         dirty_ = false;
         clean_ = false;
         ++gen_;
@@ -934,6 +1013,7 @@ class World {
     std::vector<float> tune_times_;
     std::vector<size_t> tune_slot_;
     std::vector<hipEvent_t> tune_ev_;
+    std::vector<PinnedChunk> pins_;
     bool graph_on_ = false;
     uint64_t gen_ = 0;                 // bumped whenever a buffer address or table changes
     hipGraphExec_t gexec_[2] = {};     // one per arena parity
@@ -987,9 +1067,16 @@ static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *word
     key[0] = tag;
     for (int i = 0; i < 24; ++i) key[1 + i] = (is_sand(tag) && i < 3) ? 0u : words[i];
     std::memcpy(&key[25], &damping, 4);
+    // Consecutive particles usually share their material: try the last class first.
+    if (sc->last_class != NONE && key == sc->last_key) {
+        *out = sc->last_class;
+        return NXM_OK;
+    }
     auto it = sc->class_index.find(key);
     if (it != sc->class_index.end()) {
         *out = it->second;
+        sc->last_key = key;
+        sc->last_class = it->second;
         return NXM_OK;
     }
     if (sc->classes.size() >= MAX_CLASSES) return fail(NXM_ERR_INVALID_ARGUMENT, "more than 16384 distinct materials in one scene");
@@ -1000,6 +1087,8 @@ static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *word
     uint32_t id = (uint32_t)sc->classes.size();
     sc->classes.push_back(c);
     sc->class_index.emplace(key, id);
+    sc->last_key = key;
+    sc->last_class = id;
     *out = id;
     if (sc->world) sc->world->mark_dirty();
     return NXM_OK;
@@ -1151,7 +1240,7 @@ void nxm_scene_destroy(nxm_scene *sc)
     if (sc->batch) {
         auto &m = sc->batch->members;
         m.erase(std::remove(m.begin(), m.end(), sc), m.end());
-        if (sc->world) (void)sc->world->remove(sc);
+        if (sc->world) sc->world->drop(sc);   // the state is discarded with the scene
     }
     sc->own.reset();
     nxm_ctx *ctx = sc->ctx;

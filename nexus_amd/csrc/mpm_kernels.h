@@ -50,6 +50,7 @@ struct LaunchConfig {
 
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev);
 void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st);
+void launch_set_particles(Globals *g, uint32_t p_in, hipStream_t st);
 void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st);
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,
                       hipStream_t st);

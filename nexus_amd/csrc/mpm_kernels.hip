@@ -1182,6 +1182,13 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 
+__global__ void k_set_particles(Globals *g, uint32_t p_in) { g->p_in = p_in; }
+
+void launch_set_particles(Globals *g, uint32_t p_in, hipStream_t st)
+{
+    k_set_particles<<<1, 1, 0, st>>>(g, p_in);
+}
+
 void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st)
 {
     const int T = 256;

@@ -403,8 +403,14 @@ class Batch:
         self.members.append(state)
 
     def remove(self, state: MpmState) -> None:
+        """Takes the scene out of the batch; its state comes back to the host."""
         _check(self.lib.nxm_batch_remove_scene(self.handle, state.handle))
         self.members.remove(state)
+
+    def discard(self, state: MpmState) -> None:
+        """Destroys a member scene without reading its state back."""
+        self.members.remove(state)
+        state.close()
 
     def step(self, timings: bool = False):
         if timings:

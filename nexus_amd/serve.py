@@ -59,6 +59,9 @@ class ServeReport:
     wait_s: list = field(default_factory=list)               # arrival -> admission, per request
     peak_active: int = 0
     peak_bytes: int = 0
+    step_end_s: list = field(default_factory=list)          # end of each substep (wall clock)
+    step_particles: list = field(default_factory=list)      # resident particles in it
+    arrival_window_s: tuple = (0.0, 0.0)                    # first and last arrival
 
     def _weighted_pct(self, q: float) -> float:
         if not self.step_latency_ms:
@@ -73,6 +76,14 @@ class ServeReport:
     def summary(self) -> dict:
         """Per-substep latency percentiles over (scene, substep) pairs, end-to-end latency
         percentiles over requests, sustained particle-substeps/s over the makespan."""
+        # Throughput while requests keep arriving (the second half of the arrival window,
+        # once the batch has filled), beside the whole run's.
+        a0, a1 = self.arrival_window_s
+        mid = 0.5 * (a0 + a1)
+        ends = np.asarray(self.step_end_s)
+        parts = np.asarray(self.step_particles, np.float64)
+        sel = (ends > mid) & (ends <= a1)
+        loaded = float(parts[sel].sum() / (a1 - mid)) if a1 > mid and sel.any() else float("nan")
         e2e = np.asarray(self.end_to_end_s) if self.end_to_end_s else np.asarray([np.nan])
         wait = np.asarray(self.wait_s) if self.wait_s else np.asarray([np.nan])
         return {
@@ -83,6 +94,7 @@ class ServeReport:
             "makespan_s": self.makespan_s,
             "sustained_particle_substeps_per_s": self.particle_substeps / self.makespan_s if self.makespan_s > 0 else 0.0,
             "busy_particle_substeps_per_s": self.particle_substeps / self.busy_s if self.busy_s > 0 else 0.0,
+            "loaded_particle_substeps_per_s": loaded,
             "substep_latency_ms_p50": self._weighted_pct(0.50),
             "substep_latency_ms_p99": self._weighted_pct(0.99),
             "end_to_end_s_p50": float(np.percentile(e2e, 50)),
@@ -219,6 +231,9 @@ class Server:
                 rep.step_latency_ms.append(1e3 * (te - ts))
                 rep.step_weights.append(len(active))
                 rep.particle_substeps += resident
+                if wall:
+                    rep.step_end_s.append(te - t0)
+                    rep.step_particles.append(resident)
                 done = []
                 for r in active:
                     r.steps += 1
@@ -229,8 +244,7 @@ class Server:
                         r.result = r.state.read_positions()
                     elif r.keep == "particles":
                         r.result = r.state.read_particles()
-                    self.batch.remove(r.state)
-                    r.state.close()
+                    self.batch.discard(r.state)      # results read: the state is not needed
                     r.state = None
                     used -= r.bytes
                     resident -= r.n
@@ -249,11 +263,11 @@ class Server:
             for r in list(active) + list(queue) + list(ready):
                 if r.state is not None:
                     if r in active:
-                        try:
-                            self.batch.remove(r.state)
-                        except mpm.MpmError:
-                            pass
-                    r.state.close()
+                        self.batch.discard(r.state)
+                    else:
+                        r.state.close()
                     r.state = None
         rep.makespan_s = (time.perf_counter() - t0) - (first_arrival if wall else 0.0)
+        if order:
+            rep.arrival_window_s = (order[0].arrival, order[-1].arrival)
         return rep

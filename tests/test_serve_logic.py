@@ -44,6 +44,10 @@ class FakeBatch:
     def remove(self, st):
         self.members.remove(st)
 
+    def discard(self, st):
+        self.members.remove(st)
+        st.close()
+
     def step(self):
         FakeBatch.log.append(len(self.members))
         for m in self.members:
