@@ -40,7 +40,7 @@ typedef int nxm_status;
 #define NXM_ERR_GRID_CAPACITY 3     /* more active blocks than the grid capacity */
 #define NXM_ERR_DOMAIN 4            /* a block coordinate outside the packable range */
 #define NXM_ERR_TOO_MANY_COLLIDERS 5
-#define NXM_ERR_UNSUPPORTED_MODEL 6
+#define NXM_ERR_UNSUPPORTED_MODEL 6 /* model tag above NXM_MODEL_SNOW */
 #define NXM_ERR_DEVICE 7
 #define NXM_ERR_BUSY 8              /* scene is attached to a batch */
 
@@ -50,7 +50,7 @@ typedef int nxm_status;
 #define NXM_MODEL_SAND_LINEAR 2
 #define NXM_MODEL_SAND_NEO_HOOKEAN 3
 #define NXM_MODEL_FLUID 4
-#define NXM_MODEL_SNOW 5            /* accepted by the reference; rejected here */
+#define NXM_MODEL_SNOW 5            /* models/snow.rs; tags above 5 are rejected */
 #define NXM_MODEL_WORDS 24
 
 /* BoundaryCondition::ty (src_mpm_shaders/solver/boundary_condition.rs:4-7). */

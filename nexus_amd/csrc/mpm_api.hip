@@ -96,6 +96,11 @@ namespace nxm {
 
 class World;
 
+// Leading model words that are per-particle plastic state rather than parameters: the
+// sand tags' {det Fp, hardening q, log volume gain} (models/default.rs:60-76) and snow's
+// plastic determinant (models/default.rs:78-110).
+static int pstate_words(uint32_t tag) { return (tag == 2u || tag == 3u) ? 3 : tag == 5u ? 1 : 0; }
+
 // Host copy of a scene.  Particle state lives here while the scene is not resident
 // in a World; static per-particle properties always live here (indexed by id).
 struct HostParticles {
@@ -343,13 +348,13 @@ class World {
             for (int c = 0; c < 9; ++c) HIP_TRY(get(PF0 + c, hp.F.data(), 9, c));
             HIP_TRY(get(PMASS, hp.mass.data(), 1, 0));
             HIP_TRY(get(PVOL0, hp.vol0.data(), 1, 0));
-            // plastic state: only the sand tags carry it
+            // plastic state: the sand tags carry three words, snow one
             std::vector<float> ps(3 * (size_t)r.count);
             for (int c = 0; c < 3; ++c) {
                 HIP_TRY(hipMemcpy(buf.data(), a.plane(PS0 + c) + r.start, r.count * 4, hipMemcpyDeviceToHost));
                 for (uint32_t i = 0; i < r.count; ++i) {
                     uint32_t id = ids[i];
-                    if (hp.tag[id] == 2u || hp.tag[id] == 3u) hp.pstate[3 * (size_t)id + c] = buf[i];
+                    if (c < pstate_words(hp.tag[id])) hp.pstate[3 * (size_t)id + c] = buf[i];
                 }
             }
         }
@@ -1056,16 +1061,14 @@ struct nxm_batch {
 // ======================================================================================
 // Scene helpers
 
-static bool is_sand(uint32_t tag) { return tag == 2u || tag == 3u; }
-
-// Interns the material class of one particle: the model tag, its parameter words (the
-// sand tags' first three words are per-particle plastic state, not parameters) and the
-// damping.
+// Interns the material class of one particle: the model tag, its parameter words
+// (without the per-particle plastic state) and the damping.
 static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *words, float damping, uint32_t *out)
 {
     std::array<uint32_t, 26> key{};
     key[0] = tag;
-    for (int i = 0; i < 24; ++i) key[1 + i] = (is_sand(tag) && i < 3) ? 0u : words[i];
+    const int pw = pstate_words(tag);
+    for (int i = 0; i < 24; ++i) key[1 + i] = i < pw ? 0u : words[i];
     std::memcpy(&key[25], &damping, 4);
     // Consecutive particles usually share their material: try the last class first.
     if (sc->last_class != NONE && key == sc->last_key) {
@@ -1149,7 +1152,8 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
             tag = p->model_tag[k];
             words = p->model_data + 24 * k;
         }
-        if (tag == NXM_MODEL_SNOW) return fail(NXM_ERR_UNSUPPORTED_MODEL, "the snow model is not supported");
+        // The reference's model constructors only produce tags 0-5 (particle_model.rs:282-325).
+        if (tag > NXM_MODEL_SNOW) return fail(NXM_ERR_UNSUPPORTED_MODEL, "unknown model tag");
         std::memcpy(&hp.x[3 * i], pos, 12);
         std::memcpy(&hp.v[3 * i], vel, 12);
         std::memcpy(&hp.F[9 * i], F, 36);
@@ -1165,8 +1169,8 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
         hp.fixed[i] = fx;
         hp.tag[i] = tag;
         std::memcpy(&hp.words[24 * i], words, 96);
-        if (is_sand(tag)) std::memcpy(&hp.pstate[3 * i], words, 12);
-        else hp.pstate[3 * i] = hp.pstate[3 * i + 1] = hp.pstate[3 * i + 2] = 0.0f;
+        hp.pstate[3 * i] = hp.pstate[3 * i + 1] = hp.pstate[3 * i + 2] = 0.0f;
+        std::memcpy(&hp.pstate[3 * i], words, 4 * (size_t)pstate_words(tag));
         nxm_status st = intern_class(sc, tag, words, damp, &hp.cls[i]);
         if (st != NXM_OK) return st;
     }
@@ -1366,7 +1370,7 @@ nxm_status nxm_scene_read_particles_soa(nxm_scene *sc, const nxm_particles_soa *
     if (o->model_data) {
         for (size_t i = 0; i < m; ++i) {
             std::memcpy(o->model_data + 24 * i, &hp.words[24 * i], 96);
-            if (is_sand(hp.tag[i])) std::memcpy(o->model_data + 24 * i, &hp.pstate[3 * i], 12);
+            std::memcpy(o->model_data + 24 * i, &hp.pstate[3 * i], 4 * (size_t)pstate_words(hp.tag[i]));
         }
     }
     return latched(sc, NXM_OK);

@@ -377,6 +377,26 @@ __device__ inline void dp_project(const float *dp, float &pdet, float &q, float 
     F = recompose(U, s, V);
 }
 
+// SnowPlasticity::project and hardening_factor (snow.rs:77-113), sp = {critical
+// compression, critical stretch, hardening}: the singular values are clamped to
+// [1 - theta_c, 1 + theta_s], the removed volume change goes into the plastic
+// determinant (kept in [0.1, 4]) and the elastic moduli scale by
+// exp(hardening (1 - plastic det)).  Returns that factor; F and the plastic determinant
+// are updated in place and (U, s, V) is the SVD of the returned F.
+__device__ inline float snow_project(const float *sp, float &pdet, m3 &F, m3 &U, f3 &s, m3 &V)
+{
+    svd3(F, U, s, V);
+    if (sp[0] <= 0.0f && sp[1] <= 0.0f) return __expf(sp[2] * (1.0f - pdet));
+    const float lo = 1.0f - sp[0], hi = 1.0f + sp[1];
+    const f3 c = mk3(fminf(fmaxf(s.x, lo), hi), fminf(fmaxf(s.y, lo), hi), fminf(fmaxf(s.z, lo), hi));
+    const float prev_det = s.x * s.y * s.z;
+    const float new_det = c.x * c.y * c.z;
+    pdet = fminf(fmaxf(pdet * prev_det / new_det, 0.1f), 4.0f);
+    s = c;
+    F = recompose(U, s, V);
+    return __expf(sp[2] * (1.0f - pdet));
+}
+
 // FluidModel (fluid.rs:54-84)
 __device__ inline float fluid_pressure(const float *fl, float j)
 {

@@ -901,9 +901,11 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
         f3 npos = mk3(px, py, pz);
         const ClassDev &cls = w.classes[sc.class_base + flags_class(fl)];
         const uint32_t tag = cls.tag;
-        const bool sand = tag == 2u || tag == 3u;
+        // Per-particle plastic state: three words for the sand tags, one for snow.
+        const bool sand = tag == 2u || tag == 3u, snow = tag == 5u;
         float ps0 = 0.0f, ps1 = 0.0f, ps2 = 0.0f;
-        if (sand) { ps0 = w.in.plane(PS0)[slot]; ps1 = w.in.plane(PS1)[slot]; ps2 = w.in.plane(PS2)[slot]; }
+        if (sand || snow) ps0 = w.in.plane(PS0)[slot];
+        if (sand) { ps1 = w.in.plane(PS1)[slot]; ps2 = w.in.plane(PS2)[slot]; }
         if (bv.x == OVERFLOW_VID) {
             // No block: keep the particle unchanged (the scene reports an error).
             vel = mk3(w.in.plane(PVX)[slot], w.in.plane(PVY)[slot], w.in.plane(PVZ)[slot]);
@@ -986,6 +988,13 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 break;
             }
             case 4u: tau = fluid_stress(cls.w, F, gradv); break;
+            case 5u: {
+                m3 U, V;
+                f3 sv;
+                const float hard = snow_project(cls.w + 1, ps0, F, U, sv, V);
+                tau = linear_stress_svd(cls.w[4] * hard, cls.w[5] * hard, U, sv);
+                break;
+            }
             default: tau = mzero(); break;
             }
             C = gradv * mass - tau * (vol0 * inv_d * dt);
@@ -1011,7 +1020,8 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
         w.out.plane(PF6)[pos] = F.c2.x; w.out.plane(PF7)[pos] = F.c2.y; w.out.plane(PF8)[pos] = F.c2.z;
         w.out.plane(PMASS)[pos] = mass;
         w.out.plane(PVOL0)[pos] = vol0;
-        if (sand) { w.out.plane(PS0)[pos] = ps0; w.out.plane(PS1)[pos] = ps1; w.out.plane(PS2)[pos] = ps2; }
+        if (sand || snow) w.out.plane(PS0)[pos] = ps0;
+        if (sand) { w.out.plane(PS1)[pos] = ps1; w.out.plane(PS2)[pos] = ps2; }
         w.out.uplane(PFLAGS)[pos] = fl;
         w.out.uplane(PID)[pos] = id;
     }
@@ -1055,16 +1065,21 @@ __device__ inline uint32_t secs_to_int(float dt)
 }
 
 __device__ inline float particle_dt_bound(const ClassDev &cls, float h, float mass, float vol0, const f3 &v,
-                                          const m3 &F, const m3 &A)
+                                          const m3 &F, const m3 &A, float pdet)
 {
     const uint32_t tag = cls.tag;
     float bulk, shear, cfl;
     if (tag <= 1u) { bulk = cls.w[0] + 2.0f * cls.w[1] / 3.0f; shear = cls.w[1]; cfl = cls.w[2]; }
     else if (tag <= 3u) { bulk = cls.w[10] + 2.0f * cls.w[11] / 3.0f; shear = cls.w[11]; cfl = cls.w[12]; }
     else if (tag == 4u) { bulk = cls.w[0] * cls.w[1]; shear = 0.0f; cfl = cls.w[3]; }
+    else if (tag == 5u) {
+        // Snow: the linear elastic backbone with its moduli scaled by the hardening factor.
+        const float hard = expf(cls.w[3] * (1.0f - pdet));
+        bulk = (cls.w[4] + 2.0f * cls.w[5] / 3.0f) * hard; shear = cls.w[5] * hard; cfl = cls.w[6];
+    }
     else { bulk = shear = cfl = 0.0f; }
     float dt = 0.0f;
-    if (tag <= 4u) {
+    if (tag <= 5u) {
         const float density0 = mass / vol0;
         const float density = density0 / fmaxf(det(F), 1.0e-6f);
         const float c = sqrtf((bulk + shear * 4.0f / 3.0f) / density);
@@ -1096,7 +1111,8 @@ __global__ void __launch_bounds__(256) k_timestep_bound(WorldDev w, uint32_t n, 
         A.c2 = mk3(w.in.plane(PC6)[i], w.in.plane(PC7)[i], w.in.plane(PC8)[i]);
         const f3 v = mk3(w.in.plane(PVX)[i], w.in.plane(PVY)[i], w.in.plane(PVZ)[i]);
         const ClassDev &cls = w.classes[sc.class_base + flags_class(fl)];
-        const uint32_t cand = secs_to_int(particle_dt_bound(cls, sc.h, w.in.plane(PMASS)[i], w.in.plane(PVOL0)[i], v, F, A));
+        const float pdet = cls.tag == 5u ? w.in.plane(PS0)[i] : 1.0f;
+        const uint32_t cand = secs_to_int(particle_dt_bound(cls, sc.h, w.in.plane(PMASS)[i], w.in.plane(PVOL0)[i], v, F, A, pdet));
         if (s != cur_s) {
             if (cur_s != NONE) atomicMin(&out[cur_s], cur_min);
             cur_s = s;

@@ -104,6 +104,15 @@ def model_sand(young: float, poisson: float, friction_deg: float | None = None, 
     return tag, _words([1.0, 1.0, 0.0, ha, hb, hc, hd, lam, mu, 0.0, lam, mu, 0.5])
 
 
+def model_snow(young: float, poisson: float, critical_compression: float = 2.5e-2,
+               critical_stretch: float = 7.5e-3, hardening: float = 10.0):
+    """ParticleModel::snow / snow_with_params (src_mpm/solver/particle_model.rs:218-251;
+    defaults :45-49): plastic determinant 1, the yield box, the hardening coefficient and
+    the linear elastic backbone."""
+    lam, mu = lame_lambda_mu(young, poisson)
+    return MODEL_SNOW, _words([1.0, critical_compression, critical_stretch, hardening, lam, mu, 0.5])
+
+
 def model_fluid(bulk: float, gamma: float = 7.0, viscosity: float = 0.001, tensile: float = 0.25):
     """ParticleModel::fluid / water (src_mpm/solver/particle_model.rs:161-178)."""
     return MODEL_FLUID, _words([bulk, gamma, viscosity, 0.5, tensile])

@@ -422,6 +422,30 @@ static m3 dp_project(const REAL dp[7], REAL st[3], m3 F)
     return mmul(mmul(U, mdiag(ns)), mtrans(V));
 }
 
+/* SnowPlasticity::hardening_factor (models/snow.rs:77-79): exp(xi (1 - det Fp)). */
+static REAL snow_hardening(REAL xi, REAL pdet) { return REXP(xi * (K(1.0) - pdet)); }
+
+/* SnowPlasticity::project (models/snow.rs:83-113), sp = {theta_c, theta_s, xi}: clamp the
+ * singular values to [1 - theta_c, 1 + theta_s]; det Fp absorbs the removed volume change
+ * and stays in [0.1, 4].  st[0] = det Fp; returns the projected F, *hard the hardening. */
+static m3 snow_project(const REAL sp[3], REAL st[3], m3 F, REAL *hard)
+{
+    if (sp[0] <= K(0.0) && sp[1] <= K(0.0)) {
+        *hard = snow_hardening(sp[2], st[0]);
+        return F;
+    }
+    m3 U, V;
+    v3 s;
+    svd3_impl(F, &U, &s, &V);
+    const REAL lo = K(1.0) - sp[0], hi = K(1.0) + sp[1];
+    const v3 c = V3(rclamp(s.x, lo, hi), rclamp(s.y, lo, hi), rclamp(s.z, lo, hi));
+    const REAL prev_det = s.x * s.y * s.z;
+    const REAL new_det = c.x * c.y * c.z;
+    st[0] = rclamp(st[0] * prev_det / new_det, K(0.1), K(4.0));
+    *hard = snow_hardening(sp[2], st[0]);
+    return mmul(mmul(U, mdiag(c)), mtrans(V));
+}
+
 /* FluidModel::pressure / kirchoff_stress (models/fluid.rs:54-84) */
 static REAL fluid_pressure(const REAL fl[5], REAL j)
 {
@@ -449,8 +473,8 @@ static m3 fluid_stress(const REAL fl[5], m3 F, m3 gradv)
     return stress;
 }
 
-/* DefaultParticleModel::update (models/default.rs:208-264).  Returns -1 for the
- * snow tag (out of scope), 0 otherwise; F and st are updated in place. */
+/* DefaultParticleModel::update (models/default.rs:208-264).  Returns 0; F and the
+ * per-particle plastic state st are updated in place (unknown tags: zero stress). */
 static int model_update(uint32_t tag, const float *w, REAL st[3], m3 *F, m3 gradv, m3 *tau)
 {
     switch (tag) {
@@ -475,9 +499,15 @@ static int model_update(uint32_t tag, const float *w, REAL st[3], m3 *F, m3 grad
         *tau = fluid_stress(fl, *F, gradv);
         return 0;
     }
-    case 5:
-        *tau = mzero();
-        return -1;
+    case 5: {
+        /* SnowModel: words 0 = det Fp (state), 1-3 plasticity, 4-6 linear elastic backbone
+         * (models/default.rs:78-120, 254-261). */
+        REAL sp[3] = {wordf(w, 1), wordf(w, 2), wordf(w, 3)};
+        REAL hard;
+        *F = snow_project(sp, st, *F, &hard);
+        *tau = linear_stress(wordf(w, 4) * hard, wordf(w, 5) * hard, *F);
+        return 0;
+    }
     default:
         *tau = mzero();
         return 0;
@@ -920,10 +950,11 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
             Fm = mdiag(V3(nd, nd, nd));
         }
         REAL st[3] = {0, 0, 0};
-        if (tag[p] == 2 || tag[p] == 3) { st[0] = pstate[3 * p]; st[1] = pstate[3 * p + 1]; st[2] = pstate[3 * p + 2]; }
+        const int has_state = tag[p] == 2 || tag[p] == 3 || tag[p] == 5;
+        if (has_state) { st[0] = pstate[3 * p]; st[1] = pstate[3 * p + 1]; st[2] = pstate[3 * p + 2]; }
         m3 tau;
         if (model_update(tag[p], params + 24 * (size_t)p, st, &Fm, gradv, &tau) != 0) bad_model = 1;
-        if (tag[p] == 2 || tag[p] == 3) { pstate[3 * p] = st[0]; pstate[3 * p + 1] = st[1]; pstate[3 * p + 2] = st[2]; }
+        if (has_state) { pstate[3 * p] = st[0]; pstate[3 * p + 1] = st[1]; pstate[3 * p + 2] = st[2]; }
         m3 aff = msub(mscale(gradv, mass[p]), mscale(tau, vol0[p] * inv_d * dt));
         int nan_pos = npos.x != npos.x || npos.y != npos.y || npos.z != npos.z;
         if (!nan_pos && mdet(Fm) > K(0.0)) {
@@ -1041,12 +1072,17 @@ uint32_t orc_timestep_bound(int64_t n, const REAL *v, const REAL *C, const REAL 
         case 0: case 1: bulk = (REAL)w[0] + K(2.0) * (REAL)w[1] / K(3.0); shear = (REAL)w[1]; cfl = (REAL)w[2]; break;
         case 2: case 3: bulk = (REAL)w[10] + K(2.0) * (REAL)w[11] / K(3.0); shear = (REAL)w[11]; cfl = (REAL)w[12]; break;
         case 4: bulk = (REAL)w[0] * (REAL)w[1]; shear = K(0.0); cfl = (REAL)w[3]; break;
+        case 5: { /* snow: hardened linear backbone; word 0 holds the current det Fp */
+            const REAL hard = snow_hardening((REAL)w[3], (REAL)w[0]);
+            bulk = ((REAL)w[4] + K(2.0) * (REAL)w[5] / K(3.0)) * hard; shear = (REAL)w[5] * hard; cfl = (REAL)w[6];
+            break;
+        }
         default: bulk = shear = cfl = K(0.0); break;
         }
         const m3 Fm = mload(F + 9 * p);
         const v3 vel = vload(v + 3 * p);
         REAL dt;
-        if (tag[p] <= 4u) {
+        if (tag[p] <= 5u) {
             const REAL density0 = mass[p] / vol0[p];
             const REAL det = mdet(Fm);
             const REAL density = density0 / RFMAX(det, (REAL)1.0e-6f);

@@ -84,10 +84,13 @@ class OracleState:
     @staticmethod
     def from_scene(scene, bits: int = 32) -> "OracleState":
         R = real(bits)
+        # Per-particle plastic state: the sand tags' first three words, snow's first word.
         sand = np.isin(scene.tag, (2, 3))
+        snow = np.asarray(scene.tag) == 5
         words = scene.model.view(np.float32).reshape(-1, 24)
         pstate = np.zeros((scene.n, 3), R)
         pstate[sand] = words[sand, 0:3]
+        pstate[snow, 0] = words[snow, 0]
         sim = np.array([*scene.gravity, scene.dt, scene.cell_width], np.float32)
         return OracleState(
             bits=bits,
@@ -228,6 +231,9 @@ def timestep_bound(state: "OracleState") -> tuple[int, float]:
     """CFL timestep bound of the oracle's current particle state: the reference's
     fixed-point value (1e-12 s units) and the seconds it stands for."""
     s = state
+    params = np.array(s.params, np.float32)
+    snow = s.tag == 5
+    params[snow, 0] = s.pstate[snow, 0]          # current plastic determinant
     u = lib(s.bits).orc_timestep_bound(s.n, _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.enabled),
-                                       _p(s.tag), _p(s.params), ct.c_float(float(s.sim[4])))
+                                       _p(s.tag), _p(params), ct.c_float(float(s.sim[4])))
     return int(u), float(np.float32(u) / np.float32(1.0e12))

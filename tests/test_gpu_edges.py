@@ -75,6 +75,30 @@ def test_boundary_conditions_in_contact(pipeline, bc, friction):
     st.close()
 
 
+def test_snow_compaction_matches_oracle(pipeline):
+    """Snow (models/snow.rs, tag 5) thrown onto a stick floor: within a few substeps the
+    singular values of F leave the yield box [1 - 2.5e-2, 1 + 7.5e-3], so the return map
+    clamps them and the plastic determinant (model word 0) and the hardening change.
+    10 substeps against the oracle: x, v, C, F and the plastic determinant."""
+    h = 1 / 64
+    parts = _block(10, (0.5, 0.3, 0.5), h, scenes.model_snow(1.4e5, 0.2), 400.0, velocity=(0.5, -2.0, 0.2))
+    floor = scenes.cuboid((0.5, 0.2, 0.5), (0.4, 0.05, 0.4), bc=scenes.BC_STICK)
+    sc = scenes.make_scene(parts, h, 1e-4, colliders=floor)
+    st = mpm.MpmState.from_scene(pipeline, sc)
+    for _ in range(10):
+        pipeline.step(st)
+    p = st.read_particles()
+    o32, o64 = oracle_pair(sc, 10)
+    assert np.array_equal(p["enabled"], o32.s.enabled)
+    for f in ("x", "v", "C", "F"):
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+    pdet = p["model"][:, 0].view(np.float32)
+    assert np.abs(pdet - 1.0).max() > 1e-3                      # plasticity acted
+    assert_field_close("plastic det", pdet, o32.s.pstate[:, 0], o64.s.pstate[:, 0])
+    assert np.array_equal(p["model"][:, 1:], sc.model[:, 1:])    # parameters come back unchanged
+    st.close()
+
+
 def test_domain_error_is_reported(pipeline):
     """A particle outside the packable block range (|block x| > 1023) is reported as
     NXM_ERR_DOMAIN at the next synchronizing call; the step still runs for the others."""
@@ -108,9 +132,9 @@ def test_rejected_inputs(pipeline):
     h = 1 / 64
     st = mpm.MpmState(pipeline, 1024, 16)
     st.set_cell_width(h, 1024)
-    snow = _block(2, (0.5, 0.5, 0.5), h, (5, np.zeros(24, np.uint32)), 400.0)
+    unknown = _block(2, (0.5, 0.5, 0.5), h, (6, np.zeros(24, np.uint32)), 400.0)
     with pytest.raises(mpm.MpmError) as e:
-        st.add_particles(scenes.make_scene(snow, h, 1e-4))
+        st.add_particles(scenes.make_scene(unknown, h, 1e-4))
     assert e.value.code == 6
     cols = np.concatenate([scenes.cuboid((0.1 * i, 0, 0), (0.01, 0.01, 0.01)) for i in range(17)])
     with pytest.raises(mpm.MpmError) as e:

@@ -157,8 +157,20 @@ def test_graph_replay_matches_eager(pipeline):
     batch.close()
 
 
+def small_snow(n_side=14, seed=43):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h = 1 / 64
+    s = h / 2
+    idx = np.stack(np.meshgrid(*(np.arange(n_side),) * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    pos = 0.5 - n_side * s / 2 + (idx + 0.5) * s + rng.uniform(-0.25 * s, 0.25 * s, idx.shape)
+    vel = rng.normal(0.0, 0.5, pos.shape)
+    parts = scenes.particles(pos, s / 2, 400.0, scenes.model_snow(1.4e5, 0.2), velocity=vel)
+    return scenes.make_scene(parts, h, 1e-4)
+
+
 @pytest.mark.parametrize("make", [lambda: scenes.elastic_cube(12), lambda: small_sand(8000, seed=41),
-                                  lambda: small_fluid(12000, seed=42)], ids=["cube", "sand", "fluid"])
+                                  lambda: small_fluid(12000, seed=42), small_snow],
+                         ids=["cube", "sand", "fluid", "snow"])
 def test_timestep_bound_matches_oracle(pipeline, make):
     """The CFL bound of the GPU state equals the oracle's FP32 bound of the same state
     (the reference's 1e-12 s fixed point, within FP32 rounding), alone and in a batch."""
@@ -169,6 +181,8 @@ def test_timestep_bound_matches_oracle(pipeline, make):
     o = O.Oracle(sc, 32)
     for f in ("v", "C", "F", "mass", "enabled"):
         getattr(o.s, f)[...] = p[f]
+    snow = p["tag"] == scenes.MODEL_SNOW
+    o.s.pstate[snow, 0] = p["model"][snow, 0].view(np.float32)
     _, secs = O.timestep_bound(o.s)
     got = st.timestep_bound()
     assert abs(got - secs) <= 1e-5 * secs, (got, secs)

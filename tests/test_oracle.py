@@ -173,6 +173,59 @@ def test_fluid_pressure_tait_and_viscosity():
         assert np.allclose(from_colmajor(tau), -p * J * np.eye(3) + 2 * eta * J * dev, rtol=1e-9, atol=1e-9)
 
 
+def test_snow_return_map():
+    """SnowPlasticity::project (models/snow.rs:83-113): singular values inside the box
+    [1 - theta_c, 1 + theta_s] leave F and det Fp alone; outside, they are clamped to the
+    box (F = U diag(clamp s) V^T), det Fp takes the removed volume change (clamped to
+    [0.1, 4]), and the stress is the corotated stress with lambda and mu scaled by
+    exp(xi (1 - det Fp))."""
+    tag, words = scenes.model_snow(1.4e5, 0.2)
+    p = words.view(np.float32).astype(np.float64)
+    tc, ts, xi, lam, mu = p[1], p[2], p[3], p[4], p[5]
+    Q1 = np.linalg.qr(RNG.normal(size=(3, 3)))[0]
+    Q2 = np.linalg.qr(RNG.normal(size=(3, 3)))[0]
+    Q1 *= np.sign(np.linalg.det(Q1))
+    Q2 *= np.sign(np.linalg.det(Q2))
+
+    def corotated(F, l, m):
+        U, s, Vt = np.linalg.svd(F)
+        J = np.prod(s)
+        return 2 * m * (F - U @ Vt) @ F.T + l * (J - 1) * J * np.eye(3)
+
+    # inside the box: elastic
+    s = np.array([1.004, 0.99, 0.98])
+    F = Q1 @ np.diag(s) @ Q2.T
+    rc, Fo, st, tau = O.model_update(tag, words, colmajor(F), np.zeros(9), pstate=[0.9, 0, 0])
+    assert rc == 0 and np.allclose(from_colmajor(Fo), F, atol=1e-13) and np.isclose(st[0], 0.9, rtol=1e-14)
+    hard = np.exp(xi * (1 - 0.9))
+    assert np.allclose(from_colmajor(tau), corotated(F, lam * hard, mu * hard), rtol=1e-9, atol=1e-6)
+    # stretched and compressed beyond the box
+    s = np.array([1.05, 0.9, 0.99])
+    F = Q1 @ np.diag(s) @ Q2.T
+    rc, Fo, st, tau = O.model_update(tag, words, colmajor(F), np.zeros(9), pstate=[1.0, 0, 0])
+    c = np.clip(s, 1 - tc, 1 + ts)
+    assert np.allclose(from_colmajor(Fo), Q1 @ np.diag(c) @ Q2.T, atol=1e-12)
+    pdet = np.prod(s) / np.prod(c)
+    assert np.isclose(st[0], pdet, rtol=1e-12)
+    hard = np.exp(xi * (1 - pdet))
+    assert np.allclose(from_colmajor(tau), corotated(Q1 @ np.diag(c) @ Q2.T, lam * hard, mu * hard),
+                       rtol=1e-9, atol=1e-6)
+    # the plastic determinant is kept in [0.1, 4]
+    s = np.array([0.5, 0.5, 0.5])
+    rc, Fo, st, _ = O.model_update(tag, words, colmajor(Q1 @ np.diag(s) @ Q2.T), np.zeros(9), pstate=[0.2, 0, 0])
+    assert st[0] == 0.1
+    s = np.array([1.6, 1.6, 1.6])
+    rc, Fo, st, _ = O.model_update(tag, words, colmajor(Q1 @ np.diag(s) @ Q2.T), np.zeros(9), pstate=[3.0, 0, 0])
+    assert st[0] == 4.0
+    # no yield box: purely elastic, hardening from the current state
+    t2, w2 = scenes.model_snow(1.4e5, 0.2, critical_compression=0.0, critical_stretch=0.0)
+    F = Q1 @ np.diag([1.2, 0.7, 1.0]) @ Q2.T
+    rc, Fo, st, tau = O.model_update(t2, w2, colmajor(F), np.zeros(9), pstate=[0.5, 0, 0])
+    assert np.allclose(from_colmajor(Fo), F, atol=1e-14) and st[0] == 0.5
+    hard = np.exp(xi * 0.5)
+    assert np.allclose(from_colmajor(tau), corotated(F, lam * hard, mu * hard), rtol=1e-9, atol=1e-5)
+
+
 def dp_setup(friction=30.0):
     tag, words = scenes.model_sand(3.5e5, 0.3, friction_deg=friction)
     p = words.view(np.float32).astype(np.float64)
@@ -411,6 +464,22 @@ def test_timestep_bound_is_p_wave_cfl(young, poisson):
         expect = cfl * h / math.sqrt(M / rho)
         u, secs = O.timestep_bound(o.s)
         assert abs(u - expect * 1e12) <= 1e-6 * expect * 1e12 + 1, (u, expect * 1e12)
+
+
+def test_timestep_bound_snow_hardening():
+    """Snow: the P-wave speed of the elastic backbone with both moduli scaled by the
+    hardening factor exp(xi (1 - det Fp)), so c grows by the factor's square root: a
+    compacted particle (det Fp = 0.8) gets a bound sqrt(exp(0.2 xi)) times smaller."""
+    h = 1.0 / 64
+    young, poisson, rho = 1.4e5, 0.2, 400.0
+    M = young * (1 - poisson) / ((1 + poisson) * (1 - 2 * poisson))
+    sc = _one_particle(scenes.model_snow(young, poisson), density=rho, h=h)
+    o = O.Oracle(sc, 64)
+    expect = 0.5 * h / math.sqrt(M / rho)
+    assert abs(O.timestep_bound(o.s)[0] - expect * 1e12) <= 1e-6 * expect * 1e12 + 1
+    o.s.pstate[0, 0] = 0.8
+    expect = 0.5 * h / math.sqrt(M * math.exp(10.0 * 0.2) / rho)
+    assert abs(O.timestep_bound(o.s)[0] - expect * 1e12) <= 1e-6 * expect * 1e12 + 1
 
 
 def test_timestep_bound_fluid_tait_sound_speed_and_velocity():
