@@ -140,6 +140,8 @@ class Scene:
     colliders: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=CUBOID_DTYPE))
     substeps: int = 1
     name: str = ""
+    cpic: bool = False                       # MpmState::use_cpic (src_mpm/pipeline.rs:69)
+    boundary_friction: np.ndarray | None = None   # (n,) f32, None = 1 (Particle default)
 
     @property
     def n(self) -> int:
@@ -211,10 +213,11 @@ def random_unit_quaternion(rng) -> np.ndarray:
     return q.astype(np.float32)
 
 
-def make_scene(parts: dict, cell_width, dt, *, colliders=None, gravity=(0.0, -9.81, 0.0), substeps=1, name=""):
+def make_scene(parts: dict, cell_width, dt, *, colliders=None, gravity=(0.0, -9.81, 0.0), substeps=1, name="",
+               cpic=False):
     cols = np.zeros(0, dtype=CUBOID_DTYPE) if colliders is None else colliders
     return Scene(cell_width=float(np.float32(cell_width)), dt=float(np.float32(dt)), gravity=gravity,
-                 colliders=cols, substeps=substeps, name=name, **parts)
+                 colliders=cols, substeps=substeps, name=name, cpic=cpic, **parts)
 
 
 # --------------------------------------------------------------------------- #

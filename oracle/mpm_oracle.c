@@ -626,6 +626,164 @@ static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt,
 }
 
 /* ------------------------------------------------------------------------- */
+/* CPIC: contact distance field (solver/grid_update_cdf.rs, g2p_cdf.rs, p2g.rs,
+ * g2p.rs and particle_update.rs with use_cpic).  Affinity words: bit i = collider i is
+ * within reach, bit 16 + i = the side of collider i (grid/grid.rs:593-645). */
+
+#define AFF_MASK 0x0000FFFFu
+#define CPIC_MAX_COLLIDERS 16
+
+static int aff_compatible(uint32_t a, uint32_t b)
+{
+    uint32_t common = a & b & AFF_MASK;
+    return ((a >> 16) & common) == ((b >> 16) & common);
+}
+
+/* gpu_grid_update_cdf's collide (grid_update_cdf.rs:18-60): over the colliders, the
+ * closest (strictly smaller distance wins) of those containing the node or within
+ * 1.5 h on every axis, and their affinity bits with the side (inside) as sign. */
+static void node_cdf(const orc_cuboid *cols, int ncol, REAL h, v3 pt, REAL *dist, uint32_t *aff,
+                     uint32_t *closest)
+{
+    REAL best = MAX_FLT_REF;
+    uint32_t a = 0, cid = NONE;
+    const REAL cap = h * K(1.5);
+    for (int i = 0; i < ncol; ++i) {
+        int inside;
+        v3 proj = project_cuboid(&cols[i], pt, &inside);
+        v3 dpt = vsub(proj, pt);
+        int within = RFABS(dpt.x) <= cap && RFABS(dpt.y) <= cap && RFABS(dpt.z) <= cap;
+        if (inside || within) {
+            REAL d = vlen(dpt);
+            if (d < best) { cid = (uint32_t)i; best = d; }
+            a |= (inside ? 0x00010001u : 0x00000001u) << i;
+        }
+    }
+    *dist = best;
+    *aff = a;
+    *closest = cid;
+}
+
+/* 4x4 determinant and inverse through the 2x2 minors of the first two and the last two
+ * rows (Laplace expansion); m is row-major here (symmetric inputs only). */
+static REAL m4_minors(const REAL m[16], REAL s[6], REAL c[6])
+{
+    s[0] = m[0] * m[5] - m[4] * m[1];
+    s[1] = m[0] * m[6] - m[4] * m[2];
+    s[2] = m[0] * m[7] - m[4] * m[3];
+    s[3] = m[1] * m[6] - m[5] * m[2];
+    s[4] = m[1] * m[7] - m[5] * m[3];
+    s[5] = m[2] * m[7] - m[6] * m[3];
+    c[5] = m[10] * m[15] - m[14] * m[11];
+    c[4] = m[9] * m[15] - m[13] * m[11];
+    c[3] = m[9] * m[14] - m[13] * m[10];
+    c[2] = m[8] * m[15] - m[12] * m[11];
+    c[1] = m[8] * m[14] - m[12] * m[10];
+    c[0] = m[8] * m[13] - m[12] * m[9];
+    return s[0] * c[5] - s[1] * c[4] + s[2] * c[3] + s[3] * c[2] - s[4] * c[1] + s[5] * c[0];
+}
+
+/* adj(m) b / det, with the minors of m4_minors. */
+static void m4_solve(const REAL m[16], const REAL s[6], const REAL c[6], REAL det, const REAL b[4], REAL out[4])
+{
+    REAL inv[16];
+    inv[0] = m[5] * c[5] - m[6] * c[4] + m[7] * c[3];
+    inv[1] = -m[1] * c[5] + m[2] * c[4] - m[3] * c[3];
+    inv[2] = m[13] * s[5] - m[14] * s[4] + m[15] * s[3];
+    inv[3] = -m[9] * s[5] + m[10] * s[4] - m[11] * s[3];
+    inv[4] = -m[4] * c[5] + m[6] * c[2] - m[7] * c[1];
+    inv[5] = m[0] * c[5] - m[2] * c[2] + m[3] * c[1];
+    inv[6] = -m[12] * s[5] + m[14] * s[2] - m[15] * s[1];
+    inv[7] = m[8] * s[5] - m[10] * s[2] + m[11] * s[1];
+    inv[8] = m[4] * c[4] - m[5] * c[2] + m[7] * c[0];
+    inv[9] = -m[0] * c[4] + m[1] * c[2] - m[3] * c[0];
+    inv[10] = m[12] * s[4] - m[13] * s[2] + m[15] * s[0];
+    inv[11] = -m[8] * s[4] + m[9] * s[2] - m[11] * s[0];
+    inv[12] = -m[4] * c[3] + m[5] * c[1] - m[6] * c[0];
+    inv[13] = m[0] * c[3] - m[1] * c[1] + m[2] * c[0];
+    inv[14] = -m[12] * s[3] + m[13] * s[1] - m[14] * s[0];
+    inv[15] = m[8] * s[3] - m[9] * s[1] + m[10] * s[0];
+    REAL r = K(1.0) / det;
+    for (int i = 0; i < 4; ++i)
+        out[i] = (inv[4 * i] * b[0] + inv[4 * i + 1] * b[1] + inv[4 * i + 2] * b[2] + inv[4 * i + 3] * b[3]) * r;
+}
+
+/* One particle's contact distance field (g2p_cdf.rs:150-331) from the node CDFs of its
+ * 3x3x3 stencil: the affinity bits (the union of the nodes', with the side of each newly
+ * reached collider from the weighted signed distances and the previous side kept
+ * otherwise), then the moving-least-squares fit d(x) = n . (x - x_p) + d_p over the
+ * nodes sharing an affinity, signed by whether the node is on the particle's side. */
+static void particle_cdf(v3 refp, REAL h, const REAL *ndist[27], const uint32_t *naff[27], uint32_t prev,
+                         uint32_t *aff, v3 *normal, REAL *sd)
+{
+    v3 w[3] = {qk_eval_all(-refp.x / h), qk_eval_all(-refp.y / h), qk_eval_all(-refp.z / h)};
+    uint32_t pa = 0;
+    REAL signs[CPIC_MAX_COLLIDERS];
+    for (int c = 0; c < CPIC_MAX_COLLIDERS; ++c) signs[c] = K(0.0);
+    for (int i = 0; i < 27; ++i) {
+        int sx = NBH_SHIFTS[i][0], sy = NBH_SHIFTS[i][1], sz = NBH_SHIFTS[i][2];
+        uint32_t a = *naff[i];
+        REAL d = *ndist[i];
+        pa |= a & AFF_MASK;
+        REAL weight = vget(w[0], sx) * vget(w[1], sy) * vget(w[2], sz);
+        for (int c = 0; c < CPIC_MAX_COLLIDERS; ++c) {
+            REAL compatible = (a >> c) & 1u ? K(1.0) : K(0.0);
+            REAL sign = (a >> (16 + c)) & 1u ? K(-1.0) : K(1.0);
+            signs[c] += compatible * weight * sign * d;
+        }
+    }
+    for (int c = 0; c < CPIC_MAX_COLLIDERS; ++c) {
+        if (!((prev >> c) & 1u)) {
+            if (signs[c] < K(0.0)) pa |= 0x00010000u << c;
+        } else {
+            pa |= prev & (0x00010000u << c);
+        }
+    }
+    REAL q[16], u[4];
+    for (int k = 0; k < 16; ++k) q[k] = K(0.0);
+    for (int k = 0; k < 4; ++k) u[k] = K(0.0);
+    for (int i = 0; i < 27; ++i) {
+        int sx = NBH_SHIFTS[i][0], sy = NBH_SHIFTS[i][1], sz = NBH_SHIFTS[i][2];
+        uint32_t a = *naff[i];
+        v3 dpt = vadd(refp, vmul(V3((REAL)sx, (REAL)sy, (REAL)sz), h));
+        REAL weight = vget(w[0], sx) * vget(w[1], sy) * vget(w[2], sz);
+        uint32_t comb = a & pa & AFF_MASK;
+        uint32_t sdiff = ((a >> 16) ^ (pa >> 16)) & comb;
+        if (comb != 0u) {
+            REAL pv[4] = {dpt.x, dpt.y, dpt.z, K(1.0)};
+            REAL d = sdiff == 0u ? *ndist[i] : -*ndist[i];
+            for (int r = 0; r < 4; ++r) {
+                for (int cc = 0; cc < 4; ++cc) q[4 * r + cc] += pv[r] * pv[cc] * weight;
+                u[r] += pv[r] * (weight * d);
+            }
+        }
+    }
+    REAL sm[6], cm[6];
+    REAL det = m4_minors(q, sm, cm);
+    if (det > K(1.0e-8)) {
+        REAL res[4];
+        m4_solve(q, sm, cm, det, u, res);
+        v3 nv = V3(res[0], res[1], res[2]);
+        REAL len = vlen(nv);
+        *normal = len > K(1.0e-6) ? vdiv(nv, len) : V3(0, 0, 0);
+        *sd = res[3];
+        *aff = pa;
+    } else {
+        *normal = V3(0, 0, 0);
+        *sd = K(0.0);
+        *aff = 0u;
+    }
+}
+
+static v3 body_point_velocity(const orc_cuboid *c, v3 pt)
+{
+    v3 com = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 lin = V3(c->linvel[0], c->linvel[1], c->linvel[2]);
+    v3 ang = V3(c->angvel[0], c->angvel[1], c->angvel[2]);
+    return vadd(lin, vcross(ang, vsub(pt, com)));
+}
+
+/* ------------------------------------------------------------------------- */
 /* Workspace. */
 
 struct orc_ws {
@@ -634,6 +792,9 @@ struct orc_ws {
     uint32_t *np, *npe, *first, *buckets, *nbh; /* nb, nb, nb, nb*10, nb*7 */
     uint32_t *sorted;  /* nsorted */
     REAL *mv, *m, *vel;
+    int has_cpic;
+    REAL *ndist, *mv_i, *m_i, *vel_i;   /* CPIC: node CDF distance, incompatible fields */
+    uint32_t *naff, *nclosest;
     int64_t cap_b, cap_s;
 };
 
@@ -648,6 +809,7 @@ void orc_ws_destroy(orc_ws *ws)
     if (!ws) return;
     free(ws->vid); free(ws->np); free(ws->npe); free(ws->first); free(ws->buckets);
     free(ws->nbh); free(ws->sorted); free(ws->mv); free(ws->m); free(ws->vel);
+    free(ws->ndist); free(ws->mv_i); free(ws->m_i); free(ws->vel_i); free(ws->naff); free(ws->nclosest);
     free(ws);
 }
 
@@ -664,6 +826,12 @@ static void ws_reserve(orc_ws *ws, int64_t nb, int64_t ns)
         ws->mv = (REAL *)realloc(ws->mv, sizeof(REAL) * 64 * 3 * c);
         ws->m = (REAL *)realloc(ws->m, sizeof(REAL) * 64 * c);
         ws->vel = (REAL *)realloc(ws->vel, sizeof(REAL) * 64 * 3 * c);
+        ws->ndist = (REAL *)realloc(ws->ndist, sizeof(REAL) * 64 * c);
+        ws->mv_i = (REAL *)realloc(ws->mv_i, sizeof(REAL) * 64 * 3 * c);
+        ws->m_i = (REAL *)realloc(ws->m_i, sizeof(REAL) * 64 * c);
+        ws->vel_i = (REAL *)realloc(ws->vel_i, sizeof(REAL) * 64 * 3 * c);
+        ws->naff = (uint32_t *)realloc(ws->naff, sizeof(uint32_t) * 64 * c);
+        ws->nclosest = (uint32_t *)realloc(ws->nclosest, sizeof(uint32_t) * 64 * c);
         ws->cap_b = c;
     }
     if (ns > ws->cap_s) {
@@ -702,6 +870,19 @@ int orc_ws_grid(const orc_ws *ws, REAL *mv, REAL *m, REAL *vel)
     return 0;
 }
 
+int orc_ws_grid_cpic(const orc_ws *ws, REAL *dist, uint32_t *aff, uint32_t *closest, REAL *mv_i, REAL *m_i,
+                     REAL *vel_i)
+{
+    if (!ws->has_cpic) return -1;
+    memcpy(dist, ws->ndist, sizeof(REAL) * 64 * ws->nb);
+    memcpy(aff, ws->naff, sizeof(uint32_t) * 64 * ws->nb);
+    memcpy(closest, ws->nclosest, sizeof(uint32_t) * 64 * ws->nb);
+    memcpy(mv_i, ws->mv_i, sizeof(REAL) * 64 * 3 * ws->nb);
+    memcpy(m_i, ws->m_i, sizeof(REAL) * 64 * ws->nb);
+    memcpy(vel_i, ws->vel_i, sizeof(REAL) * 64 * 3 * ws->nb);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* One substep. */
 
@@ -710,7 +891,17 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
              const uint32_t *fixed, const uint32_t *tag, const float *params, const float *sim,
              const orc_cuboid *cols, int ncol, int keep_debug)
 {
+    return orc_step_ex(ws, n, x, v, C, F, mass, vol0, damping, pstate, enabled, fixed, tag, params, sim, cols,
+                       ncol, keep_debug, NULL);
+}
+
+int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
+                const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
+                const uint32_t *fixed, const uint32_t *tag, const float *params, const float *sim,
+                const orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp)
+{
     (void)keep_debug;
+    if (cp && ncol > CPIC_MAX_COLLIDERS) return -3;
     const v3 g = V3((REAL)sim[0], (REAL)sim[1], (REAL)sim[2]);
     const REAL dt = (REAL)sim[3];
     const REAL h = (REAL)sim[4];
@@ -801,10 +992,75 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
         }
     }
 
-    /* --- P2G: one gather per node over the block's sorted list (p2g.rs:64-351) --- */
+    /* --- CPIC: node CDFs (grid_update_cdf.rs:60-132), then each primary particle's CDF
+     *     (g2p_cdf.rs:331-407); no mesh colliders, so p2g_cdf adds nothing --- */
+    const int64_t nbn = nb > 0 ? nb : 1;
+    REAL *ndist = NULL;
+    uint32_t *naff = NULL, *nclosest = NULL;
+    if (cp) {
+        ndist = (REAL *)malloc(sizeof(REAL) * 64 * nbn);
+        naff = (uint32_t *)malloc(sizeof(uint32_t) * 64 * nbn);
+        nclosest = (uint32_t *)malloc(sizeof(uint32_t) * 64 * nbn);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < 64 * nb; ++i) {
+            int64_t b = i / 64;
+            int t = (int)(i % 64);
+            const int32_t *vid = bvid + 3 * b;
+            v3 pt = vmul(V3((REAL)(vid[0] * 4 + (t & 3)), (REAL)(vid[1] * 4 + ((t >> 2) & 3)), (REAL)(vid[2] * 4 + (t >> 4))), h);
+            node_cdf(cols, ncol, h, pt, ndist + i, naff + i, nclosest + i);
+        }
+        const REAL none_dist = MAX_FLT_REF;
+        const uint32_t none_aff = 0u;
+#pragma omp parallel
+        {
+            const REAL *tdist[216];
+            const uint32_t *taff[216];
+#pragma omp for schedule(dynamic, 4)
+            for (int64_t b = 0; b < nb; ++b) {
+                if (npb[b] == 0) continue;
+                const int32_t *vid = bvid + 3 * b;
+                for (int o = 0; o < 8; ++o) {
+                    int ox = ASSOC_SHIFT[o][0], oy = ASSOC_SHIFT[o][1], oz = ASSOC_SHIFT[o][2];
+                    int32_t nv[3] = {vid[0] + ox, vid[1] + oy, vid[2] + oz};
+                    uint32_t hid = hmap_find(&hm, pack_vid(nv));
+                    for (int t = 0; t < 64; ++t) {
+                        int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
+                        if ((ox && tx > 1) || (oy && ty > 1) || (oz && tz > 1)) continue;
+                        int si = (ox * 4 + tx) + 6 * (oy * 4 + ty) + 36 * (oz * 4 + tz);
+                        tdist[si] = hid != NONE ? ndist + 64 * (int64_t)hid + t : &none_dist;
+                        taff[si] = hid != NONE ? naff + 64 * (int64_t)hid + t : &none_aff;
+                    }
+                }
+                for (uint32_t k = first[b]; k < first[b] + npb[b]; ++k) {
+                    uint32_t pid = sorted[k];
+                    v3 pos = vload(x + 3 * pid);
+                    v3 refp = vsub(vmul(vsub(V3(RROUND(pos.x / h), RROUND(pos.y / h), RROUND(pos.z / h)), V3(1, 1, 1)), h), pos);
+                    const uint32_t *loc = ploc + 3 * pid;
+                    int packed = (int)(loc[0] + 6 * loc[1] + 36 * loc[2]);
+                    const REAL *sd27[27];
+                    const uint32_t *af27[27];
+                    for (int i = 0; i < 27; ++i) {
+                        int si = packed + NBH_SHIFTS[i][0] + 6 * NBH_SHIFTS[i][1] + 36 * NBH_SHIFTS[i][2];
+                        sd27[i] = tdist[si];
+                        af27[i] = taff[si];
+                    }
+                    v3 nrm;
+                    particle_cdf(refp, h, sd27, af27, cp->affinity[pid], &cp->affinity[pid], &nrm, &cp->signed_distance[pid]);
+                    vstore(cp->normal + 3 * pid, nrm);
+                }
+            }
+        }
+    }
+
+    /* --- P2G: one gather per node over the block's sorted list (p2g.rs:64-351); with
+     *     CPIC, contributions of particles not compatible with the node go to the
+     *     node's second (incompatible) momentum and mass --- */
     REAL *gmv = (REAL *)malloc(sizeof(REAL) * 64 * 3 * (nb > 0 ? nb : 1));
     REAL *gm = (REAL *)malloc(sizeof(REAL) * 64 * (nb > 0 ? nb : 1));
     REAL *gvel = (REAL *)malloc(sizeof(REAL) * 64 * 3 * (nb > 0 ? nb : 1));
+    REAL *gmv_i = cp ? (REAL *)malloc(sizeof(REAL) * 64 * 3 * nbn) : NULL;
+    REAL *gm_i = cp ? (REAL *)malloc(sizeof(REAL) * 64 * nbn) : NULL;
+    REAL *gvel_i = cp ? (REAL *)malloc(sizeof(REAL) * 64 * 3 * nbn) : NULL;
     const REAL inv_h = K(1.0) / h;
 #pragma omp parallel
     {
@@ -817,8 +1073,9 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
             for (int t = 0; t < 64; ++t) {
                 int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
                 v3 cell_pos = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
-                v3 acc_mv = V3(0, 0, 0);
-                REAL acc_m = K(0.0);
+                v3 acc_mv = V3(0, 0, 0), acc_mv_i = V3(0, 0, 0);
+                REAL acc_m = K(0.0), acc_m_i = K(0.0);
+                const uint32_t node_aff = cp ? naff[64 * b + t] : 0u;
                 for (uint32_t ch = 0; ch < num_chunks; ++ch) {
                     uint32_t base = fst + ch * 64;
                     uint32_t len = last - base < 64 ? last - base : 64;
@@ -846,13 +1103,22 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
                         REAL w = qk_eval(dpt.x * inv_h) * qk_eval(dpt.y * inv_h) * qk_eval(dpt.z * inv_h);
                         if (w != K(0.0)) {
                             v3 vc = vmul(vadd(mvmul(aff, dpt), mom), w);
-                            acc_mv = vadd(acc_mv, vc);
-                            acc_m = acc_m + pm * w;
+                            if (cp && !aff_compatible(cp->affinity[pid], node_aff)) {
+                                acc_mv_i = vadd(acc_mv_i, vc);
+                                acc_m_i = acc_m_i + pm * w;
+                            } else {
+                                acc_mv = vadd(acc_mv, vc);
+                                acc_m = acc_m + pm * w;
+                            }
                         }
                     }
                 }
                 vstore(gmv + 3 * (64 * b + t), acc_mv);
                 gm[64 * b + t] = acc_m;
+                if (cp) {
+                    vstore(gmv_i + 3 * (64 * b + t), acc_mv_i);
+                    gm_i[64 * b + t] = acc_m_i;
+                }
             }
         }
     }
@@ -871,8 +1137,19 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
         v3 vel = vmul(vadd(mom, vmul(g, m * dt)), inv_mass);
         vel = V3(RFMIN(RFMAX(vel.x, -vlim), vlim), RFMIN(RFMAX(vel.y, -vlim), vlim),
                  RFMIN(RFMAX(vel.z, -vlim), vlim));
-        v3 pt = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
-        vel = collide_node(cols, ncol, h, dt, pt, vel);
+        if (cp) {
+            /* CPIC: both velocity fields, no collider pass (grid_update.rs:55-66,
+             * solver/grid_update.rs launch) */
+            v3 mom_i = vload(gmv_i + 3 * i);
+            REAL mi = gm_i[i];
+            REAL inv_mi = mi > K(0.0) ? K(1.0) / mi : K(0.0);
+            v3 vi = vmul(vadd(mom_i, vmul(g, mi * dt)), inv_mi);
+            vi = V3(RFMIN(RFMAX(vi.x, -vlim), vlim), RFMIN(RFMAX(vi.y, -vlim), vlim), RFMIN(RFMAX(vi.z, -vlim), vlim));
+            vstore(gvel_i + 3 * i, vi);
+        } else {
+            v3 pt = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
+            vel = collide_node(cols, ncol, h, dt, pt, vel);
+        }
         vstore(gvel + 3 * i, vel);
     }
 
@@ -883,7 +1160,8 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
     const REAL inv_d = qk_inv_d(h);
 #pragma omp parallel
     {
-        v3 tile[216];
+        v3 tile[216], tile_i[216];
+        uint32_t tile_aff[216], tile_cid[216];
 #pragma omp for schedule(dynamic, 4)
         for (int64_t b = 0; b < nb; ++b) {
             if (npb[b] == 0) continue;
@@ -897,6 +1175,11 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
                     if ((ox && tx > 1) || (oy && ty > 1) || (oz && tz > 1)) continue;
                     int si = (ox * 4 + tx) + 6 * (oy * 4 + ty) + 36 * (oz * 4 + tz);
                     tile[si] = hid != NONE ? vload(gvel + 3 * (64 * (int64_t)hid + t)) : V3(0, 0, 0);
+                    if (cp) {
+                        tile_i[si] = hid != NONE ? vload(gvel_i + 3 * (64 * (int64_t)hid + t)) : V3(0, 0, 0);
+                        tile_aff[si] = hid != NONE ? naff[64 * (int64_t)hid + t] : 0u;
+                        tile_cid[si] = hid != NONE ? nclosest[64 * (int64_t)hid + t] : NONE;
+                    }
                 }
             }
             for (uint32_t k = first[b]; k < first[b] + npb[b]; ++k) {
@@ -912,13 +1195,38 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
                     int packed = (int)(loc[0] + 6 * loc[1] + 36 * loc[2]);
                     for (int i = 0; i < 27; ++i) {
                         int sx = NBH_SHIFTS[i][0], sy = NBH_SHIFTS[i][1], sz = NBH_SHIFTS[i][2];
-                        v3 cv = tile[packed + sx + 6 * sy + 36 * sz];
+                        const int si = packed + sx + 6 * sy + 36 * sz;
+                        v3 cv = tile[si];
                         v3 dpt = vadd(refp, vmul(V3((REAL)sx, (REAL)sy, (REAL)sz), h));
+                        if (cp && !aff_compatible(cp->affinity[pid], tile_aff[si])) {
+                            /* g2p.rs:206-233: the incompatible field, seen through the
+                             * closest collider's boundary condition (its friction scaled by
+                             * the particle's) relative to the body's velocity there */
+                            cv = tile_i[si];
+                            uint32_t cid = tile_cid[si];
+                            if (cid != NONE) {
+                                const orc_cuboid *c = &cols[cid];
+                                v3 bpv = body_point_velocity(c, vadd(dpt, pos));
+                                v3 nrm = vload(cp->normal + 3 * pid);
+                                cv = vadd(bpv, bc_project(c->bc_type, (REAL)c->bc_friction * cp->boundary_friction[pid],
+                                                          vsub(cv, bpv), nrm));
+                            }
+                        }
                         REAL weight = vget(w[0], sx) * vget(w[1], sy) * vget(w[2], sz);
                         vel = vadd(vel, vmul(cv, weight));
                         grad = madd(grad, mscale(mouter(cv, dpt), weight * inv_d));
                         div = div + weight * inv_d * vdot(cv, dpt);
                     }
+                    if (cp) {
+                        /* g2p.rs:248-258: the bodies' velocities at the particle, summed over
+                         * its affinities */
+                        v3 rv = V3(0, 0, 0);
+                        for (int c = 0; c < ncol; ++c)
+                            if ((cp->affinity[pid] >> c) & 1u) rv = vadd(rv, body_point_velocity(&cols[c], pos));
+                        vstore(cp->rigid_vel + 3 * pid, rv);
+                    }
+                } else if (cp) {
+                    vstore(cp->rigid_vel + 3 * pid, V3(0, 0, 0));
                 }
                 vstore(pv + 3 * pid, vel);
                 mstore(pg + 9 * pid, grad);
@@ -927,18 +1235,29 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
         }
     }
 
-    /* --- particle update (particle_update.rs:60-201), non-CPIC: cdf = 0 --- */
+    /* --- particle update (particle_update.rs:60-201); the CDF terms only with CPIC --- */
     int bad_model = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad_model)
     for (int64_t p = 0; p < n; ++p) {
         v3 vel = vload(pv + 3 * p);
         m3 gradv = mload(pg + 9 * p);
+        /* CPIC reprojection of a penetrating particle (particle_update.rs:92-102) */
+        const int penetrating = cp && cp->signed_distance[p] < K(-0.05) * h;
+        if (penetrating) {
+            v3 rv = vload(cp->rigid_vel + 3 * p);
+            vel = vadd(rv, bc_project(1u, K(0.0), vsub(vel, rv), vload(cp->normal + 3 * p)));
+        }
         REAL vl = vlen(vel);
         if (vl > h / dt) vel = vdiv(vmul(vdiv(vel, vl), h), dt);
         vel = vdiv(vel, K(1.0) + damping[p] * dt);
         if (fixed[p]) vel = V3(0, 0, 0);
         v3 pos = vload(x + 3 * p);
         v3 npos = vadd(pos, vmul(vel, dt));
+        if (penetrating) {
+            /* penalty impulse (particle_update.rs:126-136) */
+            REAL corrected = RFMAX(cp->signed_distance[p], K(-0.3) * h);
+            vel = vadd(vel, vmul(vload(cp->normal + 3 * p), dt * -corrected * K(1.0e3)));
+        }
         m3 Fm = mload(F + 9 * p);
         if (tag[p] != 4) {
             Fm = madd(Fm, mmul(mscale(gradv, dt), Fm));
@@ -987,7 +1306,17 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
     memcpy(ws->mv, gmv, sizeof(REAL) * 64 * 3 * nb);
     memcpy(ws->m, gm, sizeof(REAL) * 64 * nb);
     memcpy(ws->vel, gvel, sizeof(REAL) * 64 * 3 * nb);
+    ws->has_cpic = cp != NULL;
+    if (cp) {
+        memcpy(ws->ndist, ndist, sizeof(REAL) * 64 * nb);
+        memcpy(ws->naff, naff, sizeof(uint32_t) * 64 * nb);
+        memcpy(ws->nclosest, nclosest, sizeof(uint32_t) * 64 * nb);
+        memcpy(ws->mv_i, gmv_i, sizeof(REAL) * 64 * 3 * nb);
+        memcpy(ws->m_i, gm_i, sizeof(REAL) * 64 * nb);
+        memcpy(ws->vel_i, gvel_i, sizeof(REAL) * 64 * 3 * nb);
+    }
 
+    free(ndist); free(naff); free(nclosest); free(gmv_i); free(gm_i); free(gvel_i);
     free(pv); free(pg); free(pdiv);
     free(gmv); free(gm); free(gvel);
     free(sorted); free(cur); free(npe); free(npb); free(first); free(cnt); free(nbh);
@@ -1020,6 +1349,34 @@ int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F, c
     mstore(F, Fm);
     mstore(tau, t);
     return rc;
+}
+
+int orc_node_cdf(const orc_cuboid *cols, int ncol, float cell_width, const REAL *pt, REAL *dist, uint32_t *aff,
+                 uint32_t *closest)
+{
+    if (ncol > CPIC_MAX_COLLIDERS) return -3;
+    node_cdf(cols, ncol, (REAL)cell_width, vload(pt), dist, aff, closest);
+    return 0;
+}
+
+int orc_particle_cdf(const REAL *refp, float cell_width, const REAL *dist27, const uint32_t *aff27, uint32_t prev,
+                     uint32_t *aff, REAL *normal, REAL *sd)
+{
+    const REAL *d[27];
+    const uint32_t *a[27];
+    for (int i = 0; i < 27; ++i) { d[i] = dist27 + i; a[i] = aff27 + i; }
+    v3 n;
+    particle_cdf(vload(refp), (REAL)cell_width, d, a, prev, aff, &n, sd);
+    vstore(normal, n);
+    return 0;
+}
+
+REAL orc_m4_solve(const REAL *m, const REAL *b, REAL *out)
+{
+    REAL s6[6], c6[6];
+    REAL det = m4_minors(m, s6, c6);
+    m4_solve(m, s6, c6, det, b, out);
+    return det;
 }
 
 int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *inside)

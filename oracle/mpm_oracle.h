@@ -1,7 +1,7 @@
 /*
  * mpm_oracle.h — TEST INFRASTRUCTURE ONLY.
  *
- * CPU restatement of dimforge/nexus's non-CPIC sparse-grid MLS-MPM substep
+ * CPU restatement of dimforge/nexus's sparse-grid MLS-MPM substep
  * (binning/sort -> P2G -> grid update + cuboid colliders -> G2P -> particle
  * update with the constitutive models).  It is the checker for the HIP
  * library in nexus_amd/: only tests/, __graft_entry__.smoke() and bench.py's
@@ -60,6 +60,31 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
              const uint32_t *fixed, const uint32_t *tag, const float *params,
              const float *sim, const orc_cuboid *cols, int ncol, int keep_debug);
 
+/*
+ * CPIC state (use_cpic = true in the reference, src_mpm/pipeline.rs:412-460): per
+ * particle the affinity word (bit i: collider i within reach, bit 16 + i: its side;
+ * read as the previous substep's and overwritten), and the outputs of the substep's
+ * contact distance field: the normal (n*3), signed distance (n) and the bodies'
+ * velocity at the particle (rigid_vel, n*3).  boundary_friction (n) scales the
+ * colliders' friction per particle.  At most 16 colliders.
+ */
+typedef struct {
+    uint32_t *affinity;
+    REAL *normal;
+    REAL *signed_distance;
+    REAL *rigid_vel;
+    const REAL *boundary_friction;
+} orc_cpic;
+
+/* orc_step with the CPIC transfers when cp != NULL: node contact distance field,
+ * particle CDF, compatible / incompatible P2G, both grid fields (no collider pass),
+ * CPIC G2P and the particle update's reprojection and penalty.  Returns -3 for more
+ * than 16 colliders. */
+int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
+                const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
+                const uint32_t *fixed, const uint32_t *tag, const float *params,
+                const float *sim, const orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp);
+
 /* Inspection of the last substep run with keep_debug != 0. */
 int64_t orc_ws_nblocks(const orc_ws *ws);
 int64_t orc_ws_nprimary_blocks(const orc_ws *ws);
@@ -72,6 +97,12 @@ int orc_ws_sorted_ids(const orc_ws *ws, uint32_t *ids);
  * collider boundary (vel: nb*64*3).  Node order inside a block: x + 4y + 16z. */
 int orc_ws_grid(const orc_ws *ws, REAL *mv, REAL *m, REAL *vel);
 
+/* CPIC grid data of the last substep (orc_step_ex with cp): node CDF distance, affinity
+ * word and closest collider (NONE = 0xFFFFFFFF) per node (nb*64), incompatible momentum
+ * and mass after P2G, incompatible velocity after the grid update.  -1 without CPIC. */
+int orc_ws_grid_cpic(const orc_ws *ws, REAL *dist, uint32_t *aff, uint32_t *closest,
+                     REAL *mv_i, REAL *m_i, REAL *vel_i);
+
 /* Building blocks exposed for unit tests. */
 int orc_svd3(const REAL *F, REAL *U, REAL *s, REAL *V);
 int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F,
@@ -81,6 +112,14 @@ int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim,
                      const REAL *node_pos, REAL *vel);
 int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid,
                    uint32_t *local);
+/* Node CDF at pt (one collider pass of grid_update_cdf); particle CDF from the 27 stencil
+ * nodes in NBH_SHIFTS order (refp = base node - particle); 4x4 solve m x = b (m
+ * row-major), returning det m. */
+int orc_node_cdf(const orc_cuboid *cols, int ncol, float cell_width, const REAL *pt, REAL *dist,
+                 uint32_t *aff, uint32_t *closest);
+int orc_particle_cdf(const REAL *refp, float cell_width, const REAL *dist27, const uint32_t *aff27,
+                     uint32_t prev, uint32_t *aff, REAL *normal, REAL *sd);
+REAL orc_m4_solve(const REAL *m, const REAL *b, REAL *out);
 /* CFL timestep bound over the enabled particles, in units of 1e-12 s (the reference's
  * GpuTimestepBounds::computed_max_dt_as_uint); u32::MAX when none is enabled. */
 uint32_t orc_timestep_bound(int64_t n, const REAL *v, const REAL *C, const REAL *F, const REAL *mass,

@@ -35,6 +35,16 @@ def lib(bits: int = 32) -> ct.CDLL:
     L.orc_ws_destroy.argtypes = [P]
     L.orc_step.restype = ct.c_int
     L.orc_step.argtypes = [P, ct.c_int64] + [P] * 14 + [ct.c_int, ct.c_int]
+    L.orc_step_ex.restype = ct.c_int
+    L.orc_step_ex.argtypes = [P, ct.c_int64] + [P] * 14 + [ct.c_int, ct.c_int, P]
+    L.orc_ws_grid_cpic.restype = ct.c_int
+    L.orc_ws_grid_cpic.argtypes = [P] * 7
+    L.orc_node_cdf.restype = ct.c_int
+    L.orc_node_cdf.argtypes = [P, ct.c_int, ct.c_float, P, P, P, P]
+    L.orc_particle_cdf.restype = ct.c_int
+    L.orc_particle_cdf.argtypes = [P, ct.c_float, P, P, ct.c_uint32, P, P, P]
+    L.orc_m4_solve.restype = ct.c_double if bits == 64 else ct.c_float
+    L.orc_m4_solve.argtypes = [P, P, P]
     for f in ("orc_ws_nblocks", "orc_ws_nprimary_blocks", "orc_ws_nsorted"):
         getattr(L, f).restype = ct.c_int64
         getattr(L, f).argtypes = [P]
@@ -61,6 +71,11 @@ def real(bits: int):
     return np.float32 if bits == 32 else np.float64
 
 
+class _Cpic(ct.Structure):
+    _fields_ = [("affinity", ct.c_void_p), ("normal", ct.c_void_p), ("signed_distance", ct.c_void_p),
+                ("rigid_vel", ct.c_void_p), ("boundary_friction", ct.c_void_p)]
+
+
 @dataclass
 class OracleState:
     """Particle state held by the oracle in its own precision."""
@@ -80,6 +95,12 @@ class OracleState:
     params: np.ndarray  # (n, 24) f32 view of the model words
     sim: np.ndarray     # gx, gy, gz, dt, h (f32)
     colliders: np.ndarray
+    cpic: bool = False
+    affinity: np.ndarray | None = None          # CPIC per-particle state and outputs
+    normal: np.ndarray | None = None
+    signed_distance: np.ndarray | None = None
+    rigid_vel: np.ndarray | None = None
+    boundary_friction: np.ndarray | None = None
 
     @staticmethod
     def from_scene(scene, bits: int = 32) -> "OracleState":
@@ -102,6 +123,11 @@ class OracleState:
             fixed=np.array(scene.fixed, np.uint32), tag=np.array(scene.tag, np.uint32),
             params=np.array(words, np.float32), sim=sim,
             colliders=np.array(scene.colliders),
+            cpic=bool(getattr(scene, "cpic", False)),
+            affinity=np.zeros(scene.n, np.uint32), normal=np.zeros((scene.n, 3), R),
+            signed_distance=np.zeros(scene.n, R), rigid_vel=np.zeros((scene.n, 3), R),
+            boundary_friction=(np.ones(scene.n, R) if getattr(scene, "boundary_friction", None) is None
+                               else np.array(scene.boundary_friction, R)),
         )
 
     @property
@@ -126,11 +152,16 @@ class Oracle:
 
     def step(self, count: int = 1, keep_debug: bool = True) -> None:
         s = self.s
+        cp = None
+        if s.cpic:
+            cp = _Cpic(_p(s.affinity), _p(s.normal), _p(s.signed_distance), _p(s.rigid_vel),
+                       _p(s.boundary_friction))
         for _ in range(count):
-            rc = self.L.orc_step(
+            rc = self.L.orc_step_ex(
                 self.ws, s.n, _p(s.x), _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.damping),
                 _p(s.pstate), _p(s.enabled), _p(s.fixed), _p(s.tag), _p(s.params), _p(s.sim),
-                _p(s.colliders) if s.colliders.size else None, int(s.colliders.size), int(keep_debug))
+                _p(s.colliders) if s.colliders.size else None, int(s.colliders.size), int(keep_debug),
+                ct.byref(cp) if cp is not None else None)
             if rc != 0:
                 raise RuntimeError(f"oracle step failed: {rc}")
 
@@ -157,7 +188,59 @@ class Oracle:
         m = np.zeros((nb, 64), R)
         vel = np.zeros((nb, 64, 3), R)
         L.orc_ws_grid(ws, _p(mv), _p(m), _p(vel))
-        return dict(mv=mv, m=m, vel=vel)
+        out = dict(mv=mv, m=m, vel=vel)
+        if self.s.cpic:
+            dist = np.zeros((nb, 64), R)
+            aff = np.zeros((nb, 64), np.uint32)
+            closest = np.zeros((nb, 64), np.uint32)
+            mv_i = np.zeros((nb, 64, 3), R)
+            m_i = np.zeros((nb, 64), R)
+            vel_i = np.zeros((nb, 64, 3), R)
+            L.orc_ws_grid_cpic(ws, _p(dist), _p(aff), _p(closest), _p(mv_i), _p(m_i), _p(vel_i))
+            out.update(cdf_dist=dist, cdf_aff=aff, cdf_closest=closest, mv_i=mv_i, m_i=m_i, vel_i=vel_i)
+        return out
+
+
+def node_cdf(cols: np.ndarray, cell_width: float, pt, bits: int = 64):
+    """(distance, affinity word, closest collider or 0xFFFFFFFF) of one grid node."""
+    R = real(bits)
+    p = np.ascontiguousarray(pt, R)
+    d = np.zeros(1, R)
+    a = np.zeros(1, np.uint32)
+    c = np.zeros(1, np.uint32)
+    rc = lib(bits).orc_node_cdf(_p(cols) if cols.size else None, int(cols.size), ct.c_float(cell_width), _p(p),
+                                _p(d), _p(a), _p(c))
+    assert rc == 0
+    return float(d[0]), int(a[0]), int(c[0])
+
+
+def particle_cdf(refp, cell_width: float, dist27, aff27, prev: int, bits: int = 64):
+    """(affinity word, normal, signed distance) of a particle from its 27 stencil nodes
+    (NBH_SHIFTS order; refp = base node - particle)."""
+    R = real(bits)
+    r = np.ascontiguousarray(refp, R)
+    d = np.ascontiguousarray(dist27, R)
+    a = np.ascontiguousarray(aff27, np.uint32)
+    aff = np.zeros(1, np.uint32)
+    n = np.zeros(3, R)
+    sd = np.zeros(1, R)
+    lib(bits).orc_particle_cdf(_p(r), ct.c_float(cell_width), _p(d), _p(a), ct.c_uint32(prev), _p(aff), _p(n), _p(sd))
+    return int(aff[0]), n, float(sd[0])
+
+
+def m4_solve(m: np.ndarray, b: np.ndarray, bits: int = 64):
+    R = real(bits)
+    mm = np.ascontiguousarray(m, R).reshape(16)
+    bb = np.ascontiguousarray(b, R)
+    out = np.zeros(4, R)
+    det = lib(bits).orc_m4_solve(_p(mm), _p(bb), _p(out))
+    return float(det), out
+
+
+NBH_SHIFTS = np.array([
+    (2, 2, 2), (2, 0, 2), (2, 1, 2), (0, 2, 2), (0, 0, 2), (0, 1, 2), (1, 2, 2), (1, 0, 2), (1, 1, 2),
+    (2, 2, 0), (2, 0, 0), (2, 1, 0), (0, 2, 0), (0, 0, 0), (0, 1, 0), (1, 2, 0), (1, 0, 0), (1, 1, 0),
+    (2, 2, 1), (2, 0, 1), (2, 1, 1), (0, 2, 1), (0, 0, 1), (0, 1, 1), (1, 2, 1), (1, 0, 1), (1, 1, 1)])
 
 
 def canonical_sort(blocks: dict) -> dict:
