@@ -184,9 +184,23 @@ nxm_status nxm_scene_set_cell_width(nxm_scene *scene, float cell_width, uint32_t
 /* GpuParticles::append (src_mpm/solver/particle.rs:533) / from_particles (:391). */
 nxm_status nxm_scene_add_particles(nxm_scene *scene, const nxm_particle *particles, size_t n);
 nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa *p, size_t n);
-/* MpmState::set_coupling (src_mpm/pipeline.rs:220) restricted to one-way cuboids,
- * non-CPIC (use_cpic = false). */
+/* MpmState::set_coupling (src_mpm/pipeline.rs:220) restricted to one-way cuboids (at
+ * most 16, the reference's GpuMaterials limit). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
+/* MpmState::use_cpic (src_mpm/pipeline.rs:69, default off here): compatible
+ * particle-in-cell transfers (pipeline.rs:433-460 with the use_cpic branches of p2g.rs,
+ * grid_update.rs, g2p.rs and particle_update.rs) instead of the grid collider pass.
+ * The contact distance field comes from the cuboids (grid_update_cdf.rs); mesh
+ * colliders and their rigid particles (p2g_cdf.rs) are not part of this boundary.
+ * Switching it resets every particle's contact field to zero (as for new particles);
+ * synchronizes when the scene is resident. */
+nxm_status nxm_scene_set_cpic(nxm_scene *scene, int enabled);
+/* Per-particle contact field of the last step (Kinematics::cdf, solver/particle.rs:
+ * 35-50 without rigid_vel, which lives only inside the step): affinity word (bit i:
+ * collider i in reach, bit 16 + i: its side), normal (n*3) and signed distance (n), in
+ * original particle order; any pointer may be NULL.  Synchronizes. */
+nxm_status nxm_scene_read_cdf(nxm_scene *scene, uint32_t *affinity, float *normal, float *signed_distance,
+                              size_t n);
 size_t nxm_scene_len(const nxm_scene *scene);
 /* GpuParticles::swap_remove (src_mpm/solver/particle.rs:575-631): removes the particles
  * with the given ids (duplicates and ids >= len ignored); processing ids in descending
@@ -220,6 +234,12 @@ nxm_status nxm_scene_read_sort_canonical(nxm_scene *scene, const nxm_sort_canoni
  * (x + 4y + 16z) of {mass, momentum.xyz} after P2G and velocity.xyz after the grid
  * update and collider boundary.  mass: nblocks*64, momentum/velocity: nblocks*64*3. */
 nxm_status nxm_scene_read_grid(nxm_scene *scene, float *mass, float *momentum, float *velocity);
+/* CPIC grid of the last step, in read_grid's order: per node the contact distance field
+ * (NodeCdf, grid/grid.rs:367-374: distance, affinity word, closest collider or
+ * 0xFFFFFFFF) and the incompatible mass after P2G and velocity after the grid update
+ * (Node::mass_incompatible / momentum_velocity_incompatible).  Any pointer may be NULL. */
+nxm_status nxm_scene_read_grid_cpic(nxm_scene *scene, float *distance, uint32_t *affinity, uint32_t *closest,
+                                    float *mass_incompatible, float *velocity_incompatible);
 nxm_status nxm_scene_synchronize(nxm_scene *scene);
 /* Keep the per-node momentum after P2G for nxm_scene_read_grid (costs one extra
  * write per node per step).  No counterpart in the reference: a test hook. */

@@ -108,6 +108,8 @@ struct HostParticles {
     // state
     std::vector<float> x, v, F, C, mass, vol0, pstate;
     std::vector<uint32_t> enabled, fixed;
+    std::vector<uint32_t> aff;    // CPIC affinity word
+    std::vector<float> cdf;       // CPIC contact normal and signed distance (4 per particle)
     // static
     std::vector<float> radius, phase, bfric, damping;
     std::vector<uint32_t> group, tag, words, cls;
@@ -118,6 +120,7 @@ struct HostParticles {
         x.resize(3 * m); v.resize(3 * m); F.resize(9 * m); C.resize(9 * m);
         mass.resize(m); vol0.resize(m); pstate.resize(3 * m);
         enabled.resize(m); fixed.resize(m);
+        aff.resize(m); cdf.resize(4 * m);
         radius.resize(m); phase.resize(m); bfric.resize(m); damping.resize(m);
         group.resize(m); tag.resize(m); words.resize(24 * m); cls.resize(m);
     }
@@ -132,6 +135,7 @@ static void move_particle(HostParticles &hp, size_t from, size_t to)
     mv(hp.x, 3); mv(hp.v, 3); mv(hp.F, 9); mv(hp.C, 9); mv(hp.mass, 1); mv(hp.vol0, 1); mv(hp.pstate, 3);
     mv(hp.enabled, 1); mv(hp.fixed, 1); mv(hp.radius, 1); mv(hp.phase, 1); mv(hp.bfric, 1); mv(hp.damping, 1);
     mv(hp.group, 1); mv(hp.tag, 1); mv(hp.words, 24); mv(hp.cls, 1);
+    mv(hp.aff, 1); mv(hp.cdf, 4);
 }
 
 struct SceneRange {
@@ -149,8 +153,8 @@ struct nxm_scene {
     uint32_t grid_cap = 32768;
     std::vector<ColliderDev> cols;
     std::vector<ClassDev> classes;
-    std::map<std::array<uint32_t, 26>, uint32_t> class_index;
-    std::array<uint32_t, 26> last_key{};   // the most recently interned class
+    std::map<std::array<uint32_t, 27>, uint32_t> class_index;
+    std::array<uint32_t, 27> last_key{};   // the most recently interned class
     uint32_t last_class = 0xFFFFFFFFu;
     HostParticles hp;
     bool on_host = true;                // hp state authoritative
@@ -162,6 +166,7 @@ struct nxm_scene {
     uint32_t error = 0;                 // latched device error bits
     bool debug_grid = false;
     bool graph = false;                 // standalone steps replay a captured graph
+    bool cpic = false;                  // MpmState::use_cpic
 };
 
 namespace nxm {
@@ -274,6 +279,8 @@ class World {
         put(PMASS, hp.mass.data(), 1, 0);
         put(PVOL0, hp.vol0.data(), 1, 0);
         for (int c = 0; c < 3; ++c) put(PS0 + c, hp.pstate.data(), 3, c);
+        std::memcpy(f + (size_t)PAFF * count, hp.aff.data() + first, count * 4);
+        for (int c = 0; c < 4; ++c) put(PNX + c, hp.cdf.data(), 4, c);
         uint32_t *uf = reinterpret_cast<uint32_t *>(f + (size_t)PFLAGS * count);
         uint32_t *ui = reinterpret_cast<uint32_t *>(f + (size_t)PID * count);
         for (size_t i = 0; i < count; ++i) {
@@ -356,6 +363,11 @@ class World {
                     uint32_t id = ids[i];
                     if (c < pstate_words(hp.tag[id])) hp.pstate[3 * (size_t)id + c] = buf[i];
                 }
+            }
+            if (sc->cpic) {
+                HIP_TRY(hipMemcpy(buf.data(), a.plane(PAFF) + r.start, r.count * 4, hipMemcpyDeviceToHost));
+                for (uint32_t i = 0; i < r.count; ++i) std::memcpy(&hp.aff[ids[i]], &buf[i], 4);
+                for (int c = 0; c < 4; ++c) HIP_TRY(get(PNX + c, hp.cdf.data(), 4, c));
             }
         }
         return NXM_OK;
@@ -624,6 +636,29 @@ class World {
         return NXM_OK;
     }
 
+    nxm_status grid_cpic(nxm_scene *sc, float *dist, uint32_t *aff, uint32_t *closest, float *mass_i, float *vel_i)
+    {
+        if (!sc->cpic) return fail(NXM_ERR_INVALID_ARGUMENT, "the scene does not use CPIC");
+        size_t nb = 0;
+        nxm_status st = sort_size(sc, &nb);
+        if (st != NXM_OK) return st;
+        const uint32_t b0 = last_b_start_;
+        std::vector<uint4> nc(nb * 64);
+        std::vector<float4> gv(nb * 64);
+        if (nb) {
+            HIP_TRY(hipMemcpy(nc.data(), ncdf_.p + (size_t)b0 * 64, nb * 64 * sizeof(uint4), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(gv.data(), gridv_i_.p + (size_t)b0 * 64, nb * 64 * sizeof(float4), hipMemcpyDeviceToHost));
+        }
+        for (size_t i = 0; i < nb * 64; ++i) {
+            if (dist) std::memcpy(&dist[i], &nc[i].x, 4);
+            if (aff) aff[i] = nc[i].y;
+            if (closest) closest[i] = nc[i].z;
+            if (mass_i) mass_i[i] = gv[i].w;
+            if (vel_i) { vel_i[3 * i] = gv[i].x; vel_i[3 * i + 1] = gv[i].y; vel_i[3 * i + 2] = gv[i].z; }
+        }
+        return NXM_OK;
+    }
+
   private:
     // Pinned host staging, reused once the copies that read it have executed.
     struct PinnedChunk {
@@ -699,6 +734,7 @@ class World {
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
         w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
+        w.ncdf = ncdf_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
     }
@@ -740,6 +776,7 @@ class World {
         std::vector<ColliderDev> cols;
         std::vector<uint32_t> region_start, region_scene;
         uint32_t b = 0, hsum = 0;
+        bool any_cpic = false;
         for (uint32_t i = 0; i < slots_.size(); ++i) {
             SceneDev &L = layout_[i];
             if (slots_[i].state != SLOT_ACTIVE) continue;
@@ -763,6 +800,8 @@ class World {
             cols.insert(cols.end(), sc->cols.begin(), sc->cols.end());
             L.class_base = (uint32_t)cls.size();
             cls.insert(cls.end(), sc->classes.begin(), sc->classes.end());
+            L.cpic = sc->cpic ? 1u : 0u;
+            any_cpic |= sc->cpic;
         }
         // Dense block ids are ranks of occupied hash slots: sizing the block tables by
         // the slot count keeps every rank (and each scene's overflow block) in range
@@ -776,7 +815,8 @@ class World {
                           std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(cols.size(), 1) > cols_.n ||
                           hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
                           std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
-                          (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n);
+                          (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n) ||
+                          (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > ncdf_.n);
         if (grow) {
             nxm_status st = sync();
             if (st != NXM_OK) return st;
@@ -811,6 +851,13 @@ class World {
         scan_stride_ = scan_stride;
         HIP_TRY(scan_state_.reserve(2 * (size_t)scan_stride_));
         if (debug_) HIP_TRY(dbg_mom_.reserve((size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(b_total_, 1)));
+        if (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(b_total_, 1) > ncdf_.n) {
+            const size_t nb = std::max<size_t>(b_total_, 1) + b_total_ / 4;
+            HIP_TRY(ncdf_.reserve((size_t)NODES_PER_BLOCK * nb));
+            HIP_TRY(stage_i_.reserve((size_t)4 * TILE_NODES * nb));
+            HIP_TRY(gridv_i_.reserve((size_t)NODES_PER_BLOCK * nb));
+        }
+        any_cpic_ = any_cpic;
         HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hvals_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hmark_.reserve(std::max<uint32_t>(h_total_, 1)));
@@ -1043,6 +1090,10 @@ class World {
     DevBuf<int4> bvid_;
     DevBuf<float> stage_, dbg_mom_;
     DevBuf<float4> gridv_;
+    DevBuf<uint4> ncdf_;               // CPIC buffers, allocated once a CPIC scene is present
+    DevBuf<float> stage_i_;
+    DevBuf<float4> gridv_i_;
+    bool any_cpic_ = false;
     DevBuf<SceneDev> scenes_;
     DevBuf<SceneCtr> ctr_;
     DevBuf<ClassDev> classes_;
@@ -1063,13 +1114,15 @@ struct nxm_batch {
 
 // Interns the material class of one particle: the model tag, its parameter words
 // (without the per-particle plastic state) and the damping.
-static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *words, float damping, uint32_t *out)
+static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *words, float damping, float bfric,
+                               uint32_t *out)
 {
-    std::array<uint32_t, 26> key{};
+    std::array<uint32_t, 27> key{};
     key[0] = tag;
     const int pw = pstate_words(tag);
     for (int i = 0; i < 24; ++i) key[1 + i] = i < pw ? 0u : words[i];
     std::memcpy(&key[25], &damping, 4);
+    std::memcpy(&key[26], &bfric, 4);
     // Consecutive particles usually share their material: try the last class first.
     if (sc->last_class != NONE && key == sc->last_key) {
         *out = sc->last_class;
@@ -1086,6 +1139,7 @@ static nxm_status intern_class(nxm_scene *sc, uint32_t tag, const uint32_t *word
     ClassDev c{};
     c.tag = tag;
     c.damping = damping;
+    c.bfric = bfric;
     std::memcpy(c.w, words, sizeof(c.w));
     uint32_t id = (uint32_t)sc->classes.size();
     sc->classes.push_back(c);
@@ -1171,7 +1225,9 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
         std::memcpy(&hp.words[24 * i], words, 96);
         hp.pstate[3 * i] = hp.pstate[3 * i + 1] = hp.pstate[3 * i + 2] = 0.0f;
         std::memcpy(&hp.pstate[3 * i], words, 4 * (size_t)pstate_words(tag));
-        nxm_status st = intern_class(sc, tag, words, damp, &hp.cls[i]);
+        hp.aff[i] = 0u;
+        hp.cdf[4 * i] = hp.cdf[4 * i + 1] = hp.cdf[4 * i + 2] = hp.cdf[4 * i + 3] = 0.0f;
+        nxm_status st = intern_class(sc, tag, words, damp, bfr, &hp.cls[i]);
         if (st != NXM_OK) return st;
     }
     if (sc->world) return sc->world->upload(sc, base, n);
@@ -1441,6 +1497,49 @@ nxm_status nxm_scene_read_grid(nxm_scene *sc, float *mass, float *momentum, floa
     if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
     if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
     return sc->world->grid(sc, mass, momentum, velocity);
+}
+
+nxm_status nxm_scene_read_grid_cpic(nxm_scene *sc, float *distance, uint32_t *affinity, uint32_t *closest,
+                                    float *mass_incompatible, float *velocity_incompatible)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    return sc->world->grid_cpic(sc, distance, affinity, closest, mass_incompatible, velocity_incompatible);
+}
+
+nxm_status nxm_scene_set_cpic(nxm_scene *sc, int enabled)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    const bool on = enabled != 0;
+    if (on == sc->cpic) return NXM_OK;
+    World *w = sc->world;
+    if (w) {
+        // The world's tables and buffers change: bring the state to the host and re-enter.
+        nxm_status st = w->remove(sc);
+        if (st != NXM_OK) return st;
+    }
+    sc->cpic = on;
+    std::fill(sc->hp.aff.begin(), sc->hp.aff.end(), 0u);
+    std::fill(sc->hp.cdf.begin(), sc->hp.cdf.end(), 0.0f);
+    if (w) return w->add(sc);
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_read_cdf(nxm_scene *sc, uint32_t *affinity, float *normal, float *signed_distance, size_t n)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (n < sc->hp.n) return fail(NXM_ERR_INVALID_ARGUMENT, "output too small");
+    if (sc->world) {
+        nxm_status st = sc->world->pull(sc);
+        if (st != NXM_OK) return st;
+    }
+    const HostParticles &hp = sc->hp;
+    for (size_t i = 0; i < hp.n; ++i) {
+        if (affinity) affinity[i] = hp.aff[i];
+        if (normal) { normal[3 * i] = hp.cdf[4 * i]; normal[3 * i + 1] = hp.cdf[4 * i + 1]; normal[3 * i + 2] = hp.cdf[4 * i + 2]; }
+        if (signed_distance) signed_distance[i] = hp.cdf[4 * i + 3];
+    }
+    return latched(sc, NXM_OK);
 }
 
 nxm_status nxm_scene_set_debug_grid(nxm_scene *sc, int enabled)

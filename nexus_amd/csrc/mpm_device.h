@@ -36,7 +36,8 @@ enum Plane : int {
     PC0, PC1, PC2, PC3, PC4, PC5, PC6, PC7, PC8,   // affine, column-major
     PF0, PF1, PF2, PF3, PF4, PF5, PF6, PF7, PF8,   // deformation gradient, column-major
     PMASS, PVOL0,
-    PS0, PS1, PS2,                                  // Drucker-Prager plastic state
+    PS0, PS1, PS2,                                  // plastic state (sand: 3 words, snow: 1)
+    PAFF, PNX, PNY, PNZ, PSD,                       // CPIC: affinity word, contact normal, signed distance
     PFLAGS, PID,
     NUM_PLANES
 };
@@ -57,6 +58,7 @@ struct SceneDev {
     uint32_t h_base, h_cap;       // hash region [h_base, h_base + h_cap) (h_cap a power of two), then one overflow slot
     uint32_t col_start, col_count;
     uint32_t class_base;          // first entry of the scene's material classes
+    uint32_t cpic;                // compatible particle-in-cell transfers (MpmState::use_cpic)
 };
 
 struct SceneCtr {
@@ -78,12 +80,14 @@ struct Globals {
 };
 
 // Material class: model tag and words (src_mpm_shaders/models/default.rs:41-44),
-// plus the particle damping (ParticleProperties::damping, solver/particle.rs:166).
+// plus the particle damping (ParticleProperties::damping, solver/particle.rs:166) and
+// the boundary friction factor (Kinematics::boundary_friction, solver/particle.rs:143).
 struct ClassDev {
     uint32_t tag;
     float damping;
     float w[24];
-    uint32_t pad[2];
+    float bfric;
+    uint32_t pad;
 };
 
 // nxm_cuboid layout.
@@ -526,6 +530,83 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
         }
     }
     return vel;
+}
+
+// ------------------------------------------------------------------------------------
+// CPIC contact distance field (solver/grid_update_cdf.rs, g2p_cdf.rs; affinity words as
+// grid/grid.rs:593-645: bit i = collider i within reach, bit 16 + i = its side).
+
+__device__ inline bool aff_compatible(uint32_t a, uint32_t b)
+{
+    const uint32_t common = a & b & 0xFFFFu;
+    return (((a ^ b) >> 16) & common) == 0u;
+}
+
+// gpu_grid_update_cdf's collide (grid_update_cdf.rs:18-60) for one node: among the
+// colliders containing it or within 1.5 h on every axis, the strictly closest (its
+// distance and id) and every such collider's affinity bit with its side.
+__device__ inline void node_cdf(const ColliderDev *cols, uint32_t ncol, uint32_t mask, float h, f3 pt,
+                                float &dist, uint32_t &aff, uint32_t &closest)
+{
+    const float cap = h * 1.5f;
+    dist = MAX_FLT_REF;
+    aff = 0u;
+    closest = NONE;
+    for (uint32_t i = 0; i < ncol; ++i) {
+        if (!(mask & (1u << i))) continue;
+        bool inside;
+        const f3 proj = project_cuboid(cols[i], pt, inside);
+        const f3 dpt = proj - pt;
+        const bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;
+        if (inside || within) {
+            const float d = length(dpt);
+            if (d < dist) { dist = d; closest = i; }
+            aff |= (inside ? 0x00010001u : 0x00000001u) << i;
+        }
+    }
+}
+
+// Velocity::velocity_at_point (src_rbd_shaders/dynamics/body.rs:239-250), the cuboid's
+// pose translation being its centre of mass.
+__device__ inline f3 body_point_velocity(const ColliderDev &c, f3 pt)
+{
+    const f3 com = mk3(c.t[0], c.t[1], c.t[2]);
+    return mk3(c.linvel[0], c.linvel[1], c.linvel[2]) + cross(mk3(c.angvel[0], c.angvel[1], c.angvel[2]), pt - com);
+}
+
+// Symmetric 4x4 system of the particle's moving-least-squares distance fit: the
+// determinant by the 2x2 minors of rows {0,1} and {2,3} (Laplace expansion) and, when it
+// exceeds the reference's 1e-8 (g2p_cdf.rs:309), the solution adj(Q) u / det.
+struct Sym4 {
+    float q00, q01, q02, q03, q11, q12, q13, q22, q23, q33;
+};
+
+__device__ inline bool sym4_solve(const Sym4 &m, const float u[4], float out[4])
+{
+    const float a00 = m.q00, a01 = m.q01, a02 = m.q02, a03 = m.q03;
+    const float a10 = m.q01, a11 = m.q11, a12 = m.q12, a13 = m.q13;
+    const float a20 = m.q02, a21 = m.q12, a22 = m.q22, a23 = m.q23;
+    const float a30 = m.q03, a31 = m.q13, a32 = m.q23, a33 = m.q33;
+    const float s0 = a00 * a11 - a10 * a01, s1 = a00 * a12 - a10 * a02, s2 = a00 * a13 - a10 * a03;
+    const float s3 = a01 * a12 - a11 * a02, s4 = a01 * a13 - a11 * a03, s5 = a02 * a13 - a12 * a03;
+    const float c5 = a22 * a33 - a32 * a23, c4 = a21 * a33 - a31 * a23, c3 = a21 * a32 - a31 * a22;
+    const float c2 = a20 * a33 - a30 * a23, c1 = a20 * a32 - a30 * a22, c0 = a20 * a31 - a30 * a21;
+    const float det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    if (!(det > 1.0e-8f)) return false;
+    const float b00 = a11 * c5 - a12 * c4 + a13 * c3, b01 = -a01 * c5 + a02 * c4 - a03 * c3;
+    const float b02 = a31 * s5 - a32 * s4 + a33 * s3, b03 = -a21 * s5 + a22 * s4 - a23 * s3;
+    const float b10 = -a10 * c5 + a12 * c2 - a13 * c1, b11 = a00 * c5 - a02 * c2 + a03 * c1;
+    const float b12 = -a30 * s5 + a32 * s2 - a33 * s1, b13 = a20 * s5 - a22 * s2 + a23 * s1;
+    const float b20 = a10 * c4 - a11 * c2 + a13 * c0, b21 = -a00 * c4 + a01 * c2 - a03 * c0;
+    const float b22 = a30 * s4 - a31 * s2 + a33 * s0, b23 = -a20 * s4 + a21 * s2 - a23 * s0;
+    const float b30 = -a10 * c3 + a11 * c1 - a12 * c0, b31 = a00 * c3 - a01 * c1 + a02 * c0;
+    const float b32 = -a30 * s3 + a31 * s1 - a32 * s0, b33 = a20 * s3 - a21 * s1 + a22 * s0;
+    const float r = 1.0f / det;
+    out[0] = (b00 * u[0] + b01 * u[1] + b02 * u[2] + b03 * u[3]) * r;
+    out[1] = (b10 * u[0] + b11 * u[1] + b12 * u[2] + b13 * u[3]) * r;
+    out[2] = (b20 * u[0] + b21 * u[1] + b22 * u[2] + b23 * u[3]) * r;
+    out[3] = (b30 * u[0] + b31 * u[1] + b32 * u[2] + b33 * u[3]) * r;
+    return true;
 }
 
 }  // namespace nxm

@@ -39,6 +39,13 @@ struct WorldDev {
     unsigned long long *scan_state;   // single-pass scan tile status, 2 x scan_stride words
     uint32_t scan_stride;
     float *dbg_mom;                // optional per-node momentum after P2G (3 floats)
+    // CPIC (only when some scene uses it): per node the contact distance field
+    // {distance bits, affinity word, closest collider, 0}; the incompatible P2G tiles and
+    // velocities.
+    uint4 *ncdf;
+    float *stage_i;
+    float4 *gridv_i;
+    uint32_t any_cpic;
     uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
 };
 

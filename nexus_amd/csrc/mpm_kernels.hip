@@ -562,12 +562,16 @@ __device__ inline float wave_max(float v)
 // then add to different nodes at the same instruction instead of serializing on one LDS
 // address; the integer sums do not depend on the order.  CHECK adds the non-finite mask
 // (only for suspect particles, so the common path carries none of the tests).
-constexpr int BAD_ANY = (4 * TILE_NODES + 31) / 32;   // words of the non-finite mask
+// Words of the non-finite mask (one bit per tile word); the word after them flags any.
+// With CPIC the tile holds a second (incompatible) momentum and mass after the first.
+template <bool CPIC> constexpr int tile_fields() { return CPIC ? 8 : 4; }
+template <bool CPIC> constexpr int bad_any() { return (tile_fields<CPIC>() * TILE_NODES + 31) / 32; }
 
-template <bool CHECK>
+template <bool CHECK, bool CPIC>
 __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int rot,
                                             const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wxy)[3][3],
-                                            float wz0, float wz1, float wz2, float smom, float msm)
+                                            float wz0, float wz1, float wz2, float smom, float msm,
+                                            const uint32_t *naff, uint32_t pa)
 {
     #pragma unroll
     for (int kz = 0; kz < 3; ++kz) {
@@ -588,13 +592,16 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
                 const float wgt = wxy[sy][sx] * wzs;
                 const f3 vc = a * (wgt * smom);
                 const float mw = msm * wgt;
-                const int t = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
+                const int tn = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
+                // CPIC (p2g.rs:268-302): a particle not compatible with the node adds to
+                // the node's incompatible fields.
+                const int t = CPIC && !aff_compatible(pa, naff[tn]) ? tn + 4 * TILE_NODES : tn;
                 atomicAdd(&tile[t], Fixed::to(vc.x, 1.0f));
                 atomicAdd(&tile[TILE_NODES + t], Fixed::to(vc.y, 1.0f));
                 atomicAdd(&tile[2 * TILE_NODES + t], Fixed::to(vc.z, 1.0f));
                 atomicAdd(&tile[3 * TILE_NODES + t], Fixed::to(mw, 1.0f));
                 if (CHECK && (!isfinite(vc.x) || !isfinite(vc.y) || !isfinite(vc.z) || !isfinite(mw))) {
-                    atomicOr(&bad[BAD_ANY], 1u);
+                    atomicOr(&bad[bad_any<CPIC>()], 1u);
                     atomicOr(&bad[t >> 5], 1u << (t & 31));
                     atomicOr(&bad[(TILE_NODES + t) >> 5], 1u << ((TILE_NODES + t) & 31));
                     atomicOr(&bad[(2 * TILE_NODES + t) >> 5], 1u << ((2 * TILE_NODES + t) & 31));
@@ -609,30 +616,35 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
 // it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
 // block's node order, so the consumer reads each region as one run of float4.  Ends with
 // the tile free for the next block.
+template <bool CPIC>
 __device__ __forceinline__ void p2g_flush(const WorldDev &w, const unsigned long long *tile, const uint32_t *bad,
                                           uint32_t d, float smom, float sm, uint32_t l)
 {
     __syncthreads();
-    float4 *st = reinterpret_cast<float4 *>(w.stage) + (size_t)d * TILE_NODES;
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
-    const bool any_bad = bad[BAD_ANY] != 0u;
-    for (int t = l; t < TILE_NODES; t += WAVE) {
-        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
-        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-        const int e = 4 * ex + 2 * ey + ez;
-        const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
-        const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
-        float v4[4];
-        #pragma unroll
-        for (int c = 0; c < 4; ++c) v4[c] = Fixed::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
-        if (any_bad) {
+    const bool any_bad = bad[bad_any<CPIC>()] != 0u;
+    #pragma unroll
+    for (int f = 0; f < (CPIC ? 2 : 1); ++f) {
+        float4 *st = reinterpret_cast<float4 *>(f ? w.stage_i : w.stage) + (size_t)d * TILE_NODES;
+        for (int t = l; t < TILE_NODES; t += WAVE) {
+            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+            const int e = 4 * ex + 2 * ey + ez;
+            const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+            const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+            float v4[4];
             #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int k = c * TILE_NODES + t;
-                if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
+            for (int c = 0; c < 4; ++c)
+                v4[c] = Fixed::back(tile[(4 * f + c) * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
+            if (any_bad) {
+                #pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int k = (4 * f + c) * TILE_NODES + t;
+                    if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
+                }
             }
+            st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
         }
-        st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
     }
     __syncthreads();
 }
@@ -668,8 +680,10 @@ __device__ inline void p2g_bound(const P2GIn &p, float h, float &bmom, float &bm
 }
 
 // Scatter of one enabled particle of block vid into the tile.
+template <bool CPIC>
 __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
-                                        const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t l)
+                                        const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t l,
+                                        const uint32_t *naff, uint32_t pa)
 {
     const float h = sc.h;
     const float px = p.px, py = p.py, pz = p.pz, m = p.m;
@@ -721,16 +735,33 @@ __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad,
     const float msm = m * sm;
     const int rot = (int)(l % 3u);
     if (inside) {
-        if (suspect) p2g_scatter<true>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm);
-        else p2g_scatter<false>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm);
+        if (suspect)
+            p2g_scatter<true, CPIC>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
+        else
+            p2g_scatter<false, CPIC>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
     }
 }
 
+// Node affinity words of block d's 6^3 tile (CPIC scenes; zero, compatible with every
+// particle, elsewhere and for missing blocks).
+__device__ inline void load_tile_affinity(const WorldDev &w, uint32_t d, bool cpic, uint32_t *naff, uint32_t l)
+{
+    for (int t = l; t < TILE_NODES; t += WAVE) {
+        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+        const int e = 4 * (tx >> 2) + 2 * (ty >> 2) + (tz >> 2);
+        const uint32_t blk = cpic ? w.bup[8 * (size_t)d + e] : NONE;
+        naff[t] = blk == NONE ? 0u : w.ncdf[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)].y;
+    }
+}
+
+template <bool CPIC>
 __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 {
     using FX = Fixed;
-    __shared__ typename FX::word tile[4 * TILE_NODES];
-    __shared__ uint32_t bad[BAD_ANY + 1];   // non-finite contributions; bad[BAD_ANY]: any at all
+    constexpr int NF = tile_fields<CPIC>();
+    __shared__ typename FX::word tile[NF * TILE_NODES];
+    __shared__ uint32_t bad[bad_any<CPIC>() + 1];   // non-finite contributions; last word: any at all
+    __shared__ uint32_t naff[CPIC ? TILE_NODES : 1];
     const uint32_t l = threadIdx.x;
     const uint32_t *FL = w.in.uplane(PFLAGS);
     // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
@@ -750,23 +781,26 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             // One particle per lane: load it once, bound, then scatter from registers.
             P2GIn p;
             bool on = false;
+            uint32_t pa = 0u;
             if (l < np) {
                 const uint32_t slot = w.perm[first + l];
                 on = (FL[slot] & FLAG_ENABLED) != 0u;
                 if (on) {
                     p = p2g_load(w, slot);
                     p2g_bound(p, h, bmom, bm);
+                    if (CPIC && sc.cpic) pa = w.out.uplane(PAFF)[first + l];
                 }
             }
             bmom = wave_max(bmom);
             bm = wave_max(bm);
             const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
             const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
-            for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
-            if (l <= BAD_ANY) bad[l] = 0u;
+            for (int k = l; k < NF * TILE_NODES; k += WAVE) tile[k] = 0;
+            if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
+            if (CPIC) load_tile_affinity(w, d, sc.cpic != 0u, naff, l);
             __syncthreads();
-            if (on) p2g_one(tile, bad, p, v, sc, smom, sm, bmom, bm, l);
-            p2g_flush(w, tile, bad, d, smom, sm, l);
+            if (on) p2g_one<CPIC>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+            p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l);
             continue;
         }
         // Pass 1: bounds over the block; pass 2: scatter, 64 particles at a time.
@@ -781,23 +815,29 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         bm = wave_max(bm);
         const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
         const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
-        for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
-        if (l <= BAD_ANY) bad[l] = 0u;
+        for (int k = l; k < NF * TILE_NODES; k += WAVE) tile[k] = 0;
+        if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
+        if (CPIC) load_tile_affinity(w, d, sc.cpic != 0u, naff, l);
         __syncthreads();
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
-                if (FL[slot] & FLAG_ENABLED) p2g_one(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l);
+                if (FL[slot] & FLAG_ENABLED) {
+                    const uint32_t pa = CPIC && sc.cpic ? w.out.uplane(PAFF)[pos] : 0u;
+                    p2g_one<CPIC>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
+                }
             }
         }
-        p2g_flush(w, tile, bad, d, smom, sm, l);
+        p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l);
     }
 }
 
 // ------------------------------------------------------------------------------------
 // Grid update: gpu_grid_update + gpu_grid_update_collide (grid_update.rs:21-88,
-// grid_update_collide.rs:105-183) on the summed tiles.
+// grid_update_collide.rs:105-183) on the summed tiles.  CPIC scenes update both velocity
+// fields and skip the collider pass (solver/grid_update.rs:26-50).
+template <bool CPIC>
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
     uint32_t nbegin, nend;
@@ -809,7 +849,10 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
         if (v.x == OVERFLOW_VID) continue;
         const uint32_t li = n & 63u;
         const int i = li & 3, j = (li >> 2) & 3, k = li >> 4;
+        const SceneDev sc = w.scenes[s];
+        const bool cp = CPIC && sc.cpic;
         float mx = 0.0f, my = 0.0f, mz = 0.0f, m = 0.0f;
+        float ix = 0.0f, iy = 0.0f, iz = 0.0f, im = 0.0f;
         #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int ex = ASSOC[e][0], ey = ASSOC[e][1], ez = ASSOC[e][2];
@@ -817,18 +860,25 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             const uint32_t src = w.bdn[8 * d + e];
             if (src == NONE) continue;
             const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
-            const float4 q = reinterpret_cast<const float4 *>(w.stage)[(size_t)src * TILE_NODES + REGION_OFFSET[e] + i + nx * (j + ny * k)];
+            const size_t at = (size_t)src * TILE_NODES + REGION_OFFSET[e] + i + nx * (j + ny * k);
+            const float4 q = reinterpret_cast<const float4 *>(w.stage)[at];
             mx += q.x;
             my += q.y;
             mz += q.z;
             m += q.w;
+            if (cp) {
+                const float4 qi = reinterpret_cast<const float4 *>(w.stage_i)[at];
+                ix += qi.x;
+                iy += qi.y;
+                iz += qi.z;
+                im += qi.w;
+            }
         }
         if (w.dbg_mom) {
             w.dbg_mom[3 * (size_t)n] = mx;
             w.dbg_mom[3 * (size_t)n + 1] = my;
             w.dbg_mom[3 * (size_t)n + 2] = mz;
         }
-        const SceneDev sc = w.scenes[s];
         const float inv_mass = m > 0.0f ? 1.0f / m : 0.0f;
         const float mdt = m * sc.dt;
         f3 vel = mk3((mx + sc.gx * mdt) * inv_mass, (my + sc.gy * mdt) * inv_mass, (mz + sc.gz * mdt) * inv_mass);
@@ -836,7 +886,16 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
         vel.x = fminf(fmaxf(vel.x, -lim), lim);
         vel.y = fminf(fmaxf(vel.y, -lim), lim);
         vel.z = fminf(fmaxf(vel.z, -lim), lim);
-        const uint32_t mask = w.bmask[d];
+        if (cp) {
+            const float inv_mi = im > 0.0f ? 1.0f / im : 0.0f;
+            const float midt = im * sc.dt;
+            f3 vi = mk3((ix + sc.gx * midt) * inv_mi, (iy + sc.gy * midt) * inv_mi, (iz + sc.gz * midt) * inv_mi);
+            vi.x = fminf(fmaxf(vi.x, -lim), lim);
+            vi.y = fminf(fmaxf(vi.y, -lim), lim);
+            vi.z = fminf(fmaxf(vi.z, -lim), lim);
+            w.gridv_i[n] = make_float4(vi.x, vi.y, vi.z, im);
+        }
+        const uint32_t mask = cp ? 0u : w.bmask[d];
         if (mask) {
             const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
             vel = collide_node(w.cols + sc.col_start, sc.col_count, mask, sc.h, sc.dt, pt, vel);
@@ -860,26 +919,35 @@ __device__ __constant__ const int8_t NBH[27][3] = {
 struct GatherGrid {
     const uint32_t *bup;
     const float4 *gridv;
+    const float4 *gridv_i;                     // CPIC: incompatible velocities
+    const uint4 *ncdf;                         // CPIC: node contact distance fields
     uint32_t n0, n1, n2, n3, n4, n5, n6, n7;   // vid + {0,1}^3 in ASSOC order (4 ox + 2 oy + oz)
     __device__ void prepare(uint32_t d)
     {
         const uint32_t *p = bup + 8 * (size_t)d;
         n0 = p[0]; n1 = p[1]; n2 = p[2]; n3 = p[3]; n4 = p[4]; n5 = p[5]; n6 = p[6]; n7 = p[7];
     }
-    __device__ f3 at(int tx, int ty, int tz) const
+    // Grid node of tile position (tx, ty, tz), NONE in a missing block.
+    __device__ uint32_t index(int tx, int ty, int tz) const
     {
         // block of the node by selects on the three offset bits (no indexed register array)
         const bool ox = tx >= 4, oy = ty >= 4, oz = tz >= 4;
         const uint32_t z0 = oz ? n1 : n0, z1 = oz ? n3 : n2, z2 = oz ? n5 : n4, z3 = oz ? n7 : n6;
         const uint32_t y0 = oy ? z1 : z0, y1 = oy ? z3 : z2;
         const uint32_t blk = ox ? y1 : y0;
-        if (blk == NONE) return mk3(0, 0, 0);
-        const float4 g = gridv[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)];
+        if (blk == NONE) return NONE;
+        return blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3);
+    }
+    __device__ f3 at(int tx, int ty, int tz) const
+    {
+        const uint32_t i = index(tx, ty, tz);
+        if (i == NONE) return mk3(0, 0, 0);
+        const float4 g = gridv[i];
         return mk3(g.x, g.y, g.z);
     }
 };
 
-template <class Gather>
+template <class Gather, bool CPIC>
 __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, uint32_t d, Gather &gather)
 {
     {
@@ -906,6 +974,16 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
         float ps0 = 0.0f, ps1 = 0.0f, ps2 = 0.0f;
         if (sand || snow) ps0 = w.in.plane(PS0)[slot];
         if (sand) { ps1 = w.in.plane(PS1)[slot]; ps2 = w.in.plane(PS2)[slot]; }
+        // CPIC: this substep's particle contact field (written at the sorted position).
+        const bool cp = CPIC && sc.cpic;
+        uint32_t pa = 0u;
+        f3 pn = mk3(0, 0, 0), rigid_vel = mk3(0, 0, 0);
+        float psd = 0.0f;
+        if (cp) {
+            pa = w.out.uplane(PAFF)[pos];
+            pn = mk3(w.out.plane(PNX)[pos], w.out.plane(PNY)[pos], w.out.plane(PNZ)[pos]);
+            psd = w.out.plane(PSD)[pos];
+        }
         if (bv.x == OVERFLOW_VID) {
             // No block: keep the particle unchanged (the scene reports an error).
             vel = mk3(w.in.plane(PVX)[slot], w.in.plane(PVY)[slot], w.in.plane(PVZ)[slot]);
@@ -940,7 +1018,34 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
                     const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
-                    const f3 cv = gather.at(bx + sx, by + sy, bz + sz);
+                    f3 cv;
+                    if (cp) {
+                        // g2p.rs:206-233: a node the particle is not compatible with gives
+                        // its incompatible velocity, seen through its closest collider's
+                        // boundary condition (friction scaled by the particle's factor)
+                        // relative to the body's velocity there.
+                        const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
+                        cv = mk3(0, 0, 0);
+                        if (ni != NONE) {
+                            const uint4 nc = gather.ncdf[ni];
+                            if (aff_compatible(pa, nc.y)) {
+                                const float4 g = gather.gridv[ni];
+                                cv = mk3(g.x, g.y, g.z);
+                            } else {
+                                const float4 g = gather.gridv_i[ni];
+                                cv = mk3(g.x, g.y, g.z);
+                                if (nc.z != NONE) {
+                                    const ColliderDev &c = w.cols[sc.col_start + nc.z];
+                                    const f3 center = mk3(ref.x + (float)sx * h + px, ref.y + (float)sy * h + py,
+                                                          ref.z + (float)sz * h + pz);
+                                    const f3 bpv = body_point_velocity(c, center);
+                                    cv = bpv + bc_project(c.bc_type, c.bc_friction * cls.bfric, cv - bpv, pn);
+                                }
+                            }
+                        }
+                    } else {
+                        cv = gather.at(bx + sx, by + sy, bz + sz);
+                    }
                     const float wgt = wx[sx] * wy[sy] * wz[sz];
                     const f3 cw = cv * wgt;
                     vel = vel + cw;
@@ -952,13 +1057,26 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 gradv.c1 = (vel * ref.y + gy * h) * inv_d;
                 gradv.c2 = (vel * ref.z + gz * h) * inv_d;
                 div = gradv.c0.x + gradv.c1.y + gradv.c2.z;
+                if (cp) {
+                    // g2p.rs:248-258: the bodies' velocities at the particle over its affinities
+                    for (uint32_t c = 0; c < sc.col_count; ++c)
+                        if ((pa >> c) & 1u) rigid_vel = rigid_vel + body_point_velocity(w.cols[sc.col_start + c], mk3(px, py, pz));
+                }
             }
-            // Particle update.
+            // Particle update.  CPIC: a particle deeper than 0.05 h behind a surface is
+            // reprojected (slip relative to the bodies) and pushed out by a penalty
+            // impulse after the position update (particle_update.rs:92-136).
+            const bool pen = cp && psd < -0.05f * h;
+            if (pen) vel = rigid_vel + bc_project(1u, 0.0f, vel - rigid_vel, pn);
             const float vl = length(vel);
             if (vl > sc.vlim) vel = vel / vl * h / dt;
             vel = vel * rcp_hw(1.0f + cls.damping * dt);
             if (fl & FLAG_FIXED) vel = mk3(0, 0, 0);
             npos = mk3(px, py, pz) + vel * dt;
+            if (pen) {
+                const float corrected = fmaxf(psd, -0.3f * h);
+                vel = vel + pn * (dt * -corrected * 1.0e3f);
+            }
             m3 tau;
             if (tag != 4u) {
                 F = F + mul(gradv * dt, F);
@@ -1039,13 +1157,145 @@ __device__ inline void clear_hash(const WorldDev &w)
 }
 
 // One thread per sorted particle, nodes read from the grid.
+template <bool CPIC>
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
-    GatherGrid grid{w.bup, w.gridv, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
-    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) g2p_particle(w, pos, w.sblock[pos], grid);
+    GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x)
+        g2p_particle<GatherGrid, CPIC>(w, pos, w.sblock[pos], grid);
     clear_hash(w);
+}
+
+// ------------------------------------------------------------------------------------
+// CPIC contact distance field.
+
+// Per node of a CPIC scene's blocks: gpu_grid_update_cdf (grid_update_cdf.rs:60-132).
+// Mesh colliders (p2g_cdf) are not part of this boundary: the field comes from the
+// analytic cuboids alone.
+__global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
+{
+    uint32_t nbegin, nend;
+    xcd_range(w.g->n_blocks * NODES_PER_BLOCK, nbegin, nend);
+    for (uint32_t n = nbegin + threadIdx.x; n < nend; n += blockDim.x) {
+        const uint32_t d = n >> 6;
+        const SceneDev sc = w.scenes[w.bscene[d]];
+        if (!sc.cpic) continue;
+        const int4 v = w.bvid[d];
+        float dist = MAX_FLT_REF;
+        uint32_t aff = 0u, closest = NONE;
+        if (v.x != OVERFLOW_VID) {
+            const uint32_t li = n & 63u;
+            const int i = li & 3, j = (li >> 2) & 3, k = li >> 4;
+            const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
+            node_cdf(w.cols + sc.col_start, sc.col_count, w.bmask[d], sc.h, pt, dist, aff, closest);
+        }
+        w.ncdf[n] = make_uint4(__float_as_uint(dist), aff, closest, 0u);
+    }
+}
+
+// Per particle of a CPIC scene: gpu_g2p_cdf (g2p_cdf.rs:150-331).  The affinity word is
+// the union of the stencil nodes' affinities; a collider reached for the first time gets
+// its side from the sign of the weighted signed node distances, a known one keeps its
+// side.  Then the moving-least-squares fit d(x) = n . (x - x_p) + d_p over the nodes that
+// share an affinity with the particle (distance negated where the node's side differs)
+// gives the normal and the signed distance; when its matrix is singular (det <= 1e-8) the
+// field is zero, affinity included.  Results go to the output arena at the sorted
+// position, where P2G and G2P read them and where they stay as the next state.
+__global__ void __launch_bounds__(256) k_particle_cdf(WorldDev w)
+{
+    uint32_t pbegin, pend;
+    xcd_range(w.g->p_out, pbegin, pend);
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
+        const uint32_t d = w.sblock[pos];
+        const SceneDev sc = w.scenes[w.bscene[d]];
+        if (!sc.cpic) continue;
+        const uint32_t slot = w.perm[pos];
+        const uint32_t prev = w.in.uplane(PAFF)[slot];
+        uint32_t aff = prev;
+        f3 nrm = mk3(0, 0, 0);
+        float sd = 0.0f;
+        const int4 bv = w.bvid[d];
+        if (bv.x != OVERFLOW_VID) {
+            const float px = w.in.plane(PX)[slot], py = w.in.plane(PY)[slot], pz = w.in.plane(PZ)[slot];
+            const float h = sc.h, ih = sc.inv_h;
+            const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
+                        cz = round_div(pz, h, ih) - 1.0f;
+            const int bx = (int)cx - 4 * bv.x, by = (int)cy - 4 * bv.y, bz = (int)cz - 4 * bv.z;
+            if ((unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u) {
+                const f3 ref = mk3(cx * h - px, cy * h - py, cz * h - pz);
+                float wx[3], wy[3], wz[3];
+                {
+                    const float fx = -ref.x * ih, fy = -ref.y * ih, fz = -ref.z * ih;
+                    wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
+                    wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
+                    wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
+                }
+                GatherGrid gather{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+                gather.prepare(d);
+                float nd[27], wq[27];
+                uint32_t na[27];
+                uint32_t pa = 0u;
+                #pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
+                    const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
+                    const uint4 nc = ni == NONE ? make_uint4(__float_as_uint(MAX_FLT_REF), 0u, NONE, 0u) : w.ncdf[ni];
+                    nd[q] = __uint_as_float(nc.x);
+                    na[q] = nc.y;
+                    wq[q] = wx[sx] * wy[sy] * wz[sz];
+                    pa |= nc.y & 0xFFFFu;
+                }
+                // Sides: kept for colliders known before, from the weighted signed
+                // distances for newly reached ones.
+                pa |= prev & ((prev & 0xFFFFu) << 16);
+                for (uint32_t c = 0; c < sc.col_count; ++c) {
+                    if ((prev >> c) & 1u) continue;
+                    float acc = 0.0f;
+                    #pragma unroll
+                    for (int q = 0; q < 27; ++q) {
+                        const float comp = (na[q] >> c) & 1u ? 1.0f : 0.0f;
+                        const float sgn = (na[q] >> (16 + c)) & 1u ? -1.0f : 1.0f;
+                        acc += comp * wq[q] * sgn * nd[q];
+                    }
+                    if (acc < 0.0f) pa |= 0x00010000u << c;
+                }
+                Sym4 m{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                #pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const uint32_t comb = na[q] & pa & 0xFFFFu;
+                    if (comb == 0u) continue;
+                    const uint32_t sdiff = ((na[q] >> 16) ^ (pa >> 16)) & comb;
+                    const float sx = (float)NBH[q][0] * h, sy = (float)NBH[q][1] * h, sz = (float)NBH[q][2] * h;
+                    const float dx = ref.x + sx, dy = ref.y + sy, dz = ref.z + sz;
+                    const float wgt = wq[q];
+                    const float dd = sdiff == 0u ? nd[q] : -nd[q];
+                    m.q00 += dx * dx * wgt; m.q01 += dx * dy * wgt; m.q02 += dx * dz * wgt; m.q03 += dx * wgt;
+                    m.q11 += dy * dy * wgt; m.q12 += dy * dz * wgt; m.q13 += dy * wgt;
+                    m.q22 += dz * dz * wgt; m.q23 += dz * wgt;
+                    m.q33 += wgt;
+                    u[0] += dx * (wgt * dd); u[1] += dy * (wgt * dd); u[2] += dz * (wgt * dd); u[3] += wgt * dd;
+                }
+                float res[4];
+                if (sym4_solve(m, u, res)) {
+                    const f3 g = mk3(res[0], res[1], res[2]);
+                    const float len = length(g);
+                    nrm = len > 1.0e-6f ? g / len : mk3(0, 0, 0);
+                    sd = res[3];
+                    aff = pa;
+                } else {
+                    aff = 0u;
+                }
+            }
+        }
+        w.out.uplane(PAFF)[pos] = aff;
+        w.out.plane(PNX)[pos] = nrm.x;
+        w.out.plane(PNY)[pos] = nrm.y;
+        w.out.plane(PNZ)[pos] = nrm.z;
+        w.out.plane(PSD)[pos] = sd;
+    }
 }
 
 
@@ -1190,11 +1440,23 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     k_scan<MODE_COUNTS><<<std::max(bscan, 1), 256, 0, st>>>(w);
     k_scatter<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_p2g<<<lc.p2g_blocks, 64, 0, st>>>(w);
-    if (ev) (void)hipEventRecord(ev[2], st);
-    k_grid_update<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
-    if (ev) (void)hipEventRecord(ev[3], st);
-    k_g2p_update<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    if (w.any_cpic) {
+        // CPIC worlds: the contact distance fields, then the CPIC variants of the
+        // transfers (scenes without CPIC take the plain path inside them).
+        k_grid_cdf<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        k_particle_cdf<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);
+        if (ev) (void)hipEventRecord(ev[2], st);
+        k_grid_update<true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        if (ev) (void)hipEventRecord(ev[3], st);
+        k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    } else {
+        k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
+        if (ev) (void)hipEventRecord(ev[2], st);
+        k_grid_update<false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        if (ev) (void)hipEventRecord(ev[3], st);
+        k_g2p_update<false><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    }
     if (ev) (void)hipEventRecord(ev[4], st);
 }
 

@@ -43,6 +43,7 @@ EXPORTS = [
     "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
+    "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
@@ -135,6 +136,9 @@ def library() -> ct.CDLL:
     L.nxm_scene_read_grid.argtypes = [P, P, P, P]
     L.nxm_scene_synchronize.argtypes = [P]
     L.nxm_scene_set_debug_grid.argtypes = [P, I]
+    L.nxm_scene_set_cpic.argtypes = [P, I]
+    L.nxm_scene_read_cdf.argtypes = [P, P, P, P, ct.c_size_t]
+    L.nxm_scene_read_grid_cpic.argtypes = [P, P, P, P, P, P]
     L.nxm_pipeline_step.argtypes = [P, P, P]
     L.nxm_pipeline_run.argtypes = [P, P, ct.c_uint32]
     L.nxm_batch_create.argtypes = [P, ct.POINTER(P)]
@@ -224,8 +228,34 @@ class MpmState:
         st.set_simulation_params(scene.gravity, scene.dt)
         if scene.colliders.size:
             st.set_colliders(scene.colliders)
+        if getattr(scene, "cpic", False):
+            st.set_cpic(True)
         st.add_particles(scene)
         return st
+
+    def set_cpic(self, enabled: bool) -> None:
+        """MpmState::use_cpic: compatible particle-in-cell transfers against the colliders."""
+        _check(self.lib.nxm_scene_set_cpic(self.handle, int(bool(enabled))))
+
+    def read_cdf(self) -> dict:
+        """Per-particle contact field of the last step: affinity word, normal, signed distance."""
+        n = len(self)
+        out = dict(affinity=np.zeros(n, np.uint32), normal=np.zeros((n, 3), np.float32),
+                   signed_distance=np.zeros(n, np.float32))
+        _check(self.lib.nxm_scene_read_cdf(self.handle, _ptr(out["affinity"]), _ptr(out["normal"]),
+                                           _ptr(out["signed_distance"]), n))
+        return out
+
+    def read_grid_cpic(self) -> dict:
+        """CPIC grid of the last step (read_grid's node order): node contact field and the
+        incompatible mass and velocity."""
+        nb = self.sort_size()
+        out = dict(cdf_dist=np.zeros((nb, 64), np.float32), cdf_aff=np.zeros((nb, 64), np.uint32),
+                   cdf_closest=np.zeros((nb, 64), np.uint32), m_i=np.zeros((nb, 64), np.float32),
+                   vel_i=np.zeros((nb, 64, 3), np.float32))
+        _check(self.lib.nxm_scene_read_grid_cpic(self.handle, _ptr(out["cdf_dist"]), _ptr(out["cdf_aff"]),
+                                                 _ptr(out["cdf_closest"]), _ptr(out["m_i"]), _ptr(out["vel_i"])))
+        return out
 
     def close(self):
         if self.handle:
@@ -303,6 +333,9 @@ class MpmState:
             model_tag=np.ascontiguousarray(scene.tag, np.uint32),
             model_data=np.ascontiguousarray(scene.model, np.uint32),
         )
+        bf = getattr(scene, "boundary_friction", None)
+        if bf is not None:
+            arrs["boundary_friction"] = np.ascontiguousarray(bf, np.float32)
         soa = _SoA(**{k: _ptr(v) for k, v in arrs.items()})
         _check(self.lib.nxm_scene_add_particles_soa(self.handle, ct.byref(soa), scene.n))
 
