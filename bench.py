@@ -8,7 +8,8 @@ Default workload: the Drucker-Prager sand column, 1,048,576 particles, h = 1/256
 are timed after W warm-up substeps (the launch-shape autotuner runs on the warm-up).  Two further measurements go into the same line:
 "batched" (256 mixed scenes stepped as one batch, against the plain per-scene loop) and
 "served" (Poisson arrivals served with continuous batching: p50/p99 substep latency and
-sustained throughput).  Particle state is resident in HBM before the
+sustained throughput), and "cpic" (the sand column at 64x scale with the CPIC
+transfers).  Particle state is resident in HBM before the
 timed region starts.  Prints one JSON line with the whole-job throughput
 (particle-substeps/s), the roofline figure of the dominant kernel (timed live with HIP
 events on the scene's stream during the timed region) and a CPU baseline (the oracle's
@@ -76,6 +77,50 @@ def cpu_baseline(scene, budget_s: float = 15.0, max_steps: int = 30):
         "kind": "port",
         "sample": f"{steps} substeps of the full {scene.n}-particle scene, FP32 oracle (C restatement of the "
                   f"reference algorithm), {threads} OpenMP threads, {el:.1f} s",
+    }
+
+
+def bench_cpic(pipe, steps: int, warmup: int = 40, autotune: bool = True) -> dict:
+    """The sand column at 64x scale (h = 0.25) with the CPIC transfers instead of the
+    grid collider pass: particle-substeps/s and stage times (the contact distance field
+    kernels are counted in the P2G stage)."""
+    from nexus_amd import mpm, scenes
+    sc = scenes.sand_column_cpic()
+    res = {}
+    for cpic in (True, False):
+        run = sc.copy()
+        run.cpic = cpic
+        st = mpm.MpmState.from_scene(pipe, run)
+        if autotune:
+            st.set_autotune(True)
+        for _ in range(warmup):
+            pipe.step(st)
+        st.synchronize()
+        st.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pipe.step(st)
+        st.synchronize()
+        el = time.perf_counter() - t0
+        stages, _ = st.profile_end()
+        aff = st.read_cdf()["affinity"] if cpic else None
+        res[cpic] = (el, stages, st.sort_size(), aff)
+        st.close()
+    el, stages, nb, aff = res[True]
+    return {
+        "workload": "Drucker-Prager sand column, 1,048,576 particles, 64x scale (h=0.25, dt=3.2e-3), 16 seeded "
+                    "cuboid colliders, CPIC",
+        "particle_substeps_per_s": sc.n * steps / el,
+        "ms_per_step": 1e3 * el / steps,
+        "steps": steps,
+        "active_blocks": nb,
+        "particles_with_affinity": int((aff & 0xFFFF != 0).sum()),
+        "stages_ms": {"sort": stages.sort_ms, "cdf_and_p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
+                      "g2p_update": stages.g2p_update_ms},
+        "same_scene_without_cpic": {"particle_substeps_per_s": sc.n * steps / res[False][0],
+                                    "stages_ms": {"sort": res[False][1].sort_ms, "p2g": res[False][1].p2g_ms,
+                                                  "grid_update": res[False][1].grid_update_ms,
+                                                  "g2p_update": res[False][1].g2p_update_ms}},
     }
 
 
@@ -220,6 +265,8 @@ def main(argv=None):
     ap.add_argument("--batch-substeps", type=int, default=500)
     ap.add_argument("--loop-substeps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-cpic", action="store_true", help="skip the CPIC measurement")
+    ap.add_argument("--cpic-steps", type=int, default=300)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -294,6 +341,8 @@ def main(argv=None):
         },
     }
     st.close()
+    if not args.no_cpic:
+        out["cpic"] = bench_cpic(pipe, args.cpic_steps, autotune=not args.no_autotune)
     if not args.no_batch:
         out["batched"] = bench_batched(pipe, args.batch_scenes, args.batch_substeps, args.loop_substeps)
     if not args.no_serve:

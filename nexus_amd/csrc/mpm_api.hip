@@ -368,6 +368,9 @@ class World {
                 HIP_TRY(hipMemcpy(buf.data(), a.plane(PAFF) + r.start, r.count * 4, hipMemcpyDeviceToHost));
                 for (uint32_t i = 0; i < r.count; ++i) std::memcpy(&hp.aff[ids[i]], &buf[i], 4);
                 for (int c = 0; c < 4; ++c) HIP_TRY(get(PNX + c, hp.cdf.data(), 4, c));
+                // An empty affinity means a zero contact field; the device does not store it.
+                for (uint32_t i = 0; i < r.count; ++i)
+                    if (hp.aff[ids[i]] == 0u) std::fill(&hp.cdf[4 * (size_t)ids[i]], &hp.cdf[4 * (size_t)ids[i] + 4], 0.0f);
             }
         }
         return NXM_OK;
@@ -734,7 +737,7 @@ class World {
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
         w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
-        w.ncdf = ncdf_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
+        w.ncdf = ncdf_.p; w.nbody = nbody_.p; w.binc = binc_.p; w.bany = bany_.p; w.clist = clist_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
     }
@@ -764,6 +767,7 @@ class World {
         HIP_TRY(prank_.reserve(cap));
         HIP_TRY(perm_.reserve(cap));
         HIP_TRY(sblock_.reserve(cap));
+        HIP_TRY(clist_.reserve(cap));
         return NXM_OK;
     }
 
@@ -854,6 +858,9 @@ class World {
         if (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(b_total_, 1) > ncdf_.n) {
             const size_t nb = std::max<size_t>(b_total_, 1) + b_total_ / 4;
             HIP_TRY(ncdf_.reserve((size_t)NODES_PER_BLOCK * nb));
+            HIP_TRY(nbody_.reserve((size_t)NODES_PER_BLOCK * nb));
+            HIP_TRY(binc_.reserve(nb));
+            HIP_TRY(bany_.reserve(nb));
             HIP_TRY(stage_i_.reserve((size_t)4 * TILE_NODES * nb));
             HIP_TRY(gridv_i_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
@@ -1091,6 +1098,8 @@ class World {
     DevBuf<float> stage_, dbg_mom_;
     DevBuf<float4> gridv_;
     DevBuf<uint4> ncdf_;               // CPIC buffers, allocated once a CPIC scene is present
+    DevBuf<float4> nbody_;
+    DevBuf<uint32_t> binc_, bany_, clist_;
     DevBuf<float> stage_i_;
     DevBuf<float4> gridv_i_;
     bool any_cpic_ = false;

@@ -76,7 +76,8 @@ struct Globals {
     uint32_t p_out;      // particles placed in the output arena (scan total)
     uint32_t n_blocks;   // dense block ids in use (scan total of occupied hash slots)
     uint32_t ticket[2];  // tile tickets of the two single-pass scans
-    uint32_t pad[3];
+    uint32_t n_contact;  // CPIC: contact particles listed for k_g2p_contact
+    uint32_t pad[2];
 };
 
 // Material class: model tag and words (src_mpm_shaders/models/default.rs:41-44),

@@ -39,10 +39,13 @@ struct WorldDev {
     unsigned long long *scan_state;   // single-pass scan tile status, 2 x scan_stride words
     uint32_t scan_stride;
     float *dbg_mom;                // optional per-node momentum after P2G (3 floats)
-    // CPIC (only when some scene uses it): per node the contact distance field
-    // {distance bits, affinity word, closest collider, 0}; the incompatible P2G tiles and
-    // velocities.
-    uint4 *ncdf;
+    // CPIC (only when some scene uses it): per node the contact distance field; the
+    // incompatible P2G tiles and velocities.
+    uint4 *ncdf;                   // {distance bits, affinity, closest collider, its bc type}
+    float4 *nbody;                 // per node with a closest collider: its velocity there, its friction
+    uint32_t *binc;                // per block: its P2G tile has incompatible contributions
+    uint32_t *bany;                // per block: some node is in reach of a collider
+    uint32_t *clist;               // sorted positions of the contact particles (n_contact of them)
     float *stage_i;
     float4 *gridv_i;
     uint32_t any_cpic;

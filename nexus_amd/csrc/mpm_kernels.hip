@@ -595,7 +595,7 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
                 const int tn = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
                 // CPIC (p2g.rs:268-302): a particle not compatible with the node adds to
                 // the node's incompatible fields.
-                const int t = CPIC && !aff_compatible(pa, naff[tn]) ? tn + 4 * TILE_NODES : tn;
+                const int t = CPIC && (pa & 0xFFFFu) != 0u && !aff_compatible(pa, naff[tn]) ? tn + 4 * TILE_NODES : tn;
                 atomicAdd(&tile[t], Fixed::to(vc.x, 1.0f));
                 atomicAdd(&tile[TILE_NODES + t], Fixed::to(vc.y, 1.0f));
                 atomicAdd(&tile[2 * TILE_NODES + t], Fixed::to(vc.z, 1.0f));
@@ -616,15 +616,20 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
 // it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
 // block's node order, so the consumer reads each region as one run of float4.  Ends with
 // the tile free for the next block.
+// With CPIC, inc (wave-uniform) says whether the block has particles with affinity bits:
+// only then does the incompatible half of the tile hold anything, and only then is it
+// written (the grid update reads w.binc before it).
 template <bool CPIC>
 __device__ __forceinline__ void p2g_flush(const WorldDev &w, const unsigned long long *tile, const uint32_t *bad,
-                                          uint32_t d, float smom, float sm, uint32_t l)
+                                          uint32_t d, float smom, float sm, uint32_t l, bool inc)
 {
     __syncthreads();
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
     const bool any_bad = bad[bad_any<CPIC>()] != 0u;
+    if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
     #pragma unroll
     for (int f = 0; f < (CPIC ? 2 : 1); ++f) {
+        if (f == 1 && !inc) break;
         float4 *st = reinterpret_cast<float4 *>(f ? w.stage_i : w.stage) + (size_t)d * TILE_NODES;
         for (int t = l; t < TILE_NODES; t += WAVE) {
             const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
@@ -795,41 +800,47 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             bm = wave_max(bm);
             const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
             const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
-            for (int k = l; k < NF * TILE_NODES; k += WAVE) tile[k] = 0;
+            const bool inc = CPIC && __ballot((pa & 0xFFFFu) != 0u) != 0ull;
+            for (int k = l; k < (inc ? 8 : 4) * TILE_NODES; k += WAVE) tile[k] = 0;
             if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
-            if (CPIC) load_tile_affinity(w, d, sc.cpic != 0u, naff, l);
+            if (inc) load_tile_affinity(w, d, true, naff, l);
             __syncthreads();
             if (on) p2g_one<CPIC>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
-            p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l);
+            p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l, inc);
             continue;
         }
         // Pass 1: bounds over the block; pass 2: scatter, 64 particles at a time.
+        bool anyc = false;
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
-                if (FL[slot] & FLAG_ENABLED) p2g_bound(p2g_load(w, slot), h, bmom, bm);
+                if (FL[slot] & FLAG_ENABLED) {
+                    p2g_bound(p2g_load(w, slot), h, bmom, bm);
+                    if (CPIC && sc.cpic) anyc |= (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
+                }
             }
         }
         bmom = wave_max(bmom);
         bm = wave_max(bm);
         const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
         const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
-        for (int k = l; k < NF * TILE_NODES; k += WAVE) tile[k] = 0;
+        const bool inc = CPIC && __ballot(anyc) != 0ull;
+        for (int k = l; k < (inc ? 8 : 4) * TILE_NODES; k += WAVE) tile[k] = 0;
         if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
-        if (CPIC) load_tile_affinity(w, d, sc.cpic != 0u, naff, l);
+        if (inc) load_tile_affinity(w, d, true, naff, l);
         __syncthreads();
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
                 if (FL[slot] & FLAG_ENABLED) {
-                    const uint32_t pa = CPIC && sc.cpic ? w.out.uplane(PAFF)[pos] : 0u;
+                    const uint32_t pa = inc ? w.out.uplane(PAFF)[pos] : 0u;
                     p2g_one<CPIC>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
                 }
             }
         }
-        p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l);
+        p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l, inc);
     }
 }
 
@@ -866,7 +877,7 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             my += q.y;
             mz += q.z;
             m += q.w;
-            if (cp) {
+            if (cp && w.binc[src]) {
                 const float4 qi = reinterpret_cast<const float4 *>(w.stage_i)[at];
                 ix += qi.x;
                 iy += qi.y;
@@ -974,16 +985,14 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
         float ps0 = 0.0f, ps1 = 0.0f, ps2 = 0.0f;
         if (sand || snow) ps0 = w.in.plane(PS0)[slot];
         if (sand) { ps1 = w.in.plane(PS1)[slot]; ps2 = w.in.plane(PS2)[slot]; }
-        // CPIC: this substep's particle contact field (written at the sorted position).
-        const bool cp = CPIC && sc.cpic;
+        // CPIC: this substep's particle contact field (written at the sorted position).  A
+        // particle without affinity bits is compatible with every node and its contact
+        // field is zero (a regular fit needs a node sharing an affinity), so it takes the
+        // plain path.
         uint32_t pa = 0u;
-        f3 pn = mk3(0, 0, 0), rigid_vel = mk3(0, 0, 0);
-        float psd = 0.0f;
-        if (cp) {
-            pa = w.out.uplane(PAFF)[pos];
-            pn = mk3(w.out.plane(PNX)[pos], w.out.plane(PNY)[pos], w.out.plane(PNZ)[pos]);
-            psd = w.out.plane(PSD)[pos];
-        }
+        if (CPIC && sc.cpic) pa = w.out.uplane(PAFF)[pos];
+        const bool cp = CPIC && (pa & 0xFFFFu) != 0u;
+        f3 rigid_vel = mk3(0, 0, 0);
         if (bv.x == OVERFLOW_VID) {
             // No block: keep the particle unchanged (the scene reports an error).
             vel = mk3(w.in.plane(PVX)[slot], w.in.plane(PVY)[slot], w.in.plane(PVZ)[slot]);
@@ -1015,43 +1024,52 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 // adds instead of an outer product; the velocity gradient and its trace
                 // (the divergence) follow after the loop.
                 f3 gx = mk3(0, 0, 0), gy = mk3(0, 0, 0), gz = mk3(0, 0, 0);
-                #pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
-                    f3 cv;
-                    if (cp) {
-                        // g2p.rs:206-233: a node the particle is not compatible with gives
-                        // its incompatible velocity, seen through its closest collider's
-                        // boundary condition (friction scaled by the particle's factor)
-                        // relative to the body's velocity there.
+                if (cp) {
+                    // g2p.rs:206-233: a node the particle is not compatible with gives its
+                    // incompatible velocity, seen through its closest collider's boundary
+                    // condition (friction scaled by the particle's factor) relative to the
+                    // body's velocity there.  Unrolled by 9: enough loads in flight without
+                    // the register count of a full unroll.
+                    const f3 pn = mk3(w.out.plane(PNX)[pos], w.out.plane(PNY)[pos], w.out.plane(PNZ)[pos]);
+                    #pragma unroll 9
+                    for (int q = 0; q < 27; ++q) {
+                        const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
                         const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
-                        cv = mk3(0, 0, 0);
+                        f3 cv = mk3(0, 0, 0);
                         if (ni != NONE) {
                             const uint4 nc = gather.ncdf[ni];
-                            if (aff_compatible(pa, nc.y)) {
-                                const float4 g = gather.gridv[ni];
-                                cv = mk3(g.x, g.y, g.z);
-                            } else {
-                                const float4 g = gather.gridv_i[ni];
-                                cv = mk3(g.x, g.y, g.z);
-                                if (nc.z != NONE) {
-                                    const ColliderDev &c = w.cols[sc.col_start + nc.z];
-                                    const f3 center = mk3(ref.x + (float)sx * h + px, ref.y + (float)sy * h + py,
-                                                          ref.z + (float)sz * h + pz);
-                                    const f3 bpv = body_point_velocity(c, center);
-                                    cv = bpv + bc_project(c.bc_type, c.bc_friction * cls.bfric, cv - bpv, pn);
-                                }
+                            const bool comp = aff_compatible(pa, nc.y);
+                            const float4 g = comp ? gather.gridv[ni] : gather.gridv_i[ni];
+                            cv = mk3(g.x, g.y, g.z);
+                            if (!comp && nc.z != NONE) {
+                                // the closest body's velocity at the node and its boundary
+                                // condition {type in ncdf.w, friction}, stored by k_grid_cdf
+                                const float4 nb = w.nbody[ni];
+                                const f3 bpv = mk3(nb.x, nb.y, nb.z);
+                                cv = bpv + bc_project(nc.w, nb.w * cls.bfric, cv - bpv, pn);
                             }
                         }
-                    } else {
-                        cv = gather.at(bx + sx, by + sy, bz + sz);
+                        const float wsx = sx == 0 ? wx[0] : (sx == 1 ? wx[1] : wx[2]);
+                        const float wsy = sy == 0 ? wy[0] : (sy == 1 ? wy[1] : wy[2]);
+                        const float wsz = sz == 0 ? wz[0] : (sz == 1 ? wz[1] : wz[2]);
+                        const f3 cw = cv * (wsx * wsy * wsz);
+                        vel = vel + cw;
+                        gx = gx + cw * (float)sx;
+                        gy = gy + cw * (float)sy;
+                        gz = gz + cw * (float)sz;
                     }
-                    const float wgt = wx[sx] * wy[sy] * wz[sz];
-                    const f3 cw = cv * wgt;
-                    vel = vel + cw;
-                    if (sx == 1) gx = gx + cw; else if (sx == 2) gx = gx + cw * 2.0f;
-                    if (sy == 1) gy = gy + cw; else if (sy == 2) gy = gy + cw * 2.0f;
-                    if (sz == 1) gz = gz + cw; else if (sz == 2) gz = gz + cw * 2.0f;
+                } else {
+                    #pragma unroll
+                    for (int q = 0; q < 27; ++q) {
+                        const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
+                        const f3 cv = gather.at(bx + sx, by + sy, bz + sz);
+                        const float wgt = wx[sx] * wy[sy] * wz[sz];
+                        const f3 cw = cv * wgt;
+                        vel = vel + cw;
+                        if (sx == 1) gx = gx + cw; else if (sx == 2) gx = gx + cw * 2.0f;
+                        if (sy == 1) gy = gy + cw; else if (sy == 2) gy = gy + cw * 2.0f;
+                        if (sz == 1) gz = gz + cw; else if (sz == 2) gz = gz + cw * 2.0f;
+                    }
                 }
                 gradv.c0 = (vel * ref.x + gx * h) * inv_d;
                 gradv.c1 = (vel * ref.y + gy * h) * inv_d;
@@ -1066,6 +1084,13 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
             // Particle update.  CPIC: a particle deeper than 0.05 h behind a surface is
             // reprojected (slip relative to the bodies) and pushed out by a penalty
             // impulse after the position update (particle_update.rs:92-136).
+            // (The contact field is re-read here so that it is not live across the node loop.)
+            f3 pn = mk3(0, 0, 0);
+            float psd = 0.0f;
+            if (cp) {
+                pn = mk3(w.out.plane(PNX)[pos], w.out.plane(PNY)[pos], w.out.plane(PNZ)[pos]);
+                psd = w.out.plane(PSD)[pos];
+            }
             const bool pen = cp && psd < -0.05f * h;
             if (pen) vel = rigid_vel + bc_project(1u, 0.0f, vel - rigid_vel, pn);
             const float vl = length(vel);
@@ -1156,16 +1181,39 @@ __device__ inline void clear_hash(const WorldDev &w)
     }
 }
 
-// One thread per sorted particle, nodes read from the grid.
+// Contact particles of CPIC scenes: those with affinity bits (the others are compatible
+// with every node and have a zero contact field, so the plain path is exact for them).
+__device__ inline bool contact_particle(const WorldDev &w, uint32_t pos, uint32_t d)
+{
+    return w.scenes[w.bscene[d]].cpic && (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
+}
+
+// One thread per sorted particle, nodes read from the grid.  In CPIC worlds this kernel
+// takes the plain particles and k_g2p_contact the contact ones, so that the contact code
+// does not weigh on the register budget of the common path.
 template <bool CPIC>
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
     GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
-    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x)
-        g2p_particle<GatherGrid, CPIC>(w, pos, w.sblock[pos], grid);
+    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
+        const uint32_t d = w.sblock[pos];
+        if (CPIC && contact_particle(w, pos, d)) continue;
+        g2p_particle<GatherGrid, false>(w, pos, d, grid);
+    }
     clear_hash(w);
+}
+
+// The contact particles listed by k_particle_cdf.
+__global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
+{
+    const uint32_t n = w.g->n_contact;
+    GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pos = w.clist[i];
+        g2p_particle<GatherGrid, true>(w, pos, w.sblock[pos], grid);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1176,22 +1224,36 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 // analytic cuboids alone.
 __global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
 {
-    uint32_t nbegin, nend;
-    xcd_range(w.g->n_blocks * NODES_PER_BLOCK, nbegin, nend);
-    for (uint32_t n = nbegin + threadIdx.x; n < nend; n += blockDim.x) {
-        const uint32_t d = n >> 6;
+    // One wave per block: lane = node; bany[d] records whether any node of block d is in
+    // reach of a collider (the particle pass skips stencils without such nodes).
+    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+    const uint32_t l = threadIdx.x % WAVE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.g->n_contact = 0u;   // contact list of this substep
+    for (uint32_t d = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; d < w.g->n_blocks; d += waves) {
         const SceneDev sc = w.scenes[w.bscene[d]];
         if (!sc.cpic) continue;
         const int4 v = w.bvid[d];
         float dist = MAX_FLT_REF;
-        uint32_t aff = 0u, closest = NONE;
-        if (v.x != OVERFLOW_VID) {
-            const uint32_t li = n & 63u;
-            const int i = li & 3, j = (li >> 2) & 3, k = li >> 4;
+        uint32_t aff = 0u, closest = NONE, bc = 0u;
+        const uint32_t mask = v.x != OVERFLOW_VID ? w.bmask[d] : 0u;
+        const size_t n = (size_t)d * NODES_PER_BLOCK + l;
+        if (mask) {
+            const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
             const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
-            node_cdf(w.cols + sc.col_start, sc.col_count, w.bmask[d], sc.h, pt, dist, aff, closest);
+            node_cdf(w.cols + sc.col_start, sc.col_count, mask, sc.h, pt, dist, aff, closest);
+            if (closest != NONE) {
+                // For G2P's incompatible nodes: the closest body's velocity at the node
+                // (g2p.rs:214-231 evaluates it at the node position as dpt + x) and its
+                // boundary condition.
+                const ColliderDev &c = w.cols[sc.col_start + closest];
+                const f3 bpv = body_point_velocity(c, pt);
+                w.nbody[n] = make_float4(bpv.x, bpv.y, bpv.z, c.bc_friction);
+                bc = c.bc_type;
+            }
         }
-        w.ncdf[n] = make_uint4(__float_as_uint(dist), aff, closest, 0u);
+        w.ncdf[n] = make_uint4(__float_as_uint(dist), aff, closest, bc);
+        const uint64_t any = __ballot(aff != 0u);
+        if (l == 0) w.bany[d] = any != 0ull ? 1u : 0u;
     }
 }
 
@@ -1203,98 +1265,131 @@ __global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
 // gives the normal and the signed distance; when its matrix is singular (det <= 1e-8) the
 // field is zero, affinity included.  Results go to the output arena at the sorted
 // position, where P2G and G2P read them and where they stay as the next state.
+__device__ __forceinline__ uint32_t particle_cdf_one(const WorldDev &w, uint32_t pos, uint32_t d, const SceneDev &sc)
+{
+    const uint32_t slot = w.perm[pos];
+    const uint32_t prev = w.in.uplane(PAFF)[slot];
+    uint32_t aff = prev;
+    f3 nrm = mk3(0, 0, 0);
+    float sd = 0.0f;
+    const int4 bv = w.bvid[d];
+    if (bv.x != OVERFLOW_VID) {
+        const float px = w.in.plane(PX)[slot], py = w.in.plane(PY)[slot], pz = w.in.plane(PZ)[slot];
+        const float h = sc.h, ih = sc.inv_h;
+        const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
+                    cz = round_div(pz, h, ih) - 1.0f;
+        const int bx = (int)cx - 4 * bv.x, by = (int)cy - 4 * bv.y, bz = (int)cz - 4 * bv.z;
+        if ((unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u) {
+            const f3 ref = mk3(cx * h - px, cy * h - py, cz * h - pz);
+            float wx[3], wy[3], wz[3];
+            {
+                const float fx = -ref.x * ih, fy = -ref.y * ih, fz = -ref.z * ih;
+                wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
+                wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
+                wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
+            }
+            GatherGrid gather{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+            gather.prepare(d);
+            // No node of the 8 blocks of the stencil is in reach of a collider: every
+            // affinity is empty, the fit has no node and the field is zero.
+            uint32_t reach = 0u;
+            reach |= gather.n0 != NONE ? w.bany[gather.n0] : 0u;
+            reach |= gather.n1 != NONE ? w.bany[gather.n1] : 0u;
+            reach |= gather.n2 != NONE ? w.bany[gather.n2] : 0u;
+            reach |= gather.n3 != NONE ? w.bany[gather.n3] : 0u;
+            reach |= gather.n4 != NONE ? w.bany[gather.n4] : 0u;
+            reach |= gather.n5 != NONE ? w.bany[gather.n5] : 0u;
+            reach |= gather.n6 != NONE ? w.bany[gather.n6] : 0u;
+            reach |= gather.n7 != NONE ? w.bany[gather.n7] : 0u;
+            if (reach == 0u) {
+                w.out.uplane(PAFF)[pos] = 0u;
+                return 0u;
+            }
+            float nd[27], wq[27];
+            uint32_t na[27];
+            uint32_t pa = 0u;
+            #pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
+                const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
+                const uint4 nc = ni == NONE ? make_uint4(__float_as_uint(MAX_FLT_REF), 0u, NONE, 0u) : w.ncdf[ni];
+                nd[q] = __uint_as_float(nc.x);
+                na[q] = nc.y;
+                wq[q] = wx[sx] * wy[sy] * wz[sz];
+                pa |= nc.y & 0xFFFFu;
+            }
+            // Sides: kept for colliders known before, from the weighted signed
+            // distances for newly reached ones.
+            pa |= prev & ((prev & 0xFFFFu) << 16);
+            for (uint32_t c = 0; c < sc.col_count; ++c) {
+                if ((prev >> c) & 1u) continue;
+                float acc = 0.0f;
+                #pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const float comp = (na[q] >> c) & 1u ? 1.0f : 0.0f;
+                    const float sgn = (na[q] >> (16 + c)) & 1u ? -1.0f : 1.0f;
+                    acc += comp * wq[q] * sgn * nd[q];
+                }
+                if (acc < 0.0f) pa |= 0x00010000u << c;
+            }
+            Sym4 m{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            #pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const uint32_t comb = na[q] & pa & 0xFFFFu;
+                if (comb == 0u) continue;
+                const uint32_t sdiff = ((na[q] >> 16) ^ (pa >> 16)) & comb;
+                const float sx = (float)NBH[q][0] * h, sy = (float)NBH[q][1] * h, sz = (float)NBH[q][2] * h;
+                const float dx = ref.x + sx, dy = ref.y + sy, dz = ref.z + sz;
+                const float wgt = wq[q];
+                const float dd = sdiff == 0u ? nd[q] : -nd[q];
+                m.q00 += dx * dx * wgt; m.q01 += dx * dy * wgt; m.q02 += dx * dz * wgt; m.q03 += dx * wgt;
+                m.q11 += dy * dy * wgt; m.q12 += dy * dz * wgt; m.q13 += dy * wgt;
+                m.q22 += dz * dz * wgt; m.q23 += dz * wgt;
+                m.q33 += wgt;
+                u[0] += dx * (wgt * dd); u[1] += dy * (wgt * dd); u[2] += dz * (wgt * dd); u[3] += wgt * dd;
+            }
+            float res[4];
+            if (sym4_solve(m, u, res)) {
+                const f3 g = mk3(res[0], res[1], res[2]);
+                const float len = length(g);
+                nrm = len > 1.0e-6f ? g / len : mk3(0, 0, 0);
+                sd = res[3];
+                aff = pa;
+            } else {
+                aff = 0u;
+            }
+        }
+    }
+    // The normal and signed distance are only read where the affinity is not empty.
+    w.out.uplane(PAFF)[pos] = aff;
+    if (aff == 0u) return 0u;
+    w.out.plane(PNX)[pos] = nrm.x;
+    w.out.plane(PNY)[pos] = nrm.y;
+    w.out.plane(PNZ)[pos] = nrm.z;
+    w.out.plane(PSD)[pos] = sd;
+    return aff;
+}
+
 __global__ void __launch_bounds__(256) k_particle_cdf(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
+    const uint32_t l = lane_id();
     for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
         const uint32_t d = w.sblock[pos];
         const SceneDev sc = w.scenes[w.bscene[d]];
-        if (!sc.cpic) continue;
-        const uint32_t slot = w.perm[pos];
-        const uint32_t prev = w.in.uplane(PAFF)[slot];
-        uint32_t aff = prev;
-        f3 nrm = mk3(0, 0, 0);
-        float sd = 0.0f;
-        const int4 bv = w.bvid[d];
-        if (bv.x != OVERFLOW_VID) {
-            const float px = w.in.plane(PX)[slot], py = w.in.plane(PY)[slot], pz = w.in.plane(PZ)[slot];
-            const float h = sc.h, ih = sc.inv_h;
-            const float cx = round_div(px, h, ih) - 1.0f, cy = round_div(py, h, ih) - 1.0f,
-                        cz = round_div(pz, h, ih) - 1.0f;
-            const int bx = (int)cx - 4 * bv.x, by = (int)cy - 4 * bv.y, bz = (int)cz - 4 * bv.z;
-            if ((unsigned)bx < 4u && (unsigned)by < 4u && (unsigned)bz < 4u) {
-                const f3 ref = mk3(cx * h - px, cy * h - py, cz * h - pz);
-                float wx[3], wy[3], wz[3];
-                {
-                    const float fx = -ref.x * ih, fy = -ref.y * ih, fz = -ref.z * ih;
-                    wx[0] = 0.5f * (1.5f - fx) * (1.5f - fx); wx[1] = 0.75f - (fx - 1.0f) * (fx - 1.0f); wx[2] = 0.5f * (fx - 0.5f) * (fx - 0.5f);
-                    wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
-                    wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
-                }
-                GatherGrid gather{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
-                gather.prepare(d);
-                float nd[27], wq[27];
-                uint32_t na[27];
-                uint32_t pa = 0u;
-                #pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
-                    const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
-                    const uint4 nc = ni == NONE ? make_uint4(__float_as_uint(MAX_FLT_REF), 0u, NONE, 0u) : w.ncdf[ni];
-                    nd[q] = __uint_as_float(nc.x);
-                    na[q] = nc.y;
-                    wq[q] = wx[sx] * wy[sy] * wz[sz];
-                    pa |= nc.y & 0xFFFFu;
-                }
-                // Sides: kept for colliders known before, from the weighted signed
-                // distances for newly reached ones.
-                pa |= prev & ((prev & 0xFFFFu) << 16);
-                for (uint32_t c = 0; c < sc.col_count; ++c) {
-                    if ((prev >> c) & 1u) continue;
-                    float acc = 0.0f;
-                    #pragma unroll
-                    for (int q = 0; q < 27; ++q) {
-                        const float comp = (na[q] >> c) & 1u ? 1.0f : 0.0f;
-                        const float sgn = (na[q] >> (16 + c)) & 1u ? -1.0f : 1.0f;
-                        acc += comp * wq[q] * sgn * nd[q];
-                    }
-                    if (acc < 0.0f) pa |= 0x00010000u << c;
-                }
-                Sym4 m{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                #pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const uint32_t comb = na[q] & pa & 0xFFFFu;
-                    if (comb == 0u) continue;
-                    const uint32_t sdiff = ((na[q] >> 16) ^ (pa >> 16)) & comb;
-                    const float sx = (float)NBH[q][0] * h, sy = (float)NBH[q][1] * h, sz = (float)NBH[q][2] * h;
-                    const float dx = ref.x + sx, dy = ref.y + sy, dz = ref.z + sz;
-                    const float wgt = wq[q];
-                    const float dd = sdiff == 0u ? nd[q] : -nd[q];
-                    m.q00 += dx * dx * wgt; m.q01 += dx * dy * wgt; m.q02 += dx * dz * wgt; m.q03 += dx * wgt;
-                    m.q11 += dy * dy * wgt; m.q12 += dy * dz * wgt; m.q13 += dy * wgt;
-                    m.q22 += dz * dz * wgt; m.q23 += dz * wgt;
-                    m.q33 += wgt;
-                    u[0] += dx * (wgt * dd); u[1] += dy * (wgt * dd); u[2] += dz * (wgt * dd); u[3] += wgt * dd;
-                }
-                float res[4];
-                if (sym4_solve(m, u, res)) {
-                    const f3 g = mk3(res[0], res[1], res[2]);
-                    const float len = length(g);
-                    nrm = len > 1.0e-6f ? g / len : mk3(0, 0, 0);
-                    sd = res[3];
-                    aff = pa;
-                } else {
-                    aff = 0u;
-                }
-            }
+        const bool contact = sc.cpic && (particle_cdf_one(w, pos, d, sc) & 0xFFFFu) != 0u;
+        // Contact particles go to a list (one counter atomic per wave) that k_g2p_contact
+        // processes densely; the order of the list does not matter.
+        const uint64_t active = __ballot(1), m = __ballot(contact);
+        if (m) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)active) - 1u;
+            uint32_t base = 0u;
+            if (l == leader) base = atomicAdd(&w.g->n_contact, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, WAVE);
+            if (contact) w.clist[base + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = pos;
         }
-        w.out.uplane(PAFF)[pos] = aff;
-        w.out.plane(PNX)[pos] = nrm.x;
-        w.out.plane(PNY)[pos] = nrm.y;
-        w.out.plane(PNZ)[pos] = nrm.z;
-        w.out.plane(PSD)[pos] = sd;
     }
 }
 
@@ -1443,13 +1538,14 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (w.any_cpic) {
         // CPIC worlds: the contact distance fields, then the CPIC variants of the
         // transfers (scenes without CPIC take the plain path inside them).
-        k_grid_cdf<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        k_grid_cdf<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);   // one wave per block
         k_particle_cdf<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
         k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);
         k_grid_update<true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
         k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     } else {
         k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);

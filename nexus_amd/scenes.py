@@ -264,6 +264,36 @@ def sand_column(n: int = 1_048_576, seed: int = SEED_BASE + 2, *, cell_width=1.0
                       name=f"Drucker-Prager sand column {n}")
 
 
+def scaled(scene: Scene, k: float, *, cpic: bool | None = None, name: str | None = None) -> Scene:
+    """The same scene k times larger: positions, radii, colliders and the cell width scale
+    by k, volumes and masses by k^3, dt by k (the same Courant number)."""
+    out = scene.copy()
+    f = np.float32(k)
+    out.x = (out.x * f).astype(np.float32)
+    out.radius = (out.radius * f).astype(np.float32)
+    out.vol0 = (out.vol0 * f ** 3).astype(np.float32)
+    out.mass = (out.mass * f ** 3).astype(np.float32)
+    cols = out.colliders.copy()
+    if cols.size:
+        cols["translation"] = cols["translation"] * f
+        cols["half_extents"] = cols["half_extents"] * f
+    out.colliders = cols
+    out.cell_width = float(np.float32(scene.cell_width * k))
+    out.dt = float(np.float32(scene.dt * k))
+    if cpic is not None:
+        out.cpic = cpic
+    if name is not None:
+        out.name = name
+    return out
+
+
+def sand_column_cpic(n: int = 1_048_576, seed: int = SEED_BASE + 2) -> Scene:
+    """The sand column 64 times larger (h = 0.25, dt = 3.2e-3, the same Courant number)
+    with CPIC: the contact distance field needs cell widths of a few tenths (its
+    least-squares matrix is singular below, g2p_cdf.rs:309)."""
+    return scaled(sand_column(n, seed), 64.0, cpic=True, name=f"Drucker-Prager sand column {n}, x64, CPIC")
+
+
 def fluid_clumps(n: int = 8_388_608, seed: int = SEED_BASE + 3, *, n_clumps: int = 64,
                  cell_width=1.0 / 512, sigma=(0.01, 0.05)) -> Scene:
     """Workload 3: weakly-compressible fluid (K = 1e4, gamma = 7, density 1000),

@@ -163,3 +163,13 @@ def test_cpic_switch_resets_the_contact_field(pipeline):
     assert np.array_equal(st.read_cdf()["affinity"], ref.read_cdf()["affinity"])
     st.close()
     ref.close()
+
+
+def test_cpic_sand_column_matches_oracle(pipeline):
+    """The benchmark's CPIC workload (the sand column at 64x scale: 16 rotated cuboids,
+    Drucker-Prager) at 20,000 particles, 4 substeps."""
+    sc = scenes.sand_column_cpic(20000)
+    st = run(pipeline, sc, 4)
+    p, cdf, o64 = check_against_oracle(st, sc, 4)
+    assert (cdf["affinity"] & 0xFFFF).any()
+    st.close()
