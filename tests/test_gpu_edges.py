@@ -99,6 +99,44 @@ def test_snow_compaction_matches_oracle(pipeline):
     st.close()
 
 
+def test_all_models_in_one_scene(pipeline):
+    """One block of each model tag (corotated linear, neo-Hookean, Drucker-Prager over the
+    linear and the neo-Hookean backbone with the reference's default hardening law, fluid,
+    snow) thrown onto a floor and onto each other: 40 substeps against the oracle, the
+    per-particle plastic state included."""
+    h = 1 / 64
+    # lower row on the floor: corotated, sand (linear backbone), snow; upper row falling onto
+    # it: neo-Hookean, sand (neo-Hookean backbone), fluid
+    models = [(scenes.model_linear(2e5, 0.3), 1000.0), (scenes.model_sand(3.5e5, 0.3), 2200.0),
+              (scenes.model_snow(1.4e5, 0.2), 400.0), (scenes.model_neo_hookean(2e5, 0.3), 1000.0),
+              (scenes.model_sand(3.5e5, 0.3, neo_hookean=True), 2200.0), (scenes.model_fluid(1e4), 1000.0)]
+    parts = []
+    for i, (model, rho) in enumerate(models):
+        c = (0.35 + 0.1 * (i % 3), 0.28 + 0.06 * (i // 3), 0.5)
+        vel = (0.8 * (1 - 2 * (i % 2)), -4.0, 0.2)
+        parts.append(_block(6, c, h, model, rho, velocity=vel, seed=20 + i))
+    # The floor's top face is kept off the grid planes: a node exactly on a face has a zero
+    # distance and no normal (grid_update_collide.rs:84-87), where FP32 and FP64 geometry
+    # disagree for the separate condition.
+    floor = scenes.cuboid((0.5, 0.2013, 0.5), (0.4, 0.05, 0.4), bc=scenes.BC_SEPARATE, friction=0.5)
+    sc = scenes.make_scene(scenes.concat(parts), h, 1e-4, colliders=floor)
+    st = mpm.MpmState.from_scene(pipeline, sc)
+    for _ in range(40):
+        pipeline.step(st)
+    p = st.read_particles()
+    o32, o64 = oracle_pair(sc, 40)
+    assert np.array_equal(p["enabled"], o32.s.enabled)
+    for f in ("x", "v", "C", "F"):
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+    words = p["model"].view(np.float32)
+    sand = np.isin(p["tag"], (2, 3))
+    snow = p["tag"] == 5
+    assert_field_close("sand plastic state", words[sand, :3], o32.s.pstate[sand], o64.s.pstate[sand])
+    assert_field_close("snow plastic det", words[snow, 0], o32.s.pstate[snow, 0], o64.s.pstate[snow, 0])
+    assert (np.abs(o64.s.pstate[sand, 0] - 1) > 1e-4).any() and (np.abs(o64.s.pstate[snow, 0] - 1) > 1e-4).any()
+    st.close()
+
+
 def test_domain_error_is_reported(pipeline):
     """A particle outside the packable block range (|block x| > 1023) is reported as
     NXM_ERR_DOMAIN at the next synchronizing call; the step still runs for the others."""
