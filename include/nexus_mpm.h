@@ -185,8 +185,16 @@ nxm_status nxm_scene_set_cell_width(nxm_scene *scene, float cell_width, uint32_t
 nxm_status nxm_scene_add_particles(nxm_scene *scene, const nxm_particle *particles, size_t n);
 nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa *p, size_t n);
 /* MpmState::set_coupling (src_mpm/pipeline.rs:220) restricted to one-way cuboids (at
- * most 16, the reference's GpuMaterials limit). */
+ * most 16, the reference's GpuMaterials limit).  Sets the poses; from then on the device
+ * integrates them every substep (nxm_scene_read_colliders). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
+/* Current poses of the scene's colliders.  MPM integrates its coupled bodies every
+ * substep (src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in
+ * src_mpm_shaders/solver/rigid_impulses.rs:66-140): the cuboids carry no mass, so each
+ * pose advances with its linear and angular velocity about its centre.  Counterpart of
+ * MpmPipeline::writeback_body_poses (src_mpm/pipeline.rs:536).  Synchronizes when the
+ * scene is resident; out sized n >= the number of colliders. */
+nxm_status nxm_scene_read_colliders(nxm_scene *scene, nxm_cuboid *out, size_t n);
 /* MpmState::use_cpic (src_mpm/pipeline.rs:69, default off here): compatible
  * particle-in-cell transfers (pipeline.rs:433-460 with the use_cpic branches of p2g.rs,
  * grid_update.rs, g2p.rs and particle_update.rs) instead of the grid collider pass.

@@ -160,6 +160,10 @@ struct nxm_scene {
     bool on_host = true;                // hp state authoritative
     World *world = nullptr;             // resident world (own or a batch's)
     uint32_t slot = 0;
+    // Colliders: the world's table holds the integrated poses while the scene is resident
+    // (dev_col_start); cols_host says the host copy is newer (set_colliders, or not resident).
+    uint32_t dev_col_start = 0xFFFFFFFFu;
+    bool cols_host = true;
     std::vector<SceneRange> ranges;     // particle ranges in the world's input arena
     std::unique_ptr<World> own;         // standalone world
     nxm_batch *batch = nullptr;
@@ -305,6 +309,8 @@ class World {
         slots_[sc->slot].scene = nullptr;
         sc->world = nullptr;
         sc->ranges.clear();
+        sc->dev_col_start = NONE;
+        sc->cols_host = true;
         dirty_ = true;
     }
 
@@ -313,6 +319,8 @@ class World {
     {
         nxm_status st = pull(sc);
         if (st != NXM_OK) return st;
+        sc->dev_col_start = NONE;
+        sc->cols_host = true;
         slots_[sc->slot].state = SLOT_RETIRING;
         slots_[sc->slot].scene = nullptr;
         sc->world = nullptr;
@@ -324,10 +332,21 @@ class World {
 
     void mark_dirty() { dirty_ = true; }
 
+    // Synchronizes and copies the scene's integrated collider poses to its host copy.
+    nxm_status pull_colliders(nxm_scene *sc)
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
+        if (!sc->cols_host && sc->dev_col_start != NONE && !sc->cols.empty())
+            HIP_TRY(hipMemcpy(sc->cols.data(), cols_[cols_cur_].p + sc->dev_col_start, sc->cols.size() * sizeof(ColliderDev),
+                              hipMemcpyDeviceToHost));
+        return NXM_OK;
+    }
+
     // Reads the scene's particle state into its host arrays (id order).
     nxm_status pull(nxm_scene *sc)
     {
-        nxm_status st = sync();
+        nxm_status st = pull_colliders(sc);
         if (st != NXM_OK) return st;
         HostParticles &hp = sc->hp;
         Arena a = arena(cur_);
@@ -733,7 +752,7 @@ class World {
         w.bup = bup_.p; w.bdn = bdn_.p; w.stage = stage_.p; w.gridv = gridv_.p;
         w.hkeys = hkeys_.p; w.hvals = hvals_.p; w.hmark = hmark_.p; w.bprim = bprim_.p; w.bmask = bmask_.p;
         w.region_start = region_start_.p; w.region_scene = region_scene_.p; w.n_regions = n_regions_;
-        w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_.p;
+        w.scenes = scenes_.p; w.ctr = ctr_.p; w.classes = classes_.p; w.cols = cols_[cols_cur_].p;
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
         w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
@@ -777,7 +796,7 @@ class World {
     {
         layout_.assign(slots_.size(), SceneDev{});
         std::vector<ClassDev> cls;
-        std::vector<ColliderDev> cols;
+        uint32_t ncols = 0;
         std::vector<uint32_t> region_start, region_scene;
         uint32_t b = 0, hsum = 0;
         bool any_cpic = false;
@@ -799,9 +818,9 @@ class World {
             hsum += L.h_cap + 1;
             region_start.push_back(L.h_base);
             region_scene.push_back(i);
-            L.col_start = (uint32_t)cols.size();
+            L.col_start = ncols;
             L.col_count = (uint32_t)sc->cols.size();
-            cols.insert(cols.end(), sc->cols.begin(), sc->cols.end());
+            ncols += L.col_count;
             L.class_base = (uint32_t)cls.size();
             cls.insert(cls.end(), sc->classes.begin(), sc->classes.end());
             L.cpic = sc->cpic ? 1u : 0u;
@@ -816,7 +835,7 @@ class World {
         const uint32_t scan_stride = std::max<uint32_t>(hsum, hsum) / 1024u + 2u;
         const uint32_t nreg = (uint32_t)region_start.size();
         const bool grow = std::max<size_t>(slots_.size(), 1) > scenes_.n || std::max<size_t>(slots_.size(), 1) > ctr_.n ||
-                          std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(cols.size(), 1) > cols_.n ||
+                          std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(ncols, 1) > cols_[0].n || std::max<size_t>(ncols, 1) > cols_[1].n ||
                           hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
                           std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
                           (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n) ||
@@ -824,6 +843,14 @@ class World {
         if (grow) {
             nxm_status st = sync();
             if (st != NXM_OK) return st;
+            // The collider tables may be reallocated: keep the integrated poses on the host.
+            for (uint32_t i = 0; i < slots_.size(); ++i) {
+                if (slots_[i].state != SLOT_ACTIVE) continue;
+                nxm_scene *sc = slots_[i].scene;
+                st = pull_colliders(sc);
+                if (st != NXM_OK) return st;
+                sc->cols_host = true;
+            }
         }
         h_total_ = hsum;
         b_total_ = hsum;
@@ -834,11 +861,31 @@ class World {
             ctr_zeroed_ = ctr_.n;
         }
         HIP_TRY(classes_.reserve(std::max<size_t>(cls.size(), 1)));
-        HIP_TRY(cols_.reserve(std::max<size_t>(cols.size(), 1)));
+        HIP_TRY(cols_[0].reserve(std::max<size_t>(ncols, 1)));
+        HIP_TRY(cols_[1].reserve(std::max<size_t>(ncols, 1)));
         nxm_status st = put_async(scenes_.p, layout_.data(), layout_.size() * sizeof(SceneDev));
         if (st == NXM_OK) st = put_async(classes_.p, cls.data(), cls.size() * sizeof(ClassDev));
-        if (st == NXM_OK) st = put_async(cols_.p, cols.data(), cols.size() * sizeof(ColliderDev));
         if (st != NXM_OK) return st;
+        // The new collider table: scenes resident before keep their integrated poses (copied
+        // on the stream from the current table), the others take their host copy.
+        const int nc = cols_cur_ ^ 1;
+        for (uint32_t i = 0; i < slots_.size(); ++i) {
+            if (slots_[i].state != SLOT_ACTIVE) continue;
+            nxm_scene *sc = slots_[i].scene;
+            const SceneDev &L = layout_[i];
+            if (L.col_count == 0) { sc->dev_col_start = L.col_start; sc->cols_host = false; continue; }
+            ColliderDev *dst = cols_[nc].p + L.col_start;
+            if (sc->cols_host || sc->dev_col_start == NONE) {
+                st = put_async(dst, sc->cols.data(), L.col_count * sizeof(ColliderDev));
+                if (st != NXM_OK) return st;
+            } else {
+                HIP_TRY(hipMemcpyAsync(dst, cols_[cols_cur_].p + sc->dev_col_start, L.col_count * sizeof(ColliderDev),
+                                       hipMemcpyDeviceToDevice, stream_));
+            }
+            sc->dev_col_start = L.col_start;
+            sc->cols_host = false;
+        }
+        cols_cur_ = nc;
         if (b_total_ > bscene_.n) {
             size_t nb = (size_t)b_total_ + b_total_ / 4;
             HIP_TRY(bscene_.reserve(nb));
@@ -1106,7 +1153,8 @@ class World {
     DevBuf<SceneDev> scenes_;
     DevBuf<SceneCtr> ctr_;
     DevBuf<ClassDev> classes_;
-    DevBuf<ColliderDev> cols_;
+    DevBuf<ColliderDev> cols_[2];      // collider tables (integrated on the device), swapped at each rebuild
+    int cols_cur_ = 0;
     DevBuf<Globals> g_;
 };
 
@@ -1366,8 +1414,23 @@ nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
     sc->cols.resize(n);
     static_assert(sizeof(ColliderDev) == sizeof(nxm_cuboid), "collider layout");
     if (n) std::memcpy(sc->cols.data(), c, n * sizeof(nxm_cuboid));
+    sc->cols_host = true;
     if (sc->world) sc->world->mark_dirty();
     return NXM_OK;
+}
+
+static nxm_status latched(nxm_scene *sc, nxm_status st);
+
+nxm_status nxm_scene_read_colliders(nxm_scene *sc, nxm_cuboid *out, size_t n)
+{
+    if (!sc || (!out && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < sc->cols.size()) return fail(NXM_ERR_INVALID_ARGUMENT, "output too small");
+    if (sc->world) {
+        nxm_status st = sc->world->pull_colliders(sc);
+        if (st != NXM_OK) return st;
+    }
+    if (!sc->cols.empty()) std::memcpy(out, sc->cols.data(), sc->cols.size() * sizeof(nxm_cuboid));
+    return latched(sc, NXM_OK);
 }
 
 size_t nxm_scene_len(const nxm_scene *sc) { return sc ? sc->hp.n : 0; }

@@ -31,7 +31,7 @@ struct WorldDev {
     const SceneDev *scenes;
     SceneCtr *ctr;
     const ClassDev *classes;
-    const ColliderDev *cols;
+    ColliderDev *cols;             // integrated in place at the end of each substep
     Globals *g;
     uint32_t *p2g_start;           // P2G workgroup g: blocks [p2g_start[g], p2g_start[g+1])
     uint32_t p2g_groups;           // workgroups of the P2G launch

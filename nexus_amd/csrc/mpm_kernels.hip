@@ -1181,6 +1181,57 @@ __device__ inline void clear_hash(const WorldDev &w)
     }
 }
 
+// Body integration at the end of a substep (the reference's "Integrate bodies" pass,
+// src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in rigid_impulses.rs:66-140):
+// MPM integrates its own copy of every coupled body.  The boundary's cuboids carry no
+// mass (fixed or velocity-driven bodies: no impulse response, no gravity; two-way
+// coupling is compiled out in the reference, src_mpm_shaders/lib.rs:38), so each pose
+// advances with its velocity about its centre, the translation: t += v dt and
+// q = normalize(exp(w dt) q) (Velocity::integrate, src_rbd_shaders/dynamics/body.rs:
+// 253-268, with glam's from_scaled_axis, quaternion product and normalize).
+__device__ inline void integrate_collider(ColliderDev &c, float dt)
+{
+    c.t[0] = c.t[0] + c.linvel[0] * dt;
+    c.t[1] = c.t[1] + c.linvel[1] * dt;
+    c.t[2] = c.t[2] + c.linvel[2] * dt;
+    const f3 sa = mk3(c.angvel[0] * dt, c.angvel[1] * dt, c.angvel[2] * dt);
+    const float len = sqrtf(sa.x * sa.x + sa.y * sa.y + sa.z * sa.z);
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, dw = 1.0f;
+    if (len != 0.0f) {
+        float sn, cs;
+        sincosf(len * 0.5f, &sn, &cs);
+        dx = sa.x / len * sn;
+        dy = sa.y / len * sn;
+        dz = sa.z / len * sn;
+        dw = cs;
+    }
+    const float qx = c.q[0], qy = c.q[1], qz = c.q[2], qw = c.q[3];
+    const float nx = dw * qx + dx * qw + dy * qz - dz * qy;
+    const float ny = dw * qy - dx * qz + dy * qw + dz * qx;
+    const float nz = dw * qz + dx * qy - dy * qx + dz * qw;
+    const float nw = dw * qw - dx * qx - dy * qy - dz * qz;
+    const float r = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz + nw * nw);
+    c.q[0] = nx * r;
+    c.q[1] = ny * r;
+    c.q[2] = nz * r;
+    c.q[3] = nw * r;
+}
+
+// Every collider of every active scene, 16 slots per scene, grid-stride.  Runs inside the
+// substep's last kernel (nothing after it in the substep reads the colliders).
+__device__ inline void integrate_colliders_part(const WorldDev &w)
+{
+    const uint32_t n = w.s_count * 16u;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const SceneDev sc = w.scenes[t >> 4];
+        const uint32_t c = t & 15u;
+        if (!sc.active || c >= sc.col_count) continue;
+        integrate_collider(w.cols[sc.col_start + c], sc.dt);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_integrate_colliders(WorldDev w) { integrate_colliders_part(w); }
+
 // Contact particles of CPIC scenes: those with affinity bits (the others are compatible
 // with every node and have a zero contact field, so the plain path is exact for them).
 __device__ inline bool contact_particle(const WorldDev &w, uint32_t pos, uint32_t d)
@@ -1203,6 +1254,7 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
         g2p_particle<GatherGrid, false>(w, pos, d, grid);
     }
     clear_hash(w);
+    if (!CPIC) integrate_colliders_part(w);   // CPIC worlds: after k_g2p_contact, which reads them
 }
 
 // The contact particles listed by k_particle_cdf.
@@ -1546,6 +1598,7 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (ev) (void)hipEventRecord(ev[3], st);
         k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
         k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        k_integrate_colliders<<<std::max(1, (int)((w.s_count * 16u + T - 1) / T)), T, 0, st>>>(w);
     } else {
         k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);

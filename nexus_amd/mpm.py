@@ -43,7 +43,7 @@ EXPORTS = [
     "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
-    "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic",
+    "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
@@ -137,6 +137,7 @@ def library() -> ct.CDLL:
     L.nxm_scene_synchronize.argtypes = [P]
     L.nxm_scene_set_debug_grid.argtypes = [P, I]
     L.nxm_scene_set_cpic.argtypes = [P, I]
+    L.nxm_scene_read_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_cdf.argtypes = [P, P, P, P, ct.c_size_t]
     L.nxm_scene_read_grid_cpic.argtypes = [P, P, P, P, P, P]
     L.nxm_pipeline_step.argtypes = [P, P, P]
@@ -233,6 +234,13 @@ class MpmState:
         st.add_particles(scene)
         return st
 
+    def read_colliders(self) -> np.ndarray:
+        """The colliders' current poses (integrated with their velocities every substep)."""
+        n = int(getattr(self, "num_colliders", 0))
+        out = np.zeros(n, dtype=_scenes.CUBOID_DTYPE)
+        _check(self.lib.nxm_scene_read_colliders(self.handle, _ptr(out) if n else None, n))
+        return out
+
     def set_cpic(self, enabled: bool) -> None:
         """MpmState::use_cpic: compatible particle-in-cell transfers against the colliders."""
         _check(self.lib.nxm_scene_set_cpic(self.handle, int(bool(enabled))))
@@ -288,6 +296,7 @@ class MpmState:
     def set_colliders(self, cuboids: np.ndarray) -> None:
         c = np.ascontiguousarray(cuboids, dtype=_scenes.CUBOID_DTYPE)
         _check(self.lib.nxm_scene_set_colliders(self.handle, _ptr(c) if c.size else None, c.size))
+        self.num_colliders = int(c.size)
 
     def set_debug_grid(self, enabled: bool = True) -> None:
         _check(self.lib.nxm_scene_set_debug_grid(self.handle, int(enabled)))

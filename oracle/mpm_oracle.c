@@ -49,6 +49,7 @@
 #define RLOG log
 #define REXP exp
 #define RSIN sin
+#define RCOS cos
 #define RFABS fabs
 #define RROUND round
 #define RFLOOR floor
@@ -60,6 +61,7 @@
 #define RLOG logf
 #define REXP expf
 #define RSIN sinf
+#define RCOS cosf
 #define RFABS fabsf
 #define RROUND roundf
 #define RFLOOR floorf
@@ -775,6 +777,36 @@ static void particle_cdf(v3 refp, REAL h, const REAL *ndist[27], const uint32_t 
     }
 }
 
+/* Velocity::integrate of a massless cuboid centred at its translation:
+ * t' = t + v dt, q' = normalize(from_scaled_axis(w dt) q) (glam's from_scaled_axis:
+ * identity for a zero vector, else (axis sin(a/2), cos(a/2)); Hamilton product;
+ * normalize = q / |q|).  The pose is stored in f32, as the reference's. */
+static void integrate_cuboid(orc_cuboid *c, float dt)
+{
+    for (int k = 0; k < 3; ++k) c->translation[k] = (float)((REAL)c->translation[k] + (REAL)c->linvel[k] * (REAL)dt);
+    REAL sx = (REAL)c->angvel[0] * (REAL)dt, sy = (REAL)c->angvel[1] * (REAL)dt, sz = (REAL)c->angvel[2] * (REAL)dt;
+    REAL len = RSQRT(sx * sx + sy * sy + sz * sz);
+    REAL dx = 0, dy = 0, dz = 0, dw = 1;
+    if (len != K(0.0)) {
+        REAL half = len * K(0.5);
+        REAL sn = RSIN(half), cs = RCOS(half);
+        dx = sx / len * sn;
+        dy = sy / len * sn;
+        dz = sz / len * sn;
+        dw = cs;
+    }
+    REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
+    REAL nx = dw * qx + dx * qw + dy * qz - dz * qy;
+    REAL ny = dw * qy - dx * qz + dy * qw + dz * qx;
+    REAL nz = dw * qz + dx * qy - dy * qx + dz * qw;
+    REAL nw = dw * qw - dx * qx - dy * qy - dz * qz;
+    REAL r = K(1.0) / RSQRT(nx * nx + ny * ny + nz * nz + nw * nw);
+    c->rotation[0] = (float)(nx * r);
+    c->rotation[1] = (float)(ny * r);
+    c->rotation[2] = (float)(nz * r);
+    c->rotation[3] = (float)(nw * r);
+}
+
 static v3 body_point_velocity(const orc_cuboid *c, v3 pt)
 {
     v3 com = V3(c->translation[0], c->translation[1], c->translation[2]);
@@ -889,7 +921,7 @@ int orc_ws_grid_cpic(const orc_ws *ws, REAL *dist, uint32_t *aff, uint32_t *clos
 int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
              const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
              const uint32_t *fixed, const uint32_t *tag, const float *params, const float *sim,
-             const orc_cuboid *cols, int ncol, int keep_debug)
+             orc_cuboid *cols, int ncol, int keep_debug)
 {
     return orc_step_ex(ws, n, x, v, C, F, mass, vol0, damping, pstate, enabled, fixed, tag, params, sim, cols,
                        ncol, keep_debug, NULL);
@@ -898,7 +930,7 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
 int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
                 const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
                 const uint32_t *fixed, const uint32_t *tag, const float *params, const float *sim,
-                const orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp)
+                orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp)
 {
     (void)keep_debug;
     if (cp && ncol > CPIC_MAX_COLLIDERS) return -3;
@@ -1290,6 +1322,11 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
         mstore(F + 9 * p, Fm);
     }
     if (bad_model) rc = -2;
+
+    /* --- body integration (src_mpm/pipeline.rs:510-520, rigid_impulses.rs:66-140): the
+     *     cuboids are massless coupled bodies, so each pose advances with its velocity
+     *     about its centre (Velocity::integrate, src_rbd_shaders/dynamics/body.rs:253-268) --- */
+    for (int i = 0; i < ncol; ++i) integrate_cuboid(&cols[i], (float)sim[3]);
 
     /* --- keep inspection data --- */
     ws_reserve(ws, nb, nsorted);

@@ -53,12 +53,13 @@ void orc_ws_destroy(orc_ws *ws);
  * plastic state {det Fp, hardening q, log volume gain} (only read/written for
  * the sand tags); params: n*24 model words (f32 bit patterns, words 0..2 of the
  * sand tags are ignored in favour of pstate).  keep_debug != 0 keeps the sort and
- * grid of this substep in ws for inspection.  Returns 0 on success.
+ * grid of this substep in ws for inspection.  The colliders' poses are integrated
+ * with their velocities at the end of the substep (in place).  Returns 0 on success.
  */
 int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
              const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
              const uint32_t *fixed, const uint32_t *tag, const float *params,
-             const float *sim, const orc_cuboid *cols, int ncol, int keep_debug);
+             const float *sim, orc_cuboid *cols, int ncol, int keep_debug);
 
 /*
  * CPIC state (use_cpic = true in the reference, src_mpm/pipeline.rs:412-460): per
@@ -83,7 +84,7 @@ typedef struct {
 int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *mass,
                 const REAL *vol0, const REAL *damping, REAL *pstate, uint32_t *enabled,
                 const uint32_t *fixed, const uint32_t *tag, const float *params,
-                const float *sim, const orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp);
+                const float *sim, orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp);
 
 /* Inspection of the last substep run with keep_debug != 0. */
 int64_t orc_ws_nblocks(const orc_ws *ws);

@@ -269,3 +269,49 @@ def test_launch_shapes_do_not_change_results(pipeline):
         for a, b in zip(results[0], other):
             for f in ("x", "v", "C", "F", "enabled"):
                 assert np.array_equal(a[f], b[f]), f
+
+
+def _moving_floor_scene(seed=71):
+    sc = small_sand(6000, seed=seed)
+    q = np.array([np.sin(0.05), 0.0, 0.0, np.cos(0.05)], np.float32)
+    floor = scenes.cuboid((0.45, 0.06, 0.45), (0.2, 0.03, 0.2), rotation=q, bc=scenes.BC_SEPARATE, friction=0.6,
+                          linvel=(0.3, 0.4, -0.2), angvel=(0.5, 2.0, -1.0))
+    sc.colliders = np.concatenate([sc.colliders[:3], floor])
+    return sc
+
+
+def test_moving_colliders_integrate_like_the_oracle(pipeline):
+    """Coupled bodies are integrated by MPM every substep (src_mpm/pipeline.rs:510-520):
+    the moving, turning floor's pose and the sand on it after 30 substeps against the
+    oracle."""
+    sc = _moving_floor_scene()
+    st = run_gpu(pipeline, sc, 30)
+    cols = st.read_colliders()
+    o32, o64 = oracle_pair(sc, 30)
+    for f in ("translation", "rotation"):
+        assert np.allclose(cols[f], o64.s.colliders[f], rtol=0, atol=2e-6), f
+    assert not np.allclose(cols["translation"][-1], sc.colliders["translation"][-1])   # it moved
+    check_state(st.read_particles(), o32, o64)
+    st.close()
+
+
+def test_moving_colliders_survive_batch_rebuilds(pipeline):
+    """The integrated poses live on the device: a scene that joins a batch after stepping
+    alone, and whose batch is rebuilt when another scene joins, continues from its moved
+    colliders, bit for bit as if it had stepped alone."""
+    sc = _moving_floor_scene(seed=72)
+    ref = run_gpu(pipeline, sc, 10)
+    a, ca = ref.read_particles(), ref.read_colliders()
+    st = run_gpu(pipeline, sc, 3)                      # 3 substeps in its own world
+    batch = mpm.Batch(pipeline)
+    batch.add(st)                                      # moves into the batch's world
+    batch.run(4)
+    other = mpm.MpmState.from_scene(pipeline, scenes.elastic_cube(8))
+    batch.add(other)                                   # table rebuild with the poses kept
+    batch.run(3)
+    b, cb = st.read_particles(), st.read_colliders()
+    for f in ("translation", "rotation"):
+        assert np.array_equal(ca[f], cb[f]), f
+    for f in ("x", "v", "C", "F", "enabled"):
+        assert np.array_equal(a[f], b[f]), f
+    batch.close()

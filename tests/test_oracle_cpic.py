@@ -171,3 +171,27 @@ def test_cpic_without_colliders_in_reach_equals_plain_substep():
     for f in ("x", "v", "C", "F"):
         assert np.array_equal(getattr(a.s, f), getattr(b.s, f)), f
     assert np.all(a.s.affinity == 0)
+
+
+def test_collider_pose_integration():
+    """Body integration of the massless cuboids at the end of each substep: the
+    translation accumulates v dt in f32 (bit-exact against a float32 loop in the FP32
+    mirror) and a constant angular velocity about z turns the quaternion by w dt per
+    substep: after n substeps it is (0, 0, sin(n w dt / 2), cos(n w dt / 2))."""
+    h, dt, n = 1.0 / 64, 1e-3, 50
+    parts = scenes.particles(np.array([[0.5, 0.5, 0.5]]), h / 4, 1000.0, scenes.model_neo_hookean(1e5, 0.3))
+    cols = np.concatenate([scenes.cuboid((2.0, 2.0, 2.0), (0.1, 0.2, 0.3), linvel=(0.7, -0.2, 0.05)),
+                           scenes.cuboid((-2.0, 2.0, 2.0), (0.1, 0.2, 0.3), angvel=(0.0, 0.0, 3.0)),
+                           scenes.cuboid((2.0, -2.0, 2.0), (0.1, 0.2, 0.3))])
+    sc = scenes.make_scene(parts, h, dt, colliders=cols)
+    o = O.Oracle(sc, 32)
+    o.step(n)
+    t = np.array(cols["translation"][0], np.float32)
+    v = np.array(cols["linvel"][0], np.float32)
+    for _ in range(n):
+        t = (t + v * np.float32(sc.dt)).astype(np.float32)
+    assert np.array_equal(o.s.colliders["translation"][0], t)
+    ang = n * 3.0 * np.float64(np.float32(sc.dt))
+    assert np.allclose(o.s.colliders["rotation"][1], [0, 0, np.sin(ang / 2), np.cos(ang / 2)], atol=2e-6)
+    assert np.array_equal(o.s.colliders["translation"][2], cols["translation"][2])
+    assert np.allclose(o.s.colliders["rotation"][2], [0, 0, 0, 1])
