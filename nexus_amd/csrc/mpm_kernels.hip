@@ -511,14 +511,17 @@ __global__ void k_scatter(WorldDev w)
 struct Fixed {
     using word = unsigned long long;
     static constexpr int BITS = 62;
-    // round(v * scale) as a two's-complement 64-bit integer, exact for |v * scale| < 2^63:
-    // r is an integer-valued float; hi = floor(r / 2^32) and lo = r - hi 2^32 in [0, 2^32)
-    // are both exact (lo is a multiple of r's ulp and below 2^32, so it has at most 24
-    // significant bits), and r = hi 2^32 + lo.  Non-finite inputs give garbage that the
-    // caller's non-finite mask overrides.
+    // r = v * scale as a two's-complement 64-bit integer hi 2^32 + lo, for |r| < 2^63:
+    // hi = floor(r / 2^32) is exact; lo = r - hi 2^32 lies in [0, 2^32) and is rounded to
+    // float and truncated to an integer.  It is exact when hi >= 0 and |r| >= 2^24 or when
+    // r is a non-negative integer below 2^24; otherwise (a fractional part, or hi = -1 with
+    // lo close to 2^32) it is off by less than 2^8 units, against 2^62 / np units for the
+    // block's largest contribution.  The result depends only on v and the scale,
+    // so integer sums stay independent of the order of the adds.  Non-finite inputs give
+    // garbage that the caller's non-finite mask overrides.
     __device__ static word to(float v, float scale)
     {
-        const float r = __builtin_rintf(v * scale);
+        const float r = v * scale;
         const float hi = floorf(r * 0x1p-32f);
         const float lo = __builtin_fmaf(hi, -0x1p32f, r);
         const uint32_t uhi = (uint32_t)(int32_t)hi;
