@@ -265,6 +265,10 @@ nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *scene, uint32_t substeps);
  * 521-523; src_viewer/viewer.rs:675-697).  end synchronizes and returns the
  * per-step averages. */
 nxm_status nxm_scene_profile_begin(nxm_scene *scene);
+/* As nxm_scene_profile_begin, recording the stage events on one substep in `every` only
+ * (an event record between kernels costs GPU time of its own; sampling keeps the timed
+ * region's throughput intact).  No reference counterpart (GpuTimestamps times every pass). */
+nxm_status nxm_scene_profile_sampled(nxm_scene *scene, uint32_t every);
 nxm_status nxm_scene_profile_end(nxm_scene *scene, nxm_stage_times *avg, uint32_t *steps);
 
 /* ---- batching: many independent scenes stepped by one launch per stage ----

@@ -438,7 +438,7 @@ class World {
         lc.persistent_blocks = (int)cfg.grid_groups;
         lc.p2g_blocks = (int)cfg.p2g_groups;
         hipEvent_t *evs = times ? ev_ : nullptr;
-        if (!times && profiling_) {
+        if (!times && profiling_ && profile_seen_++ % profile_every_ == 0) {
             nxm_status st = ring_events(&evs);
             if (st != NXM_OK) return st;
         }
@@ -524,10 +524,14 @@ class World {
     }
 
     // ---- per-stage timing without synchronization (GpuTimestamps analog) ----
-    void profile_begin()
+    // Events are recorded on one substep in `every` (event records between the kernels
+    // cost GPU time of their own).
+    void profile_begin(uint32_t every = 1)
     {
         profiling_ = true;
         ring_used_ = 0;
+        profile_every_ = every ? every : 1u;
+        profile_seen_ = 0;
     }
 
     nxm_status profile_end(nxm_stage_times *avg, uint32_t *steps)
@@ -1130,6 +1134,7 @@ class World {
     bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false, profiling_ = false, clean_ = false;
     std::vector<hipEvent_t> ring_;
     size_t ring_used_ = 0;
+    uint32_t profile_every_ = 1, profile_seen_ = 0;
     int cur_ = 0;
     uint32_t p_in_ = 0, p_cap_ = 0, b_total_ = 0, h_total_ = 0;
     size_t ctr_zeroed_ = 0;
@@ -1647,6 +1652,14 @@ nxm_status nxm_scene_profile_begin(nxm_scene *sc)
     nxm_status st = ensure_own_world(sc);
     if (st != NXM_OK) return st;
     sc->world->profile_begin();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_profile_sampled(nxm_scene *sc, uint32_t every)
+{
+    nxm_status st = nxm_scene_profile_begin(sc);
+    if (st != NXM_OK) return st;
+    sc->world->profile_begin(every);
     return NXM_OK;
 }
 

@@ -44,7 +44,7 @@ EXPORTS = [
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
-    "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end",
+    "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
     "nxm_batch_profile_begin", "nxm_batch_profile_end",
@@ -154,6 +154,7 @@ def library() -> ct.CDLL:
     L.nxm_batch_set_graph.argtypes = [P, I]
     for f in ("nxm_scene_profile_begin", "nxm_batch_profile_begin"):
         getattr(L, f).argtypes = [P]
+    L.nxm_scene_profile_sampled.argtypes = [P, ct.c_uint32]
     for f in ("nxm_scene_profile_end", "nxm_batch_profile_end"):
         getattr(L, f).argtypes = [P, P, ct.POINTER(ct.c_uint32)]
     _LIB = L
@@ -351,9 +352,12 @@ class MpmState:
     def synchronize(self) -> None:
         _check(self.lib.nxm_scene_synchronize(self.handle))
 
-    def profile_begin(self) -> None:
-        """Start recording per-stage timestamps of every step (no synchronization)."""
-        _check(self.lib.nxm_scene_profile_begin(self.handle))
+    def profile_begin(self, every: int = 1) -> None:
+        """Start recording per-stage timestamps (no synchronization), on one step in `every`."""
+        if every == 1:
+            _check(self.lib.nxm_scene_profile_begin(self.handle))
+        else:
+            _check(self.lib.nxm_scene_profile_sampled(self.handle, int(every)))
 
     def profile_end(self):
         """Synchronize; return (per-step average StageTimes, number of steps)."""
