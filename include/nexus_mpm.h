@@ -230,6 +230,13 @@ nxm_status nxm_scene_set_graph(nxm_scene *scene, int enabled);
  * reference counterpart (the reference sizes one scene's buffers at creation,
  * src_mpm/pipeline.rs:107-160). */
 size_t nxm_scene_device_bytes(const nxm_scene *scene);
+/* Packs the particles of a scene that is not resident yet for its next upload (host work
+ * only, into pinned memory shared by the context; safe on another thread than the one
+ * stepping the worlds, as long as nothing else uses this scene meanwhile): adding it to a
+ * batch then only queues the copies.  Used by a server's ingest thread.  No reference
+ * counterpart (GpuParticles::append copies synchronously, src_mpm/solver/particle.rs:533).
+ * NXM_ERR_BUSY if the scene is resident. */
+nxm_status nxm_scene_stage_upload(nxm_scene *scene);
 /* GpuParticles::read_positions (src_mpm/solver/particle.rs:634-647); synchronizes. */
 nxm_status nxm_scene_read_positions(nxm_scene *scene, float *xyz, size_t n);
 /* Full particle state in original particle order; synchronizes. */

@@ -16,6 +16,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,72 @@ struct DevBuf {
 
 using namespace nxm;
 
+// Pinned host chunks shared by the threads of one context: a scene staged for upload on
+// one thread (nxm_scene_stage_upload) holds a chunk until the thread stepping its world
+// queues the copies; the chunk is reused once those copies have executed.
+struct PinnedPool {
+    struct Chunk {
+        char *p = nullptr;
+        size_t n = 0;
+        hipEvent_t ev = nullptr;
+        bool held = false;
+    };
+    std::mutex mu;
+    std::vector<Chunk> chunks;
+
+    // The smallest free chunk of at least `bytes` whose last copies have executed.
+    hipError_t take(size_t bytes, int *idx, char **out)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        int best = -1;
+        for (int k = 0; k < (int)chunks.size(); ++k) {
+            Chunk &c = chunks[k];
+            if (c.held || c.n < bytes) continue;
+            if (best >= 0 && chunks[best].n <= c.n) continue;
+            const hipError_t q = hipEventQuery(c.ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return q;
+            best = k;
+        }
+        if (best < 0) {
+            Chunk c;
+            c.n = std::max<size_t>(bytes, (size_t)1 << 20);
+            hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&c.p), c.n, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            e = hipEventCreate(&c.ev);
+            if (e != hipSuccess) { (void)hipHostFree(c.p); return e; }
+            chunks.push_back(c);
+            best = (int)chunks.size() - 1;
+        }
+        chunks[best].held = true;
+        *idx = best;
+        *out = chunks[best].p;
+        return hipSuccess;
+    }
+    // Gives the chunk back; with a stream, reusable once the work queued so far has run.
+    hipError_t give(int idx, hipStream_t st)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        Chunk &c = chunks[idx];
+        hipError_t e = st ? hipEventRecord(c.ev, st) : hipSuccess;
+        c.held = false;
+        return e;
+    }
+    char *ptr(int idx)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        return chunks[idx].p;
+    }
+    ~PinnedPool()
+    {
+        for (auto &c : chunks) {
+            (void)hipEventSynchronize(c.ev);
+            (void)hipEventDestroy(c.ev);
+            (void)hipHostFree(c.p);
+        }
+    }
+};
+
 struct nxm_ctx {
     int device = 0;
     int num_cus = 256;
@@ -85,6 +152,7 @@ struct nxm_ctx {
     // until the last of them is destroyed.
     std::atomic<int> refs{0};
     std::atomic<bool> closed{false};
+    PinnedPool pool;
 };
 
 static void ctx_release(nxm_ctx *ctx)
@@ -164,6 +232,10 @@ struct nxm_scene {
     // (dev_col_start); cols_host says the host copy is newer (set_colliders, or not resident).
     uint32_t dev_col_start = 0xFFFFFFFFu;
     bool cols_host = true;
+    // Particles packed ahead of the next upload (nxm_scene_stage_upload): a chunk of the
+    // context's pinned pool in arena-plane layout, flags without the world slot.
+    int staged_chunk = -1;
+    size_t staged_n = 0;
     std::vector<SceneRange> ranges;     // particle ranges in the world's input arena
     std::unique_ptr<World> own;         // standalone world
     nxm_batch *batch = nullptr;
@@ -267,6 +339,29 @@ class World {
         }
         const HostParticles &hp = sc->hp;
         constexpr int NUP = NUM_PLANES;
+        if (sc->staged_chunk >= 0) {
+            if (first == 0 && count == sc->staged_n) {
+                // Packed by nxm_scene_stage_upload: add the world slot to the flags and queue.
+                const int k = sc->staged_chunk;
+                float *f = reinterpret_cast<float *>(sc->ctx->pool.ptr(k));
+                uint32_t *uf = reinterpret_cast<uint32_t *>(f + (size_t)PFLAGS * count);
+                const uint32_t sbits = sc->slot << 16;
+                for (size_t i = 0; i < count; ++i) uf[i] |= sbits;
+                Arena a = arena(cur_);
+                const uint32_t at = p_in_;
+                for (int p = 0; p < NUP; ++p)
+                    HIP_TRY(hipMemcpyAsync(a.plane(p) + at, f + (size_t)p * count, count * 4, hipMemcpyHostToDevice, stream_));
+                sc->staged_chunk = -1;
+                HIP_TRY(sc->ctx->pool.give(k, stream_));
+                sc->ranges.push_back(SceneRange{at, (uint32_t)count});
+                p_in_ += (uint32_t)count;
+                pin_dirty_ = true;
+                return NXM_OK;
+            }
+            const int k = sc->staged_chunk;   // stale: the particles changed since
+            sc->staged_chunk = -1;
+            HIP_TRY(sc->ctx->pool.give(k, nullptr));
+        }
         char *stage = nullptr;
         size_t ci = 0;
         nxm_status st = pinned(NUP * count * 4, &stage, &ci);
@@ -1227,8 +1322,17 @@ static nxm_status ensure_own_world(nxm_scene *sc)
     return NXM_OK;
 }
 
+// Forgets particles packed by nxm_scene_stage_upload (the host particles are changing).
+static void drop_staged(nxm_scene *sc)
+{
+    if (sc->staged_chunk < 0) return;
+    (void)sc->ctx->pool.give(sc->staged_chunk, nullptr);
+    sc->staged_chunk = -1;
+}
+
 static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const nxm_particle *aos, size_t n)
 {
+    drop_staged(sc);
     HostParticles &hp = sc->hp;
     size_t base = hp.n;
     // state of already-resident particles is kept on the device; only the new
@@ -1366,8 +1470,48 @@ void nxm_scene_destroy(nxm_scene *sc)
     }
     sc->own.reset();
     nxm_ctx *ctx = sc->ctx;
+    if (sc->staged_chunk >= 0) (void)ctx->pool.give(sc->staged_chunk, nullptr);
     delete sc;
     ctx_release(ctx);
+}
+
+nxm_status nxm_scene_stage_upload(nxm_scene *sc)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (sc->world) return fail(NXM_ERR_BUSY, "the scene is resident: its particles are uploaded already");
+    const HostParticles &hp = sc->hp;
+    const size_t n = hp.n;
+    if (sc->staged_chunk >= 0) {
+        (void)sc->ctx->pool.give(sc->staged_chunk, nullptr);
+        sc->staged_chunk = -1;
+    }
+    if (n == 0) return NXM_OK;
+    int k = -1;
+    char *buf = nullptr;
+    HIP_TRY(sc->ctx->pool.take((size_t)NUM_PLANES * n * 4, &k, &buf));
+    float *f = reinterpret_cast<float *>(buf);
+    auto put = [&](int p, const float *src, int stride, int comp) {
+        float *dst = f + (size_t)p * n;
+        for (size_t i = 0; i < n; ++i) dst[i] = src[i * stride + comp];
+    };
+    for (int c = 0; c < 3; ++c) put(PX + c, hp.x.data(), 3, c);
+    for (int c = 0; c < 3; ++c) put(PVX + c, hp.v.data(), 3, c);
+    for (int c = 0; c < 9; ++c) put(PC0 + c, hp.C.data(), 9, c);
+    for (int c = 0; c < 9; ++c) put(PF0 + c, hp.F.data(), 9, c);
+    put(PMASS, hp.mass.data(), 1, 0);
+    put(PVOL0, hp.vol0.data(), 1, 0);
+    for (int c = 0; c < 3; ++c) put(PS0 + c, hp.pstate.data(), 3, c);
+    std::memcpy(f + (size_t)PAFF * n, hp.aff.data(), n * 4);
+    for (int c = 0; c < 4; ++c) put(PNX + c, hp.cdf.data(), 4, c);
+    uint32_t *uf = reinterpret_cast<uint32_t *>(f + (size_t)PFLAGS * n);
+    uint32_t *ui = reinterpret_cast<uint32_t *>(f + (size_t)PID * n);
+    for (size_t i = 0; i < n; ++i) {
+        uf[i] = make_flags(hp.enabled[i] != 0, hp.fixed[i] != 0, hp.cls[i], 0u);
+        ui[i] = (uint32_t)i;
+    }
+    sc->staged_chunk = k;
+    sc->staged_n = n;
+    return NXM_OK;
 }
 
 nxm_status nxm_scene_set_simulation_params(nxm_scene *sc, const nxm_sim_params *p)
@@ -1519,6 +1663,7 @@ nxm_status nxm_scene_swap_remove(nxm_scene *sc, const uint32_t *ids, size_t n, u
         nxm_status st = w->remove(sc);
         if (st != NXM_OK) return st;
     }
+    drop_staged(sc);
     std::vector<uint32_t> targets(ids, ids + n);
     std::sort(targets.begin(), targets.end(), std::greater<uint32_t>());
     targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
@@ -1596,6 +1741,7 @@ nxm_status nxm_scene_set_cpic(nxm_scene *sc, int enabled)
         if (st != NXM_OK) return st;
     }
     sc->cpic = on;
+    drop_staged(sc);
     std::fill(sc->hp.aff.begin(), sc->hp.aff.end(), 0u);
     std::fill(sc->hp.cdf.begin(), sc->hp.cdf.end(), 0.0f);
     if (w) return w->add(sc);

@@ -44,6 +44,7 @@ EXPORTS = [
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
+    "nxm_scene_stage_upload",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
@@ -137,6 +138,7 @@ def library() -> ct.CDLL:
     L.nxm_scene_synchronize.argtypes = [P]
     L.nxm_scene_set_debug_grid.argtypes = [P, I]
     L.nxm_scene_set_cpic.argtypes = [P, I]
+    L.nxm_scene_stage_upload.argtypes = [P]
     L.nxm_scene_read_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_cdf.argtypes = [P, P, P, P, ct.c_size_t]
     L.nxm_scene_read_grid_cpic.argtypes = [P, P, P, P, P, P]
@@ -241,6 +243,10 @@ class MpmState:
         out = np.zeros(n, dtype=_scenes.CUBOID_DTYPE)
         _check(self.lib.nxm_scene_read_colliders(self.handle, _ptr(out) if n else None, n))
         return out
+
+    def stage_upload(self) -> None:
+        """Packs the (not yet resident) particles for the next upload, on this thread."""
+        _check(self.lib.nxm_scene_stage_upload(self.handle))
 
     def set_cpic(self, enabled: bool) -> None:
         """MpmState::use_cpic: compatible particle-in-cell transfers against the colliders."""

@@ -24,10 +24,11 @@ from . import mpm
 from . import scenes as _scenes
 
 
-@dataclass
+@dataclass(eq=False)
 class Request:
     """One request.  ``arrival`` is in seconds (wall clock) or in batch substeps (step
-    clock).  ``scene`` is a Scene or any object with ``build() -> Scene``."""
+    clock).  ``scene`` is a Scene or any object with ``build() -> Scene``.  Requests compare
+    by identity."""
 
     arrival: float
     scene: object
@@ -111,10 +112,11 @@ class Server:
     continuous batching at substep granularity."""
 
     def __init__(self, pipeline: mpm.MpmPipeline, hbm_budget_bytes: float = 200e9,
-                 max_active_particles: int | None = None):
+                 max_active_particles: int | None = None, ingest_workers: int = 4):
         self.pipeline = pipeline
         self.budget = float(hbm_budget_bytes)
         self.max_particles = max_active_particles
+        self.ingest_workers = max(1, int(ingest_workers))
         self.batch = mpm.Batch(pipeline)
 
     def close(self):
@@ -124,12 +126,13 @@ class Server:
         sc = r.scene if isinstance(r.scene, _scenes.Scene) else r.scene.build()
         r.n = sc.n
         r.state = mpm.MpmState.from_scene(self.pipeline, sc)   # host-side until admitted
+        r.state.stage_upload()          # packed here, so that admission only queues copies
         r.bytes = r.state.device_bytes()
 
     def run(self, requests: list[Request], clock: str = "wall") -> ServeReport:
         """Serves every request to completion.  clock="wall": arrivals in seconds from
-        the start of the call; a separate ingest thread builds each arrived request's
-        host-side scene while the GPU steps the batch.  clock="steps": arrivals in batch
+        the start of the call; ingest threads build each arrived request's host-side scene
+        (and pack it for upload) while the GPU steps the batch.  clock="steps": arrivals in batch
         substeps, requests prepared in the serving loop (reproducible)."""
         if clock not in ("wall", "steps"):
             raise ValueError("clock must be 'wall' or 'steps'")
@@ -154,9 +157,15 @@ class Server:
         ingest_error: list[BaseException] = []
         stop = threading.Event()
 
-        def ingest():
+        # Ingest workers prepare arrived requests in parallel (the native calls release the
+        # interpreter lock); requests become ready in arrival order, so admission stays FIFO.
+        prepared: set[int] = set()
+        next_ready = [0]
+
+        def ingest(w: int):
             try:
-                for r in order:
+                for k in range(w, len(order), self.ingest_workers):
+                    r = order[k]
                     while not stop.is_set():
                         delay = r.arrival - (time.perf_counter() - t0)
                         if delay <= 0:
@@ -166,18 +175,24 @@ class Server:
                         return
                     self._prepare(r)
                     with cv:
-                        ready.append(r)
+                        prepared.add(k)
+                        while next_ready[0] in prepared:
+                            prepared.discard(next_ready[0])
+                            ready.append(order[next_ready[0]])
+                            next_ready[0] += 1
                         cv.notify()
             except BaseException as e:  # surfaced by the serving loop
                 with cv:
                     ingest_error.append(e)
                     cv.notify()
 
-        worker = None
+        workers = []
         if wall:
             pending.clear()
-            worker = threading.Thread(target=ingest, name="nxm-ingest", daemon=True)
-            worker.start()
+            for w in range(min(self.ingest_workers, max(len(order), 1))):
+                t = threading.Thread(target=ingest, args=(w,), name=f"nxm-ingest-{w}", daemon=True)
+                t.start()
+                workers.append(t)
         remaining = len(order)
         try:
             while remaining:
@@ -257,10 +272,10 @@ class Server:
                         rep.wait_s.append(r.admitted - r.arrival)
         finally:
             stop.set()
-            if worker is not None:
-                worker.join()
+            for t in workers:
+                t.join()
             # On an error, release what was prepared or admitted but not finished.
-            for r in list(active) + list(queue) + list(ready):
+            for r in order:
                 if r.state is not None:
                     if r in active:
                         self.batch.discard(r.state)

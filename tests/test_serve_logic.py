@@ -21,6 +21,9 @@ class FakeState:
     def device_bytes(self):
         return 1000 * self.n
 
+    def stage_upload(self):
+        pass
+
     def read_positions(self):
         return np.full((self.n, 3), float(self.steps), np.float32)
 
