@@ -58,3 +58,28 @@ def test_served_under_tight_budget(pipeline):
             pipeline.step(st)
         assert np.array_equal(r.result, st.read_positions()), r.rid
         st.close()
+
+
+def test_served_on_the_wall_clock_equals_alone(pipeline):
+    """Wall-clock arrivals: four ingest threads build and pack the scenes in parallel
+    (nxm_scene_stage_upload) while the batch steps; each result still equals the scene
+    stepped alone, bit for bit, and the requests are admitted in arrival order."""
+    mix = _mix()
+    alone = []
+    for _, sc, horizon in mix:
+        st = mpm.MpmState.from_scene(pipeline, sc)
+        for _ in range(horizon):
+            pipeline.step(st)
+        alone.append(st.read_particles())
+        st.close()
+    reqs = [serve.Request(0.02 * a, sc, h, rid=i, keep="particles") for i, (a, sc, h) in enumerate(mix)]
+    srv = serve.Server(pipeline, ingest_workers=4)
+    rep = srv.run(reqs, clock="wall")
+    srv.close()
+    assert rep.completed == len(mix) and rep.rejected == 0
+    admitted = [r.admitted for r in sorted(reqs, key=lambda r: r.arrival)]
+    assert admitted == sorted(admitted)
+    for r, ref in zip(reqs, alone):
+        assert r.steps == r.horizon
+        for f in ("x", "v", "C", "F", "enabled"):
+            assert np.array_equal(r.result[f], ref[f]), (r.rid, f)
