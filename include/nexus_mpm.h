@@ -119,10 +119,20 @@ typedef struct {
     uint32_t *model_data;/* n*24 */
 } nxm_particles_soa;
 
-/* A fixed or kinematic cuboid collider coupled one way (rapier collider + body +
+/* A cuboid collider and the body it is attached to (rapier collider + body +
  * BoundaryCondition, src/state.rs:418-450; projection by parry3d Cuboid through
- * src_rbd_shaders/shapes/shape.rs:378-384).  The body's centre of mass is its
- * translation. */
+ * src_rbd_shaders/shapes/shape.rs:378-384).  The collider's pose is the body's pose.
+ *
+ * inv_mass and local_com are the body's mass properties as MPM holds them
+ * (GpuBodySet::from_rapier, src_rbd/dynamics/body.rs:153-170): zero for a body coupled
+ * one way and for fixed or kinematic bodies; a dynamic body coupled two ways carries
+ * its world inverse mass per axis (zero along locked axes) and its centre of mass in
+ * the body frame.  MPM integrates the body each substep (rigid_impulses.rs:66-140):
+ * the pose advances about the world centre of mass pose * local_com, then gravity is
+ * added to the linear velocity along the axes where inv_mass is nonzero.  The
+ * particles' impulses on the body are compiled out in the reference
+ * (TWO_WAYS_COUPLING_ENABLED = false, src_mpm_shaders/lib.rs:38), so they are zero here
+ * too and the body's velocity is never capped. */
 typedef struct {
     float half_extents[3];
     float rotation[4];   /* unit quaternion x, y, z, w */
@@ -131,6 +141,8 @@ typedef struct {
     float angvel[3];
     uint32_t bc_type;
     float bc_friction;
+    float inv_mass[3];
+    float local_com[3];
 } nxm_cuboid;
 
 /* Per-stage device times of the last step, milliseconds (the reference's

@@ -1561,8 +1561,12 @@ nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
     if (!sc || (!c && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
     if (n > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
     sc->cols.resize(n);
-    static_assert(sizeof(ColliderDev) == sizeof(nxm_cuboid), "collider layout");
-    if (n) std::memcpy(sc->cols.data(), c, n * sizeof(nxm_cuboid));
+    static_assert(offsetof(ColliderDev, shape) == sizeof(nxm_cuboid), "nxm_cuboid is a prefix of ColliderDev");
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(&sc->cols[i], &c[i], sizeof(nxm_cuboid));
+        sc->cols[i].shape = SHAPE_CUBOID;
+        sc->cols[i].pad = 0;
+    }
     sc->cols_host = true;
     if (sc->world) sc->world->mark_dirty();
     return NXM_OK;
@@ -1578,7 +1582,7 @@ nxm_status nxm_scene_read_colliders(nxm_scene *sc, nxm_cuboid *out, size_t n)
         nxm_status st = sc->world->pull_colliders(sc);
         if (st != NXM_OK) return st;
     }
-    if (!sc->cols.empty()) std::memcpy(out, sc->cols.data(), sc->cols.size() * sizeof(nxm_cuboid));
+    for (size_t i = 0; i < sc->cols.size(); ++i) std::memcpy(&out[i], &sc->cols[i], sizeof(nxm_cuboid));
     return latched(sc, NXM_OK);
 }
 

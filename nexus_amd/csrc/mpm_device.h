@@ -91,7 +91,10 @@ struct ClassDev {
     uint32_t pad;
 };
 
-// nxm_cuboid layout.
+// A collider on its body: the nxm_cuboid fields, then the shape (a triangle mesh's
+// geometry lives in the world's mesh tables; its half extents are zero).
+constexpr uint32_t SHAPE_CUBOID = 0u;
+constexpr uint32_t SHAPE_TRIMESH = 1u;
 struct ColliderDev {
     float he[3];
     float q[4];
@@ -100,6 +103,10 @@ struct ColliderDev {
     float angvel[3];
     uint32_t bc_type;
     float bc_friction;
+    float inv_mass[3];
+    float lcom[3];      // centre of mass in the body frame
+    uint32_t shape;
+    uint32_t pad;
 };
 
 // Block virtual id marker of a scene's overflow block (particles that could not be
@@ -437,6 +444,13 @@ __device__ inline f3 quat_rotate(float qx, float qy, float qz, float qw, f3 v)
     return v * (qw * qw - b2) + b * (dot(v, b) * 2.0f) + cross(b, v) * (qw * 2.0f);
 }
 
+// World centre of mass of the collider's body: pose * local centre of mass
+// (LocalMassProperties::to_world, rigid_impulses.rs:163-185).
+__device__ inline f3 collider_com(const ColliderDev &c)
+{
+    return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], mk3(c.lcom[0], c.lcom[1], c.lcom[2])) + mk3(c.t[0], c.t[1], c.t[2]);
+}
+
 // Closest point on the boundary of a posed cuboid: outside, the clamp to the box;
 // inside, the nearest face (first axis wins ties; the min face wins a same-axis tie),
 // as parry3d's Cuboid::project_point with solid = false
@@ -520,7 +534,7 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
     }
     if (best != MAX_FLT_REF) {
         const ColliderDev &c = cols[best_id];
-        f3 com = mk3(c.t[0], c.t[1], c.t[2]);
+        f3 com = collider_com(c);
         f3 bv = mk3(c.linvel[0], c.linvel[1], c.linvel[2]) + cross(mk3(c.angvel[0], c.angvel[1], c.angvel[2]), pt - com);
         f3 delta = vel - bv;
         float nv = dot(delta, normal);
@@ -567,11 +581,11 @@ __device__ inline void node_cdf(const ColliderDev *cols, uint32_t ncol, uint32_t
     }
 }
 
-// Velocity::velocity_at_point (src_rbd_shaders/dynamics/body.rs:239-250), the cuboid's
-// pose translation being its centre of mass.
+// Velocity::velocity_at_point (src_rbd_shaders/dynamics/body.rs:239-250) about the
+// body's world centre of mass.
 __device__ inline f3 body_point_velocity(const ColliderDev &c, f3 pt)
 {
-    const f3 com = mk3(c.t[0], c.t[1], c.t[2]);
+    const f3 com = collider_com(c);
     return mk3(c.linvel[0], c.linvel[1], c.linvel[2]) + cross(mk3(c.angvel[0], c.angvel[1], c.angvel[2]), pt - com);
 }
 

@@ -1186,17 +1186,18 @@ __device__ inline void clear_hash(const WorldDev &w)
 
 // Body integration at the end of a substep (the reference's "Integrate bodies" pass,
 // src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in rigid_impulses.rs:66-140):
-// MPM integrates its own copy of every coupled body.  The boundary's cuboids carry no
-// mass (fixed or velocity-driven bodies: no impulse response, no gravity; two-way
-// coupling is compiled out in the reference, src_mpm_shaders/lib.rs:38), so each pose
-// advances with its velocity about its centre, the translation: t += v dt and
-// q = normalize(exp(w dt) q) (Velocity::integrate, src_rbd_shaders/dynamics/body.rs:
-// 253-268, with glam's from_scaled_axis, quaternion product and normalize).
-__device__ inline void integrate_collider(ColliderDev &c, float dt)
+// MPM integrates its own copy of every coupled body (gpu_rigid_impulses_update,
+// rigid_impulses.rs:66-140).  The particles' impulses on the bodies are compiled out in
+// the reference (src_mpm_shaders/lib.rs:38), so the velocity is not changed by them and
+// never capped; the pose advances about the world centre of mass c = pose * local_com:
+// t' = c + exp(w dt) (t - c) + v dt, q' = normalize(exp(w dt) q) (Velocity::integrate,
+// src_rbd_shaders/dynamics/body.rs:253-268, with glam's from_scaled_axis, quaternion
+// product and normalize); then gravity is added to v along the axes where the body's
+// inverse mass is nonzero (a dynamic body coupled two ways; zero for the others).
+__device__ inline void integrate_collider(ColliderDev &c, float dt, f3 g)
 {
-    c.t[0] = c.t[0] + c.linvel[0] * dt;
-    c.t[1] = c.t[1] + c.linvel[1] * dt;
-    c.t[2] = c.t[2] + c.linvel[2] * dt;
+    const f3 t0 = mk3(c.t[0], c.t[1], c.t[2]);
+    const f3 com = collider_com(c);
     const f3 sa = mk3(c.angvel[0] * dt, c.angvel[1] * dt, c.angvel[2] * dt);
     const float len = sqrtf(sa.x * sa.x + sa.y * sa.y + sa.z * sa.z);
     float dx = 0.0f, dy = 0.0f, dz = 0.0f, dw = 1.0f;
@@ -1208,6 +1209,10 @@ __device__ inline void integrate_collider(ColliderDev &c, float dt)
         dz = sa.z / len * sn;
         dw = cs;
     }
+    const f3 arm = quat_rotate(dx, dy, dz, dw, t0 - com);
+    c.t[0] = (com.x + arm.x) + c.linvel[0] * dt;
+    c.t[1] = (com.y + arm.y) + c.linvel[1] * dt;
+    c.t[2] = (com.z + arm.z) + c.linvel[2] * dt;
     const float qx = c.q[0], qy = c.q[1], qz = c.q[2], qw = c.q[3];
     const float nx = dw * qx + dx * qw + dy * qz - dz * qy;
     const float ny = dw * qy - dx * qz + dy * qw + dz * qx;
@@ -1218,6 +1223,9 @@ __device__ inline void integrate_collider(ColliderDev &c, float dt)
     c.q[1] = ny * r;
     c.q[2] = nz * r;
     c.q[3] = nw * r;
+    c.linvel[0] = c.linvel[0] + (c.inv_mass[0] != 0.0f ? g.x : 0.0f) * dt;
+    c.linvel[1] = c.linvel[1] + (c.inv_mass[1] != 0.0f ? g.y : 0.0f) * dt;
+    c.linvel[2] = c.linvel[2] + (c.inv_mass[2] != 0.0f ? g.z : 0.0f) * dt;
 }
 
 // Every collider of every active scene, 16 slots per scene, grid-stride.  Runs inside the
@@ -1229,7 +1237,7 @@ __device__ inline void integrate_colliders_part(const WorldDev &w)
         const SceneDev sc = w.scenes[t >> 4];
         const uint32_t c = t & 15u;
         if (!sc.active || c >= sc.col_count) continue;
-        integrate_collider(w.cols[sc.col_start + c], sc.dt);
+        integrate_collider(w.cols[sc.col_start + c], sc.dt, mk3(sc.gx, sc.gy, sc.gz));
     }
 }
 

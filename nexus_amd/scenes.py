@@ -40,7 +40,8 @@ BC_NON_REFLECTING = 3
 SEED_BASE = 0x4E584D00
 
 # nxm_cuboid layout: half_extents[3], rotation[4] (x, y, z, w), translation[3],
-# linvel[3], angvel[3], bc_type (u32), bc_friction (f32).
+# linvel[3], angvel[3], bc_type (u32), bc_friction (f32), inv_mass[3] (zero unless the
+# body is dynamic and coupled two ways), local_com[3] (centre of mass, body frame).
 CUBOID_DTYPE = np.dtype(
     [
         ("half_extents", "<f4", 3),
@@ -50,6 +51,8 @@ CUBOID_DTYPE = np.dtype(
         ("angvel", "<f4", 3),
         ("bc_type", "<u4"),
         ("bc_friction", "<f4"),
+        ("inv_mass", "<f4", 3),
+        ("local_com", "<f4", 3),
     ]
 )
 
@@ -177,7 +180,10 @@ def concat(parts: list[dict]) -> dict:
 
 
 def cuboid(center, half_extents, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0,
-           linvel=(0, 0, 0), angvel=(0, 0, 0)):
+           linvel=(0, 0, 0), angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
+    """One collider record.  inv_mass / local_com describe a dynamic body coupled two
+    ways (gravity acts on it along the axes with a nonzero inverse mass, and it turns
+    about its centre of mass); the defaults are a body coupled one way."""
     c = np.zeros(1, dtype=CUBOID_DTYPE)
     c["half_extents"] = half_extents
     c["rotation"] = rotation
@@ -186,6 +192,8 @@ def cuboid(center, half_extents, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, fri
     c["angvel"] = angvel
     c["bc_type"] = bc
     c["bc_friction"] = friction
+    c["inv_mass"] = inv_mass
+    c["local_com"] = local_com
     return c
 
 

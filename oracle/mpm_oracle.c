@@ -528,6 +528,15 @@ static v3 quat_rotate(REAL qx, REAL qy, REAL qz, REAL qw, v3 v)
                 vmul(vcross(b, v), qw * K(2.0)));
 }
 
+/* World centre of mass of a collider's body: pose * local_com
+ * (LocalMassProperties::to_world, src_mpm_shaders/solver/rigid_impulses.rs:163-185). */
+static v3 collider_com(const orc_cuboid *c)
+{
+    v3 lc = V3(c->local_com[0], c->local_com[1], c->local_com[2]);
+    return vadd(quat_rotate(c->rotation[0], c->rotation[1], c->rotation[2], c->rotation[3], lc),
+                V3(c->translation[0], c->translation[1], c->translation[2]));
+}
+
 /* parry3d Cuboid::project_point(pose, pt, solid = false), reached through
  * Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-384). */
 static v3 project_cuboid(const orc_cuboid *c, v3 pt, int *inside)
@@ -610,7 +619,7 @@ static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt,
     }
     if (best != MAX_FLT_REF) {
         const orc_cuboid *c = &cols[best_id];
-        v3 com = V3(c->translation[0], c->translation[1], c->translation[2]);
+        v3 com = collider_com(c);
         v3 lin = V3(c->linvel[0], c->linvel[1], c->linvel[2]);
         v3 ang = V3(c->angvel[0], c->angvel[1], c->angvel[2]);
         v3 bv = vadd(lin, vcross(ang, vsub(pt, com)));
@@ -777,13 +786,17 @@ static void particle_cdf(v3 refp, REAL h, const REAL *ndist[27], const uint32_t 
     }
 }
 
-/* Velocity::integrate of a massless cuboid centred at its translation:
- * t' = t + v dt, q' = normalize(from_scaled_axis(w dt) q) (glam's from_scaled_axis:
- * identity for a zero vector, else (axis sin(a/2), cos(a/2)); Hamilton product;
- * normalize = q / |q|).  The pose is stored in f32, as the reference's. */
-static void integrate_cuboid(orc_cuboid *c, float dt)
+/* gpu_rigid_impulses_update (rigid_impulses.rs:66-140) with the particles' impulses
+ * compiled out (zero, so no velocity cap): Velocity::integrate about the world centre
+ * of mass c (src_rbd_shaders/dynamics/body.rs:253-268): t' = c + exp(w dt) (t - c) + v dt,
+ * q' = normalize(from_scaled_axis(w dt) q) (glam's from_scaled_axis: identity for a zero
+ * vector, else (axis sin(a/2), cos(a/2)); Hamilton product; normalize = q / |q|); then
+ * v += g dt along the axes with a nonzero inverse mass.  The pose and velocity are
+ * stored in f32, as the reference's. */
+static void integrate_cuboid(orc_cuboid *c, float dt, const float *g)
 {
-    for (int k = 0; k < 3; ++k) c->translation[k] = (float)((REAL)c->translation[k] + (REAL)c->linvel[k] * (REAL)dt);
+    v3 t0 = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 com = collider_com(c);
     REAL sx = (REAL)c->angvel[0] * (REAL)dt, sy = (REAL)c->angvel[1] * (REAL)dt, sz = (REAL)c->angvel[2] * (REAL)dt;
     REAL len = RSQRT(sx * sx + sy * sy + sz * sz);
     REAL dx = 0, dy = 0, dz = 0, dw = 1;
@@ -795,6 +808,11 @@ static void integrate_cuboid(orc_cuboid *c, float dt)
         dz = sz / len * sn;
         dw = cs;
     }
+    v3 arm = quat_rotate(dx, dy, dz, dw, vsub(t0, com));
+    v3 nt = vadd(vadd(com, arm), V3((REAL)c->linvel[0] * (REAL)dt, (REAL)c->linvel[1] * (REAL)dt, (REAL)c->linvel[2] * (REAL)dt));
+    c->translation[0] = (float)nt.x;
+    c->translation[1] = (float)nt.y;
+    c->translation[2] = (float)nt.z;
     REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
     REAL nx = dw * qx + dx * qw + dy * qz - dz * qy;
     REAL ny = dw * qy - dx * qz + dy * qw + dz * qx;
@@ -805,11 +823,13 @@ static void integrate_cuboid(orc_cuboid *c, float dt)
     c->rotation[1] = (float)(ny * r);
     c->rotation[2] = (float)(nz * r);
     c->rotation[3] = (float)(nw * r);
+    for (int k = 0; k < 3; ++k)
+        c->linvel[k] = (float)((REAL)c->linvel[k] + (c->inv_mass[k] != 0.0f ? (REAL)g[k] : K(0.0)) * (REAL)dt);
 }
 
 static v3 body_point_velocity(const orc_cuboid *c, v3 pt)
 {
-    v3 com = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 com = collider_com(c);
     v3 lin = V3(c->linvel[0], c->linvel[1], c->linvel[2]);
     v3 ang = V3(c->angvel[0], c->angvel[1], c->angvel[2]);
     return vadd(lin, vcross(ang, vsub(pt, com)));
@@ -1326,7 +1346,7 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
     /* --- body integration (src_mpm/pipeline.rs:510-520, rigid_impulses.rs:66-140): the
      *     cuboids are massless coupled bodies, so each pose advances with its velocity
      *     about its centre (Velocity::integrate, src_rbd_shaders/dynamics/body.rs:253-268) --- */
-    for (int i = 0; i < ncol; ++i) integrate_cuboid(&cols[i], (float)sim[3]);
+    for (int i = 0; i < ncol; ++i) integrate_cuboid(&cols[i], (float)sim[3], sim);
 
     /* --- keep inspection data --- */
     ws_reserve(ws, nb, nsorted);

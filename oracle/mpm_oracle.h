@@ -30,16 +30,21 @@
 extern "C" {
 #endif
 
-/* A fixed cuboid collider with a one-way boundary condition.  Same layout as
- * nxm_cuboid in include/nexus_mpm.h. */
+/* A cuboid collider on its body, with a one-way boundary condition.  Same layout as
+ * nxm_cuboid in include/nexus_mpm.h.  inv_mass / local_com: the body's mass
+ * properties as MPM holds them (zero unless the body is dynamic and coupled two ways):
+ * the world centre of mass is pose * local_com, and gravity acts on the body along the
+ * axes where inv_mass is nonzero. */
 typedef struct {
     float half_extents[3];
     float rotation[4];      /* unit quaternion x, y, z, w */
-    float translation[3];   /* also the body's world centre of mass */
+    float translation[3];
     float linvel[3];
     float angvel[3];
     uint32_t bc_type;       /* 0 stick, 1 slip, 2 separate, 3 non-reflecting */
     float bc_friction;
+    float inv_mass[3];
+    float local_com[3];
 } orc_cuboid;
 
 typedef struct orc_ws orc_ws;

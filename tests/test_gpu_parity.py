@@ -295,6 +295,26 @@ def test_moving_colliders_integrate_like_the_oracle(pipeline):
     st.close()
 
 
+def test_dynamic_body_falls_through_the_sand_like_the_oracle(pipeline):
+    """A dynamic body coupled two ways (nonzero inverse mass, off-centre centre of mass,
+    turning): MPM adds gravity to its velocity every substep while the sand's impulses
+    on it are compiled out (src_mpm_shaders/lib.rs:38), so it falls through the sand,
+    which it pushes aside.  Pose, velocity and sand after 30 substeps against the
+    oracle."""
+    sc = small_sand(6000, seed=73)
+    body = scenes.cuboid((0.45, 0.3, 0.45), (0.05, 0.05, 0.08), bc=scenes.BC_SLIP, linvel=(0.0, -1.0, 0.0),
+                         angvel=(0.0, 2.0, 0.0), inv_mass=(0.5, 0.5, 0.5), local_com=(0.02, 0.0, 0.0))
+    sc.colliders = np.concatenate([sc.colliders[:3], body])
+    st = run_gpu(pipeline, sc, 30)
+    cols = st.read_colliders()
+    o32, o64 = oracle_pair(sc, 30)
+    for f in ("translation", "rotation", "linvel"):
+        assert np.allclose(cols[f], o64.s.colliders[f], rtol=0, atol=2e-6), f
+    assert cols["linvel"][-1][1] < -1.0                 # gravity acted on the body
+    check_state(st.read_particles(), o32, o64)
+    st.close()
+
+
 def test_moving_colliders_survive_batch_rebuilds(pipeline):
     """The integrated poses live on the device: a scene that joins a batch after stepping
     alone, and whose batch is rebuilt when another scene joins, continues from its moved

@@ -195,3 +195,48 @@ def test_collider_pose_integration():
     assert np.allclose(o.s.colliders["rotation"][1], [0, 0, np.sin(ang / 2), np.cos(ang / 2)], atol=2e-6)
     assert np.array_equal(o.s.colliders["translation"][2], cols["translation"][2])
     assert np.allclose(o.s.colliders["rotation"][2], [0, 0, 0, 1])
+
+
+def test_dynamic_body_integration():
+    """A dynamic body coupled two ways: the pose advances with the velocity from before
+    the substep's gravity, then gravity is added along the axes with a nonzero inverse
+    mass (rigid_impulses.rs:66-140, impulses compiled out).  Bit-exact against a float32
+    loop in the FP32 mirror.  A body turning about an off-centre centre of mass keeps
+    that point fixed and its translation on a circle around it."""
+    h, dt, n = 1.0 / 64, 1e-3, 40
+    g = (0.5, -9.81, 0.25)
+    parts = scenes.particles(np.array([[0.5, 0.5, 0.5]]), h / 4, 1000.0, scenes.model_neo_hookean(1e5, 0.3))
+    lc = (0.1, 0.0, 0.0)
+    cols = np.concatenate([scenes.cuboid((2.0, 2.0, 2.0), (0.1, 0.2, 0.3), linvel=(0.7, 0.2, 0.0),
+                                         inv_mass=(0.5, 0.5, 0.5)),
+                           scenes.cuboid((-2.0, 2.0, 2.0), (0.1, 0.2, 0.3), inv_mass=(0.0, 2.0, 0.0)),
+                           scenes.cuboid((2.0, -2.0, 2.0), (0.1, 0.2, 0.3), angvel=(0.0, 0.0, 3.0),
+                                         local_com=lc)])
+    sc = scenes.make_scene(parts, h, dt, colliders=cols, gravity=g)
+    o = O.Oracle(sc, 32)
+    o.step(n)
+    f32 = np.float32
+    dtf, gf = f32(sc.dt), np.array(g, np.float32)
+    for i, mask in ((0, np.array([1, 1, 1], np.float32)), (1, np.array([0, 1, 0], np.float32))):
+        t = np.array(cols["translation"][i], np.float32)
+        v = np.array(cols["linvel"][i], np.float32)
+        for _ in range(n):
+            t = (t + v * dtf).astype(np.float32)
+            v = (v + (gf * mask) * dtf).astype(np.float32)
+        assert np.array_equal(o.s.colliders["translation"][i], t), i
+        assert np.array_equal(o.s.colliders["linvel"][i], v), i
+    # the one-way body keeps its velocity and turns about its centre of mass
+    assert np.array_equal(o.s.colliders["linvel"][2], cols["linvel"][2])
+    q = o.s.colliders["rotation"][2].astype(np.float64)
+    ang = n * 3.0 * np.float64(dtf)
+    assert np.allclose(q, [0, 0, np.sin(ang / 2), np.cos(ang / 2)], atol=2e-6)
+    com0 = cols["translation"][2].astype(np.float64) + np.array(lc)
+    t = o.s.colliders["translation"][2].astype(np.float64)
+    # world com = t + R(q) lc stays where it started
+    x, y, z, w = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    assert np.allclose(t + R @ np.array(lc), com0, atol=1e-5)
+    assert np.isclose(np.linalg.norm(t - com0), 0.1, atol=1e-5)
+    assert not np.allclose(t, cols["translation"][2])
