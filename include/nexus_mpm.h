@@ -200,10 +200,11 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa
  * most 16, the reference's GpuMaterials limit).  Sets the poses; from then on the device
  * integrates them every substep (nxm_scene_read_colliders). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
-/* Current poses of the scene's colliders.  MPM integrates its coupled bodies every
- * substep (src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in
- * src_mpm_shaders/solver/rigid_impulses.rs:66-140): the cuboids carry no mass, so each
- * pose advances with its linear and angular velocity about its centre.  Counterpart of
+/* Current poses and velocities of the scene's colliders.  MPM integrates its coupled
+ * bodies every substep (src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in
+ * src_mpm_shaders/solver/rigid_impulses.rs:66-140): each pose advances with its linear
+ * and angular velocity about the body's centre of mass, and a dynamic body coupled two
+ * ways gains g dt of velocity along the axes of nonzero inverse mass.  Counterpart of
  * MpmPipeline::writeback_body_poses (src_mpm/pipeline.rs:536).  Synchronizes when the
  * scene is resident; out sized n >= the number of colliders. */
 nxm_status nxm_scene_read_colliders(nxm_scene *scene, nxm_cuboid *out, size_t n);
