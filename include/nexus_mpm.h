@@ -145,6 +145,19 @@ typedef struct {
     float local_com[3];
 } nxm_cuboid;
 
+/* A triangle-mesh collider on its body (a rapier TriMesh or HeightField collider coupled
+ * to MPM; GpuRigidParticles::from_rapier, src_mpm/solver/particle.rs:295-345): vertices
+ * and triangles in the body frame, and the body as in nxm_cuboid (pose, velocities,
+ * boundary condition, mass properties; half_extents ignored).  A heightfield is given
+ * as its triangles (HeightField::to_trimesh). */
+typedef struct {
+    const float *vertices;      /* num_vertices*3 */
+    const uint32_t *triangles;  /* num_triangles*3 vertex indices */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    nxm_cuboid body;
+} nxm_trimesh;
+
 /* Per-stage device times of the last step, milliseconds (the reference's
  * GpuTimestamps passes, src_mpm/pipeline.rs:394-511). */
 typedef struct {
@@ -196,10 +209,29 @@ nxm_status nxm_scene_set_cell_width(nxm_scene *scene, float cell_width, uint32_t
 /* GpuParticles::append (src_mpm/solver/particle.rs:533) / from_particles (:391). */
 nxm_status nxm_scene_add_particles(nxm_scene *scene, const nxm_particle *particles, size_t n);
 nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa *p, size_t n);
-/* MpmState::set_coupling (src_mpm/pipeline.rs:220) restricted to one-way cuboids (at
- * most 16, the reference's GpuMaterials limit).  Sets the poses; from then on the device
+/* MpmState::set_coupling (src_mpm/pipeline.rs:220) for cuboid colliders (with the mesh
+ * colliders at most 16, the reference's GpuMaterials limit).  Sets the poses; from then on the device
  * integrates them every substep (nxm_scene_read_colliders). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
+/* The mesh colliders of a CPIC scene (MpmState::with_select_coupling with TriMesh or
+ * HeightField colliders, src_mpm/pipeline.rs:297-316).  Their ids follow the cuboids'
+ * (cuboids + meshes <= 16); setting either kind keeps the other with its current poses.
+ * The meshes act on the particles through the contact distance field only: each
+ * substep their rigid particles (nxm_sample_trimesh at the cell width) reach the grid
+ * nodes around them (gpu_p2g_cdf, src_mpm_shaders/solver/p2g_cdf.rs:60-259).  The
+ * reference's grid collider pass, with CPIC off, also projects onto meshes; here a scene
+ * with meshes must use CPIC: NXM_ERR_INVALID_ARGUMENT when CPIC is off, and
+ * nxm_scene_set_cpic(scene, 0) is refused while meshes are set.  The bodies are
+ * integrated like the cuboids (nxm_scene_read_colliders lists them after the cuboids,
+ * half extents zero). */
+nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *scene, const nxm_trimesh *meshes, size_t n);
+/* sample_mesh (src_mpm/sampling/sample_trimesh.rs:60-231): the rigid particles of a
+ * triangle mesh, points on every triangle and along every edge once, at most
+ * spacing / sqrt 2 apart.  Writes up to capacity points (3 floats each) and the triangle
+ * of each; *count receives how many there are.  Host-only: no device call. */
+nxm_status nxm_sample_trimesh(const float *vertices, uint32_t num_vertices, const uint32_t *triangles,
+                              uint32_t num_triangles, float spacing, float *points, uint32_t *triangle_of,
+                              size_t capacity, size_t *count);
 /* Current poses and velocities of the scene's colliders.  MPM integrates its coupled
  * bodies every substep (src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in
  * src_mpm_shaders/solver/rigid_impulses.rs:66-140): each pose advances with its linear
@@ -211,10 +243,10 @@ nxm_status nxm_scene_read_colliders(nxm_scene *scene, nxm_cuboid *out, size_t n)
 /* MpmState::use_cpic (src_mpm/pipeline.rs:69, default off here): compatible
  * particle-in-cell transfers (pipeline.rs:433-460 with the use_cpic branches of p2g.rs,
  * grid_update.rs, g2p.rs and particle_update.rs) instead of the grid collider pass.
- * The contact distance field comes from the cuboids (grid_update_cdf.rs); mesh
- * colliders and their rigid particles (p2g_cdf.rs) are not part of this boundary.
- * Switching it resets every particle's contact field to zero (as for new particles);
- * synchronizes when the scene is resident. */
+ * The contact distance field comes from the cuboids (grid_update_cdf.rs) and from the
+ * mesh colliders' rigid particles (p2g_cdf.rs).  Switching it resets every particle's
+ * contact field to zero (as for new particles); synchronizes when the scene is
+ * resident; switching it off is refused while mesh colliders are set. */
 nxm_status nxm_scene_set_cpic(nxm_scene *scene, int enabled);
 /* Per-particle contact field of the last step (Kinematics::cdf, solver/particle.rs:
  * 35-50 without rigid_vel, which lives only inside the step): affinity word (bit i:

@@ -18,7 +18,9 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
+#include <cstddef>
 
 #include "../../include/nexus_mpm.h"
 #include "mpm_kernels.h"
@@ -171,6 +173,106 @@ static int pstate_words(uint32_t tag) { return (tag == 2u || tag == 3u) ? 3 : ta
 
 // Host copy of a scene.  Particle state lives here while the scene is not resident
 // in a World; static per-particle properties always live here (indexed by id).
+// ------------------------------------------------------------------------------------
+// Rigid particles of a triangle mesh: sample_mesh (src_mpm/sampling/sample_trimesh.rs:
+// 60-231), in f32 like the reference.  Every triangle is sampled in rows across its
+// longest edge (the base) and up its height, every s = spacing / sqrt 2 along both, each
+// row ending before the nearer of the two other edges; then each edge not seen before
+// is sampled every s from its first vertex.  Neither vertices nor the base carry
+// samples of the triangle pass; the edge pass covers the edges.
+
+struct V3f {
+    float x, y, z;
+};
+static inline V3f operator-(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3f operator+(V3f a, V3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3f operator*(V3f a, float k) { return {a.x * k, a.y * k, a.z * k}; }
+static inline V3f operator/(V3f a, float k) { return {a.x / k, a.y / k, a.z / k}; }
+static inline float dot3(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float norm3(V3f a) { return std::sqrt(dot3(a, a)); }
+
+struct MeshSamples {
+    std::vector<float> pts;        // 3 per sample, body frame
+    std::vector<uint32_t> tri;     // triangle of each sample
+    void push(V3f p, uint32_t t)
+    {
+        pts.push_back(p.x);
+        pts.push_back(p.y);
+        pts.push_back(p.z);
+        tri.push_back(t);
+    }
+};
+
+static constexpr float SAMPLE_EPS = 1.0e-5f;   // sample_trimesh.rs:12
+
+static void sample_triangle_rows(V3f a, V3f b, V3f c, float spacing, uint32_t t, MeshSamples &out)
+{
+    const float lab = norm3(b - a), lbc = norm3(c - b), lca = norm3(a - c);
+    const float longest = std::max(std::max(lab, lbc), lca);
+    V3f p0 = a, p1 = b, p2 = c;                 // p0 p1: the base
+    if (longest == lbc) { p0 = b; p1 = c; p2 = a; }
+    else if (longest == lca) { p0 = c; p1 = a; p2 = b; }
+    const V3f base = p1 - p0, side = p2 - p0;
+    const float base_len = norm3(base);
+    const V3f dir = base / base_len;
+    const float s = spacing / std::sqrt(2.0f);
+    const float along = dot3(side, dir);        // foot of the apex on the base
+    const float rest = base_len - along;
+    if (along < SAMPLE_EPS || rest < SAMPLE_EPS || base_len < SAMPLE_EPS) return;
+    const V3f up_vec = side - dir * along;
+    const float up_len = norm3(up_vec);
+    const V3f up = up_vec / up_len;
+    const float slope0 = up_len / along, slope1 = up_len / rest;
+    const uint32_t rows = (uint32_t)std::ceil(base_len / s);
+    const V3f row_step = dir * s, col_step = up * s;
+    for (uint32_t i = 1; i < rows; ++i) {
+        const V3f foot = p0 + row_step * (float)i;
+        const float height = std::min(slope0 * norm3(p0 - foot), slope1 * norm3(p1 - foot));
+        const uint32_t cols = (uint32_t)std::ceil(height / s);
+        for (uint32_t j = 1; j < cols; ++j) {
+            const V3f q = foot + col_step * (float)j;
+            if (std::isfinite(q.x) && std::isfinite(q.y) && std::isfinite(q.z)) out.push(q, t);
+        }
+    }
+}
+
+static void sample_edge_points(V3f a, V3f b, float spacing, uint32_t t, MeshSamples &out)
+{
+    const V3f e = b - a;
+    const float len = norm3(e);
+    if (!(len > SAMPLE_EPS)) return;
+    const V3f dir = e / len;
+    const float s = spacing / std::sqrt(2.0f);
+    const size_t steps = (size_t)std::ceil(len / s);
+    for (size_t i = 1; i < steps; ++i) out.push(a + dir * (s * (float)i), t);
+}
+
+static MeshSamples sample_mesh_surface(const float *v, const uint32_t *tri, uint32_t nt, float spacing)
+{
+    MeshSamples out;
+    std::unordered_set<uint64_t> edges;
+    auto vert = [&](uint32_t i) { return V3f{v[3 * i], v[3 * i + 1], v[3 * i + 2]}; };
+    for (uint32_t t = 0; t < nt; ++t) {
+        const uint32_t *ix = tri + 3 * t;
+        sample_triangle_rows(vert(ix[0]), vert(ix[1]), vert(ix[2]), spacing, t, out);
+        for (int e = 0; e < 3; ++e) {
+            const uint32_t i0 = ix[e], i1 = ix[(e + 1) % 3];
+            const uint64_t key = ((uint64_t)std::max(i0, i1) << 32) | std::min(i0, i1);
+            if (edges.insert(key).second) sample_edge_points(vert(i0), vert(i1), spacing, t, out);
+        }
+    }
+    return out;
+}
+
+// A triangle-mesh collider's geometry (body frame) and its rigid particles at the cell
+// width they were sampled for.
+struct HostMesh {
+    std::vector<float> v;
+    std::vector<uint32_t> tri;
+    MeshSamples samples;
+    float sampled_h = 0.0f;
+};
+
 struct HostParticles {
     size_t n = 0;
     // state
@@ -243,6 +345,7 @@ struct nxm_scene {
     bool debug_grid = false;
     bool graph = false;                 // standalone steps replay a captured graph
     bool cpic = false;                  // MpmState::use_cpic
+    std::vector<HostMesh> meshes;       // mesh colliders: the last meshes.size() entries of cols
 };
 
 namespace nxm {
@@ -856,6 +959,8 @@ class World {
         w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
         w.ncdf = ncdf_.p; w.nbody = nbody_.p; w.binc = binc_.p; w.bany = bany_.p; w.clist = clist_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
+        w.mesh_v = mesh_v_.p; w.mesh_t = mesh_t_.p; w.mesh_rp = mesh_rp_.p; w.mkey = mkey_.p; w.maff = maff_.p;
+        w.n_rp = n_rp_;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         return w;
     }
@@ -899,6 +1004,8 @@ class World {
         std::vector<uint32_t> region_start, region_scene;
         uint32_t b = 0, hsum = 0;
         bool any_cpic = false;
+        std::vector<float4> mv, mrp;
+        std::vector<uint4> mt;
         for (uint32_t i = 0; i < slots_.size(); ++i) {
             SceneDev &L = layout_[i];
             if (slots_[i].state != SLOT_ACTIVE) continue;
@@ -924,7 +1031,29 @@ class World {
             cls.insert(cls.end(), sc->classes.begin(), sc->classes.end());
             L.cpic = sc->cpic ? 1u : 0u;
             any_cpic |= sc->cpic;
+            // Mesh colliders (CPIC scenes only): rigid particles sampled at the cell width
+            // (src_mpm/pipeline.rs:310-316), geometry appended to the world's tables.
+            const uint32_t nmesh = (uint32_t)sc->meshes.size();
+            for (uint32_t k = 0; k < nmesh && sc->cpic; ++k) {
+                HostMesh &m = sc->meshes[k];
+                if (m.sampled_h != sc->h) {
+                    m.samples = sample_mesh_surface(m.v.data(), m.tri.data(), (uint32_t)(m.tri.size() / 3), sc->h);
+                    m.sampled_h = sc->h;
+                }
+                const uint32_t v0 = (uint32_t)mv.size(), t0 = (uint32_t)mt.size();
+                const uint32_t cid = L.col_count - nmesh + k;
+                for (size_t q = 0; q < m.v.size() / 3; ++q) mv.push_back(make_float4(m.v[3 * q], m.v[3 * q + 1], m.v[3 * q + 2], 0.0f));
+                for (size_t q = 0; q < m.tri.size() / 3; ++q)
+                    mt.push_back(make_uint4(v0 + m.tri[3 * q], v0 + m.tri[3 * q + 1], v0 + m.tri[3 * q + 2], (i << 4) | cid));
+                for (size_t q = 0; q < m.samples.tri.size(); ++q) {
+                    const uint32_t tid = t0 + m.samples.tri[q];
+                    float tf;
+                    std::memcpy(&tf, &tid, 4);
+                    mrp.push_back(make_float4(m.samples.pts[3 * q], m.samples.pts[3 * q + 1], m.samples.pts[3 * q + 2], tf));
+                }
+            }
         }
+        const bool any_mesh = !mrp.empty();
         // Dense block ids are ranks of occupied hash slots: sizing the block tables by
         // the slot count keeps every rank (and each scene's overflow block) in range
         // even when a scene exceeds its capacity (reported as an error).
@@ -938,7 +1067,9 @@ class World {
                           hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
                           std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
                           (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n) ||
-                          (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > ncdf_.n);
+                          (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > ncdf_.n) ||
+                          (any_mesh && ((size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > mkey_.n ||
+                                        mv.size() > mesh_v_.n || mt.size() > mesh_t_.n || mrp.size() > mesh_rp_.n));
         if (grow) {
             nxm_status st = sync();
             if (st != NXM_OK) return st;
@@ -1011,6 +1142,29 @@ class World {
             HIP_TRY(gridv_i_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
         any_cpic_ = any_cpic;
+        if (any_mesh) {
+            // k_grid_cdf empties every node k_mesh_cdf wrote in the same substep; fresh
+            // allocations are emptied here, outside any captured graph.
+            const size_t nodes = (size_t)NODES_PER_BLOCK * (std::max<size_t>(b_total_, 1) + b_total_ / 4);
+            if (nodes > mkey_.n) {
+                HIP_TRY(mkey_.reserve(nodes));
+                HIP_TRY(maff_.reserve(nodes));
+                mesh_nodes_clear_ = 0;
+            }
+            if (mesh_nodes_clear_ < mkey_.n) {
+                HIP_TRY(hipMemsetAsync(mkey_.p, 0xFF, mkey_.n * sizeof(unsigned long long), stream_));
+                HIP_TRY(hipMemsetAsync(maff_.p, 0, maff_.n * sizeof(uint32_t), stream_));
+                mesh_nodes_clear_ = mkey_.n;
+            }
+            HIP_TRY(mesh_v_.reserve(mv.size()));
+            HIP_TRY(mesh_t_.reserve(mt.size()));
+            HIP_TRY(mesh_rp_.reserve(mrp.size()));
+            nxm_status ms = put_async(mesh_v_.p, mv.data(), mv.size() * sizeof(float4));
+            if (ms == NXM_OK) ms = put_async(mesh_t_.p, mt.data(), mt.size() * sizeof(uint4));
+            if (ms == NXM_OK) ms = put_async(mesh_rp_.p, mrp.data(), mrp.size() * sizeof(float4));
+            if (ms != NXM_OK) return ms;
+        }
+        n_rp_ = (uint32_t)mrp.size();
         HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hvals_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hmark_.reserve(std::max<uint32_t>(h_total_, 1)));
@@ -1250,6 +1404,12 @@ class World {
     DevBuf<float> stage_i_;
     DevBuf<float4> gridv_i_;
     bool any_cpic_ = false;
+    DevBuf<float4> mesh_v_, mesh_rp_;  // mesh colliders of CPIC scenes (k_mesh_cdf)
+    DevBuf<uint4> mesh_t_;
+    DevBuf<unsigned long long> mkey_;  // per node, kept empty between substeps
+    DevBuf<uint32_t> maff_;
+    size_t mesh_nodes_clear_ = 0;      // nodes of mkey_ / maff_ set to empty since allocation
+    uint32_t n_rp_ = 0;
     DevBuf<SceneDev> scenes_;
     DevBuf<SceneCtr> ctr_;
     DevBuf<ClassDev> classes_;
@@ -1556,19 +1716,92 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *sc, const nxm_particles_soa *p
     return append_host(sc, p, nullptr, n);
 }
 
+static ColliderDev collider_record(const nxm_cuboid &c, uint32_t shape)
+{
+    static_assert(offsetof(ColliderDev, shape) == sizeof(nxm_cuboid), "nxm_cuboid is a prefix of ColliderDev");
+    ColliderDev d{};
+    std::memcpy(&d, &c, sizeof(nxm_cuboid));
+    d.shape = shape;
+    if (shape != SHAPE_CUBOID) d.he[0] = d.he[1] = d.he[2] = 0.0f;
+    return d;
+}
+
+// Replaces the cuboids (ids 0..n-1) or the meshes (the ids after the cuboids) and keeps
+// the other kind with its current poses.
+static nxm_status replace_colliders(nxm_scene *sc, const std::vector<ColliderDev> &cub, const std::vector<ColliderDev> *mesh)
+{
+    if (sc->world) {
+        nxm_status st = sc->world->pull_colliders(sc);
+        if (st != NXM_OK) return st;
+    }
+    const size_t nmesh_old = sc->meshes.size();
+    std::vector<ColliderDev> cols(cub);
+    if (mesh) cols.insert(cols.end(), mesh->begin(), mesh->end());
+    else cols.insert(cols.end(), sc->cols.end() - nmesh_old, sc->cols.end());
+    sc->cols.swap(cols);
+    sc->cols_host = true;
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
 nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
 {
     if (!sc || (!c && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
-    if (n > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
-    sc->cols.resize(n);
-    static_assert(offsetof(ColliderDev, shape) == sizeof(nxm_cuboid), "nxm_cuboid is a prefix of ColliderDev");
-    for (size_t i = 0; i < n; ++i) {
-        std::memcpy(&sc->cols[i], &c[i], sizeof(nxm_cuboid));
-        sc->cols[i].shape = SHAPE_CUBOID;
-        sc->cols[i].pad = 0;
+    if (n + sc->meshes.size() > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
+    std::vector<ColliderDev> cub(n);
+    for (size_t i = 0; i < n; ++i) cub[i] = collider_record(c[i], SHAPE_CUBOID);
+    return replace_colliders(sc, cub, nullptr);
+}
+
+nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *sc, const nxm_trimesh *m, size_t n)
+{
+    if (!sc || (!m && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t ncub = sc->cols.size() - sc->meshes.size();
+    if (ncub + n > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
+    if (n && !sc->cpic)
+        return fail(NXM_ERR_INVALID_ARGUMENT, "mesh colliders act through CPIC only: enable it first (nxm_scene_set_cpic)");
+    std::vector<HostMesh> meshes(n);
+    std::vector<ColliderDev> bodies(n);
+    for (size_t k = 0; k < n; ++k) {
+        if ((!m[k].vertices && m[k].num_vertices) || (!m[k].triangles && m[k].num_triangles))
+            return fail(NXM_ERR_INVALID_ARGUMENT, "null mesh array");
+        for (size_t q = 0; q < (size_t)m[k].num_triangles * 3; ++q)
+            if (m[k].triangles[q] >= m[k].num_vertices)
+                return fail(NXM_ERR_INVALID_ARGUMENT, "triangle vertex index out of range (mesh " + std::to_string(k) + ")");
+        meshes[k].v.assign(m[k].vertices, m[k].vertices + (size_t)m[k].num_vertices * 3);
+        meshes[k].tri.assign(m[k].triangles, m[k].triangles + (size_t)m[k].num_triangles * 3);
+        bodies[k] = collider_record(m[k].body, SHAPE_TRIMESH);
     }
-    sc->cols_host = true;
-    if (sc->world) sc->world->mark_dirty();
+    std::vector<ColliderDev> cub(sc->cols.begin(), sc->cols.begin() + ncub);
+    // The cuboids' current poses first (replace_colliders pulls them), then the meshes.
+    nxm_status st = NXM_OK;
+    if (sc->world) {
+        st = sc->world->pull_colliders(sc);
+        if (st != NXM_OK) return st;
+        cub.assign(sc->cols.begin(), sc->cols.begin() + ncub);
+    }
+    st = replace_colliders(sc, cub, &bodies);
+    if (st != NXM_OK) return st;
+    sc->meshes.swap(meshes);
+    return NXM_OK;
+}
+
+nxm_status nxm_sample_trimesh(const float *vertices, uint32_t num_vertices, const uint32_t *triangles,
+                              uint32_t num_triangles, float spacing, float *points, uint32_t *triangle_of,
+                              size_t capacity, size_t *count)
+{
+    if (!count || (!vertices && num_vertices) || (!triangles && num_triangles) || (capacity && (!points || !triangle_of)))
+        return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(spacing > 0.0f)) return fail(NXM_ERR_INVALID_ARGUMENT, "spacing must be positive");
+    for (size_t q = 0; q < (size_t)num_triangles * 3; ++q)
+        if (triangles[q] >= num_vertices) return fail(NXM_ERR_INVALID_ARGUMENT, "triangle vertex index out of range");
+    MeshSamples smp = sample_mesh_surface(vertices, triangles, num_triangles, spacing);
+    *count = smp.tri.size();
+    const size_t k = std::min(capacity, smp.tri.size());
+    if (k) {
+        std::memcpy(points, smp.pts.data(), k * 3 * sizeof(float));
+        std::memcpy(triangle_of, smp.tri.data(), k * sizeof(uint32_t));
+    }
     return NXM_OK;
 }
 
@@ -1738,6 +1971,8 @@ nxm_status nxm_scene_set_cpic(nxm_scene *sc, int enabled)
     if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
     const bool on = enabled != 0;
     if (on == sc->cpic) return NXM_OK;
+    if (!on && !sc->meshes.empty())
+        return fail(NXM_ERR_INVALID_ARGUMENT, "mesh colliders act through CPIC only: remove them first");
     World *w = sc->world;
     if (w) {
         // The world's tables and buffers change: bring the state to the host and re-enter.

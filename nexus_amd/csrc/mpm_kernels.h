@@ -49,6 +49,15 @@ struct WorldDev {
     float *stage_i;
     float4 *gridv_i;
     uint32_t any_cpic;
+    // Mesh colliders of CPIC scenes: geometry in the body frame and the rigid particles
+    // (surface samples); per node the meshes' part of the field, merged into ncdf and
+    // reset to empty by k_grid_cdf.
+    const float4 *mesh_v;          // vertices
+    const uint4 *mesh_t;           // triangles: three vertex ids, (world scene slot << 4) | collider
+    const float4 *mesh_rp;         // rigid particles: point, triangle id (bits)
+    unsigned long long *mkey;      // per node: least (distance bits << 32) | collider, ~0 when empty
+    uint32_t *maff;                // per node: the meshes' affinity bits
+    uint32_t n_rp;
     uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
 };
 

@@ -395,7 +395,7 @@ __device__ inline void block_info_part(const WorldDev &w)
         // Collider mask: collider j can touch a node of this block only if its bounding
         // sphere comes within 1.5 h (the projection reach) of the block's node sphere.
         bool hit = false;
-        if (j < sc.col_count) {
+        if (j < sc.col_count && w.cols[sc.col_start + j].shape == SHAPE_CUBOID) {
             const ColliderDev &c = w.cols[sc.col_start + j];
             const float h = sc.h;
             const f3 center = mk3(((float)(4 * v.x) + 1.5f) * h, ((float)(4 * v.y) + 1.5f) * h, ((float)(4 * v.z) + 1.5f) * h);
@@ -1282,9 +1282,74 @@ __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
 // ------------------------------------------------------------------------------------
 // CPIC contact distance field.
 
-// Per node of a CPIC scene's blocks: gpu_grid_update_cdf (grid_update_cdf.rs:60-132).
-// Mesh colliders (p2g_cdf) are not part of this boundary: the field comes from the
-// analytic cuboids alone.
+// Per rigid particle of a CPIC scene's mesh colliders: gpu_p2g_cdf (p2g_cdf.rs:60-259)
+// as a scatter.  The sample and its triangle take the body's pose of the substep's start
+// (gpu_transform_sample_points / gpu_transform_shape_points, rigid_particle_update.rs:
+// 10-55).  The nodes of the 3-cell range from the sample's associated cell
+// round(x / h) - 1 that lie in active blocks get, where their projection onto the
+// triangle's plane falls inside the triangle (edge tests with n = ab x ac), the
+// collider's affinity bit with the side n . (x - a) < 0, and the candidate distance
+// |n . (x - a)| / |n|.  Per node a 64-bit minimum of (distance bits << 32) | collider
+// keeps the least distance, the lower collider on a tie (the distances are not
+// negative, so their bits order like their values).  The reference reaches the same
+// nodes through its rigid-particle sort (a primary block, extras in the +1 neighbours,
+// grid/sort.rs:165-250, 493-680) and a per-block loop; here each block of the range is
+// looked up in the scene's hash.
+__global__ void __launch_bounds__(256) k_mesh_cdf(WorldDev w)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < w.n_rp; r += gridDim.x * blockDim.x) {
+        const float4 rp = w.mesh_rp[r];
+        const uint4 tri = w.mesh_t[__float_as_uint(rp.w)];
+        const SceneDev sc = w.scenes[tri.w >> 4];
+        if (!sc.active || !sc.cpic) continue;
+        const uint32_t cid = tri.w & 15u;
+        const ColliderDev &c = w.cols[sc.col_start + cid];
+        const f3 t = mk3(c.t[0], c.t[1], c.t[2]);
+        const float qx = c.q[0], qy = c.q[1], qz = c.q[2], qw = c.q[3];
+        const float4 va = w.mesh_v[tri.x], vb = w.mesh_v[tri.y], vc = w.mesh_v[tri.z];
+        const f3 p = quat_rotate(qx, qy, qz, qw, mk3(rp.x, rp.y, rp.z)) + t;
+        const f3 a = quat_rotate(qx, qy, qz, qw, mk3(va.x, va.y, va.z)) + t;
+        const f3 b = quat_rotate(qx, qy, qz, qw, mk3(vb.x, vb.y, vb.z)) + t;
+        const f3 cc = quat_rotate(qx, qy, qz, qw, mk3(vc.x, vc.y, vc.z)) + t;
+        const f3 ab = b - a, ac = cc - a, bc = cc - b;
+        const f3 n = cross(ab, ac);
+        const float nl = length(n);
+        if (nl == 0.0f) continue;
+        const f3 eab = cross(ab, n), ebc = cross(bc, n), eac = cross(ac, n);
+        const float h = sc.h, ih = sc.inv_h;
+        const float fx = round_div(p.x, h, ih) - 1.0f, fy = round_div(p.y, h, ih) - 1.0f,
+                    fz = round_div(p.z, h, ih) - 1.0f;
+        if (!(fabsf(fx) < 4000.0f && fabsf(fy) < 2000.0f && fabsf(fz) < 4000.0f)) continue;   // NaN too
+        const int cx = (int)fx, cy = (int)fy, cz = (int)fz;
+        for (int bz = cz >> 2; bz <= (cz + 2) >> 2; ++bz)
+            for (int by = cy >> 2; by <= (cy + 2) >> 2; ++by)
+                for (int bx = cx >> 2; bx <= (cx + 2) >> 2; ++bx) {
+                    // Keys of blocks outside the packable range would alias others.
+                    if (bx < -1023 || bx > 1023 || by < -511 || by > 511 || bz < -1023 || bz > 1023) continue;
+                    const uint32_t d = find_block(w, sc, bx, by, bz);
+                    if (d == NONE) continue;
+                    const int x0 = max(cx, 4 * bx), x1 = min(cx + 2, 4 * bx + 3);
+                    const int y0 = max(cy, 4 * by), y1 = min(cy + 2, 4 * by + 3);
+                    const int z0 = max(cz, 4 * bz), z1 = min(cz + 2, 4 * bz + 3);
+                    for (int z = z0; z <= z1; ++z)
+                        for (int y = y0; y <= y1; ++y)
+                            for (int x = x0; x <= x1; ++x) {
+                                const f3 pt = mk3((float)x, (float)y, (float)z) * h;
+                                const f3 ap = pt - a, bp = pt - b, cp = pt - cc;
+                                if (!(dot(eab, ap) <= 0.0f && dot(ebc, bp) <= 0.0f && dot(eac, cp) >= 0.0f)) continue;
+                                const float sd = dot(n, ap) / nl;
+                                const size_t node = (size_t)d * NODES_PER_BLOCK + (size_t)((x - 4 * bx) + 4 * (y - 4 * by) + 16 * (z - 4 * bz));
+                                atomicOr(&w.maff[node], (sd < 0.0f ? 0x00010001u : 0x00000001u) << cid);
+                                atomicMin(&w.mkey[node], ((unsigned long long)__float_as_uint(fabsf(sd)) << 32) | cid);
+                            }
+                }
+    }
+}
+
+// Per node of a CPIC scene's blocks: gpu_grid_update_cdf (grid_update_cdf.rs:60-132) for
+// the cuboids, then the meshes' part left by k_mesh_cdf: its affinity bits are added and
+// its closest collider replaces the cuboids' when strictly nearer (p2g_cdf.rs:241).  The
+// mesh part is reset to empty for the next substep.
 __global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
 {
     // One wave per block: lane = node; bany[d] records whether any node of block d is in
@@ -1300,19 +1365,28 @@ __global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
         uint32_t aff = 0u, closest = NONE, bc = 0u;
         const uint32_t mask = v.x != OVERFLOW_VID ? w.bmask[d] : 0u;
         const size_t n = (size_t)d * NODES_PER_BLOCK + l;
-        if (mask) {
-            const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
-            const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
-            node_cdf(w.cols + sc.col_start, sc.col_count, mask, sc.h, pt, dist, aff, closest);
-            if (closest != NONE) {
-                // For G2P's incompatible nodes: the closest body's velocity at the node
-                // (g2p.rs:214-231 evaluates it at the node position as dpt + x) and its
-                // boundary condition.
-                const ColliderDev &c = w.cols[sc.col_start + closest];
-                const f3 bpv = body_point_velocity(c, pt);
-                w.nbody[n] = make_float4(bpv.x, bpv.y, bpv.z, c.bc_friction);
-                bc = c.bc_type;
+        const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
+        const int4 vb = v.x != OVERFLOW_VID ? v : make_int4(0, 0, 0, 0);   // no mesh reaches the overflow block
+        const f3 pt = mk3((float)(vb.x * 4 + i), (float)(vb.y * 4 + j), (float)(vb.z * 4 + k)) * sc.h;
+        if (mask) node_cdf(w.cols + sc.col_start, sc.col_count, mask, sc.h, pt, dist, aff, closest);
+        if (w.n_rp) {
+            const unsigned long long key = w.mkey[n];
+            if (key != ~0ull) {
+                w.mkey[n] = ~0ull;
+                const float md = __uint_as_float((uint32_t)(key >> 32));
+                if (md < dist) { dist = md; closest = (uint32_t)key & 15u; }
             }
+            const uint32_t ma = w.maff[n];
+            if (ma) { w.maff[n] = 0u; aff |= ma; }
+        }
+        if (closest != NONE) {
+            // For G2P's incompatible nodes: the closest body's velocity at the node
+            // (g2p.rs:214-231 evaluates it at the node position as dpt + x) and its
+            // boundary condition.
+            const ColliderDev &c = w.cols[sc.col_start + closest];
+            const f3 bpv = body_point_velocity(c, pt);
+            w.nbody[n] = make_float4(bpv.x, bpv.y, bpv.z, c.bc_friction);
+            bc = c.bc_type;
         }
         w.ncdf[n] = make_uint4(__float_as_uint(dist), aff, closest, bc);
         const uint64_t any = __ballot(aff != 0u);
@@ -1601,6 +1675,7 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (w.any_cpic) {
         // CPIC worlds: the contact distance fields, then the CPIC variants of the
         // transfers (scenes without CPIC take the plain path inside them).
+        if (w.n_rp) k_mesh_cdf<<<grid_for(w.n_rp, T, cap), T, 0, st>>>(w);
         k_grid_cdf<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);   // one wave per block
         k_particle_cdf<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
         k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);

@@ -44,12 +44,33 @@ EXPORTS = [
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
-    "nxm_scene_stage_upload",
+    "nxm_scene_stage_upload", "nxm_scene_set_trimesh_colliders", "nxm_sample_trimesh",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
     "nxm_batch_profile_begin", "nxm_batch_profile_end",
 ]
+
+
+class _TriMesh(ct.Structure):
+    """nxm_trimesh."""
+    _fields_ = [("vertices", ct.c_void_p), ("triangles", ct.c_void_p), ("num_vertices", ct.c_uint32),
+                ("num_triangles", ct.c_uint32), ("body", ct.c_ubyte * 96)]
+
+
+def sample_trimesh(pipeline_or_lib, vertices, triangles, spacing: float):
+    """nxm_sample_trimesh: the library's rigid particles of a mesh (points, triangle ids)."""
+    L = getattr(pipeline_or_lib, "lib", pipeline_or_lib)
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    n = ct.c_size_t(0)
+    _check(L.nxm_sample_trimesh(_ptr(v), v.shape[0], _ptr(t), t.shape[0], ct.c_float(spacing), None, None, 0,
+                                ct.byref(n)))
+    pts = np.zeros((max(n.value, 1), 3), np.float32)
+    tri = np.zeros(max(n.value, 1), np.uint32)
+    _check(L.nxm_sample_trimesh(_ptr(v), v.shape[0], _ptr(t), t.shape[0], ct.c_float(spacing), _ptr(pts),
+                                _ptr(tri), n.value, ct.byref(n)))
+    return pts[:n.value], tri[:n.value]
 
 
 class MpmError(RuntimeError):
@@ -114,6 +135,9 @@ def library() -> ct.CDLL:
     L.nxm_scene_add_particles.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_add_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_set_colliders.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_set_trimesh_colliders.argtypes = [P, P, ct.c_size_t]
+    L.nxm_sample_trimesh.argtypes = [P, ct.c_uint32, P, ct.c_uint32, ct.c_float, P, P, ct.c_size_t,
+                                     ct.POINTER(ct.c_size_t)]
     L.nxm_scene_len.argtypes = [P]
     L.nxm_scene_len.restype = ct.c_size_t
     L.nxm_scene_device_bytes.argtypes = [P]
@@ -234,6 +258,8 @@ class MpmState:
             st.set_colliders(scene.colliders)
         if getattr(scene, "cpic", False):
             st.set_cpic(True)
+        if getattr(scene, "meshes", None):
+            st.set_trimesh_colliders(scene.meshes)
         st.add_particles(scene)
         return st
 
@@ -303,7 +329,27 @@ class MpmState:
     def set_colliders(self, cuboids: np.ndarray) -> None:
         c = np.ascontiguousarray(cuboids, dtype=_scenes.CUBOID_DTYPE)
         _check(self.lib.nxm_scene_set_colliders(self.handle, _ptr(c) if c.size else None, c.size))
-        self.num_colliders = int(c.size)
+        self.num_cuboids = int(c.size)
+        self.num_colliders = self.num_cuboids + int(getattr(self, "num_meshes", 0))
+
+    def set_trimesh_colliders(self, meshes) -> None:
+        """Mesh colliders (scenes.TriMesh) of a CPIC scene; their ids follow the cuboids'."""
+        keep = []
+        arr = (_TriMesh * max(len(meshes), 1))()
+        for k, m in enumerate(meshes):
+            v = np.ascontiguousarray(m.vertices, np.float32).reshape(-1, 3)
+            t = np.ascontiguousarray(m.triangles, np.uint32).reshape(-1, 3)
+            b = np.ascontiguousarray(m.body, dtype=_scenes.CUBOID_DTYPE).reshape(1)
+            keep += [v, t]
+            arr[k].vertices = v.ctypes.data
+            arr[k].triangles = t.ctypes.data
+            arr[k].num_vertices = v.shape[0]
+            arr[k].num_triangles = t.shape[0]
+            ct.memmove(ct.addressof(arr[k].body), b.ctypes.data, _scenes.CUBOID_DTYPE.itemsize)
+        _check(self.lib.nxm_scene_set_trimesh_colliders(self.handle, ct.cast(arr, ct.c_void_p) if meshes else None,
+                                                           len(meshes)))
+        self.num_meshes = len(meshes)
+        self.num_colliders = int(getattr(self, "num_cuboids", 0)) + self.num_meshes
 
     def set_debug_grid(self, enabled: bool = True) -> None:
         _check(self.lib.nxm_scene_set_debug_grid(self.handle, int(enabled)))

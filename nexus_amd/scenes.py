@@ -145,6 +145,7 @@ class Scene:
     name: str = ""
     cpic: bool = False                       # MpmState::use_cpic (src_mpm/pipeline.rs:69)
     boundary_friction: np.ndarray | None = None   # (n,) f32, None = 1 (Particle default)
+    meshes: list = field(default_factory=list)    # TriMesh colliders (CPIC), ids after the cuboids
 
     @property
     def n(self) -> int:
@@ -153,6 +154,7 @@ class Scene:
     def copy(self) -> "Scene":
         out = Scene(**{k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
                        for k, v in self.__dict__.items()})
+        out.meshes = list(self.meshes)
         return out
 
 
@@ -195,6 +197,52 @@ def cuboid(center, half_extents, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, fri
     c["inv_mass"] = inv_mass
     c["local_com"] = local_com
     return c
+
+
+@dataclass
+class TriMesh:
+    """A triangle-mesh collider: vertices (nv, 3) and triangles (nt, 3) in the body frame,
+    and its body as one CUBOID_DTYPE record (pose, velocities, boundary condition, mass
+    properties; half extents unused)."""
+
+    vertices: np.ndarray
+    triangles: np.ndarray
+    body: np.ndarray
+
+
+def trimesh(vertices, triangles, center, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0,
+            linvel=(0, 0, 0), angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)) -> TriMesh:
+    body = cuboid(center, (0.0, 0.0, 0.0), rotation, bc, friction, linvel, angvel, inv_mass, local_com)
+    return TriMesh(np.ascontiguousarray(vertices, np.float32).reshape(-1, 3),
+                   np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3), body)
+
+
+def box_mesh(half_extents):
+    """A closed box as 8 vertices and 12 triangles wound counter-clockwise seen from
+    outside (so ab x ac points out of the box)."""
+    hx, hy, hz = (float(e) for e in half_extents)
+    v = np.array([[sx * hx, sy * hy, sz * hz] for sz in (-1, 1) for sy in (-1, 1) for sx in (-1, 1)], np.float32)
+    # vertex index = (sx > 0) + 2 (sy > 0) + 4 (sz > 0)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    t = []
+    for a, b, c, d in quads:
+        t += [(a, b, c), (a, c, d)]
+    return v, np.array(t, np.uint32)
+
+
+def ramp_mesh(half_x, half_z, rise, nx=4, nz=4):
+    """An open, tessellated ramp y = rise (x / half_x) over [-half_x, half_x] x [-half_z,
+    half_z], triangles wound so that their normals point up (+y side)."""
+    xs = np.linspace(-half_x, half_x, nx + 1, dtype=np.float64)
+    zs = np.linspace(-half_z, half_z, nz + 1, dtype=np.float64)
+    v = np.array([[x, rise * x / half_x, z] for z in zs for x in xs], np.float32)
+    t = []
+    for j in range(nz):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            b, c, d = a + 1, a + nx + 2, a + nx + 1
+            t += [(a, d, c), (a, c, b)]
+    return v, np.array(t, np.uint32)
 
 
 def box_walls(h: float, bc=BC_STICK, friction=0.0, lo=0.0, hi=1.0) -> np.ndarray:
@@ -300,6 +348,24 @@ def sand_column_cpic(n: int = 1_048_576, seed: int = SEED_BASE + 2) -> Scene:
     with CPIC: the contact distance field needs cell widths of a few tenths (its
     least-squares matrix is singular below, g2p_cdf.rs:309)."""
     return scaled(sand_column(n, seed), 64.0, cpic=True, name=f"Drucker-Prager sand column {n}, x64, CPIC")
+
+
+def sand_on_mesh_ramp(n: int = 6000, seed: int = SEED_BASE + 4, *, angle=0.2) -> Scene:
+    """Drucker-Prager sand (h = 0.25, dt = 3.2e-3, CPIC) dropped onto a tessellated ramp
+    mesh (60 triangles, turned about y by `angle`, boundary condition separate with
+    friction 0.4) that stands on a slip floor cuboid: a mesh collider and a cuboid in one
+    contact field.  The ramp's edges are kept off the grid's node lines."""
+    h = 0.25
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.array([4.03, 3.31, 4.02]) + rng.random((n, 3)) * np.array([2.0, 1.2, 2.0])
+    parts = particles(x, h / 4, 2200.0, model_sand(3.5e5, 0.3, friction_deg=30.0))
+    q = np.array([0.0, math.sin(angle / 2), 0.0, math.cos(angle / 2)], np.float32)
+    rv, rt = ramp_mesh(3.0, 3.0, 0.6, nx=6, nz=5)
+    ramp = trimesh(rv, rt, (5.013, 2.4137, 4.987), rotation=q, bc=BC_SEPARATE, friction=0.4)
+    floor = cuboid((5.0, 0.5013, 5.0), (10.0, 0.5, 10.0), bc=BC_SLIP)
+    sc = make_scene(parts, h, 3.2e-3, colliders=floor, cpic=True, name=f"sand on a mesh ramp {n}")
+    sc.meshes = [ramp]
+    return sc
 
 
 def fluid_clumps(n: int = 8_388_608, seed: int = SEED_BASE + 3, *, n_clumps: int = 64,

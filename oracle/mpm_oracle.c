@@ -675,6 +675,191 @@ static void node_cdf(const orc_cuboid *cols, int ncol, REAL h, v3 pt, REAL *dist
     *closest = cid;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Mesh colliders: rigid particles (surface samples) and p2g_cdf. */
+
+/* f32 vector helpers of the sampler: the reference samples in f32 whatever REAL is. */
+typedef struct { float x, y, z; } f3s;
+static f3s fs_sub(f3s a, f3s b) { f3s r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static f3s fs_add(f3s a, f3s b) { f3s r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static f3s fs_mul(f3s a, float k) { f3s r = {a.x * k, a.y * k, a.z * k}; return r; }
+static f3s fs_div(f3s a, float k) { f3s r = {a.x / k, a.y / k, a.z / k}; return r; }
+static float fs_dot(f3s a, f3s b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static float fs_len(f3s a) { return sqrtf(fs_dot(a, a)); }
+static f3s fs_load(const float *v, uint32_t i) { f3s r = {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; return r; }
+
+#define SAMPLE_EPS 1.0e-5f /* sample_trimesh.rs:12 */
+
+typedef struct { float *pts; uint32_t *tri; int64_t n, cap; } sample_out;
+
+static void sample_push(sample_out *o, f3s p, uint32_t tri)
+{
+    if (o->n < o->cap) {
+        o->pts[3 * o->n] = p.x;
+        o->pts[3 * o->n + 1] = p.y;
+        o->pts[3 * o->n + 2] = p.z;
+        o->tri[o->n] = tri;
+    }
+    o->n++;
+}
+
+/* sample_edge (sample_trimesh.rs:111-131): points a + dir (s i), i = 1 .. ceil(len / s) - 1,
+ * s = spacing / sqrt 2. */
+static void sample_edge(f3s a, f3s b, float spacing, uint32_t tri, sample_out *o)
+{
+    f3s ab = fs_sub(b, a);
+    float len = fs_len(ab);
+    if (!(len > SAMPLE_EPS)) return;
+    f3s dir = fs_div(ab, len);
+    float s = spacing / sqrtf(2.0f);
+    int64_t nsteps = (int64_t)ceilf(len / s);
+    for (int64_t i = 1; i < nsteps; ++i) sample_push(o, fs_add(a, fs_mul(dir, s * (float)i)), tri);
+}
+
+/* sample_triangle (sample_trimesh.rs:133-231): the longest edge is the base AB; rows of
+ * points every s along the base, each row running up the height direction every s, as
+ * far as the nearer of the lines AC and BC; neither the base nor the vertices are
+ * sampled. */
+static void sample_triangle(f3s a, f3s b, f3s c, float spacing, uint32_t tri, sample_out *o)
+{
+    float dab = fs_len(fs_sub(b, a)), dbc = fs_len(fs_sub(c, b)), dca = fs_len(fs_sub(a, c));
+    float mx = fmaxf(fmaxf(dab, dbc), dca);
+    f3s A = a, B = b, Cc = c;
+    if (mx == dbc) { A = b; B = c; Cc = a; }
+    else if (mx == dca) { A = c; B = a; Cc = b; }
+    f3s ac = fs_sub(Cc, A), base = fs_sub(B, A);
+    float base_len = fs_len(base);
+    f3s base_dir = fs_div(base, base_len);
+    float s = spacing / sqrtf(2.0f);
+    float base_steps = ceilf(base_len / s);
+    f3s base_step = fs_mul(base_dir, s);
+    float ac_off = fs_dot(ac, base_dir);
+    float bc_off = base_len - ac_off;
+    if (ac_off < SAMPLE_EPS || bc_off < SAMPLE_EPS || base_len < SAMPLE_EPS) return;
+    f3s height = fs_sub(ac, fs_mul(base_dir, ac_off));
+    float height_len = fs_len(height);
+    f3s height_dir = fs_div(height, height_len);
+    float tan_a = height_len / ac_off, tan_b = height_len / bc_off;
+    uint32_t nb = (uint32_t)base_steps;
+    for (uint32_t i = 1; i < nb; ++i) {
+        f3s bp = fs_add(A, fs_mul(base_step, (float)i));
+        float h_ac = tan_a * fs_len(fs_sub(A, bp));
+        float h_bc = tan_b * fs_len(fs_sub(B, bp));
+        float hl = fminf(h_ac, h_bc);
+        uint32_t nh = (uint32_t)ceilf(hl / s);
+        f3s hstep = fs_mul(height_dir, s);
+        for (uint32_t j = 1; j < nh; ++j) {
+            f3s p = fs_add(bp, fs_mul(hstep, (float)j));
+            if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) continue;
+            sample_push(o, p, tri);
+        }
+    }
+}
+
+int64_t orc_sample_trimesh(const float *vertices, int64_t nv, const uint32_t *triangles, int64_t nt,
+                           float spacing, float *points, uint32_t *triangle_of, int64_t cap)
+{
+    (void)nv;
+    sample_out o = {points, triangle_of, 0, cap};
+    /* edges seen so far, keyed (max, min) vertex index (sample_trimesh.rs:68-77) */
+    int64_t hcap = 16;
+    while (hcap < 6 * nt + 16) hcap *= 2;
+    uint64_t *seen = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)hcap);
+    for (int64_t i = 0; i < hcap; ++i) seen[i] = ~0ull;
+    for (int64_t t = 0; t < nt; ++t) {
+        const uint32_t *ix = triangles + 3 * t;
+        f3s v0 = fs_load(vertices, ix[0]), v1 = fs_load(vertices, ix[1]), v2 = fs_load(vertices, ix[2]);
+        sample_triangle(v0, v1, v2, spacing, (uint32_t)t, &o);
+        const uint32_t ea[3] = {ix[0], ix[1], ix[2]}, eb[3] = {ix[1], ix[2], ix[0]};
+        for (int e = 0; e < 3; ++e) {
+            uint32_t hi = ea[e] > eb[e] ? ea[e] : eb[e], lo = ea[e] > eb[e] ? eb[e] : ea[e];
+            uint64_t key = ((uint64_t)hi << 32) | lo;
+            uint64_t slot = (key * 0x9E3779B97F4A7C15ull) >> 20;
+            int fresh = 0;
+            for (;;) {
+                slot &= (uint64_t)(hcap - 1);
+                if (seen[slot] == key) break;
+                if (seen[slot] == ~0ull) { seen[slot] = key; fresh = 1; break; }
+                slot++;
+            }
+            if (fresh) sample_edge(fs_load(vertices, ea[e]), fs_load(vertices, eb[e]), spacing, (uint32_t)t, &o);
+        }
+    }
+    free(seen);
+    return o.n;
+}
+
+/* Pose of a collider's body applied to a body-frame point: rotation * p + translation. */
+static v3 body_to_world(const orc_cuboid *c, const float *p)
+{
+    return vadd(quat_rotate(c->rotation[0], c->rotation[1], c->rotation[2], c->rotation[3], V3(p[0], p[1], p[2])),
+                V3(c->translation[0], c->translation[1], c->translation[2]));
+}
+
+/* gpu_p2g_cdf (p2g_cdf.rs:60-259) as a scatter from the rigid particles: each sample's
+ * triangle, posed, reaches the nodes of the 3-cell range from its associated cell
+ * round(x / h) - 1; a node whose projection onto the triangle's plane falls inside the
+ * triangle (edge tests with n = ab x ac) takes the unsigned plane distance and the
+ * collider's affinity bit, with the side n . (x - a) < 0 as sign.  Per node the
+ * closest mesh collider is the least (distance, collider id); it replaces the
+ * analytic shapes' closest only when strictly nearer (p2g_cdf.rs:241). */
+static void mesh_cdf(const orc_cpic *cp, const orc_cuboid *cols, int ncol, REAL h, const hmap *hm,
+                     const int32_t *bvid, int64_t nb, REAL *ndist, uint32_t *naff, uint32_t *nclosest)
+{
+    (void)bvid;
+    REAL *mdist = (REAL *)malloc(sizeof(REAL) * 64 * (size_t)(nb > 0 ? nb : 1));
+    uint32_t *mid = (uint32_t *)malloc(sizeof(uint32_t) * 64 * (size_t)(nb > 0 ? nb : 1));
+    for (int64_t i = 0; i < 64 * nb; ++i) { mdist[i] = MAX_FLT_REF; mid[i] = NONE; }
+    for (int k = 0; k < cp->nmesh; ++k) {
+        const orc_trimesh *m = &cp->meshes[k];
+        const uint32_t cid = (uint32_t)(ncol - cp->nmesh + k);
+        const orc_cuboid *body = &cols[cid];
+        for (int64_t s = 0; s < m->ns; ++s) {
+            v3 p = body_to_world(body, m->samples + 3 * s);
+            const uint32_t *ix = m->triangles + 3 * m->sample_triangle[s];
+            v3 a = body_to_world(body, m->vertices + 3 * ix[0]);
+            v3 b = body_to_world(body, m->vertices + 3 * ix[1]);
+            v3 c = body_to_world(body, m->vertices + 3 * ix[2]);
+            REAL pc[3] = {p.x, p.y, p.z};
+            int64_t base[3];
+            int ok = 1;
+            for (int d = 0; d < 3; ++d) {
+                REAL cell = RROUND(pc[d] / h) - K(1.0);
+                if (!(cell > K(-4000.0) && cell < K(4000.0))) ok = 0;
+                base[d] = ok ? (int64_t)cell : 0;
+            }
+            if (!ok) continue;
+            v3 ab = vsub(b, a), ac = vsub(c, a), bc = vsub(c, b);
+            v3 n = vcross(ab, ac);
+            REAL nl = vlen(n);
+            if (nl == K(0.0)) continue;
+            for (int q = 0; q < 27; ++q) {
+                int64_t cx = base[0] + q % 3, cy = base[1] + (q / 3) % 3, cz = base[2] + q / 9;
+                int32_t vid[3] = {(int32_t)((cx - ((cx % 4 + 4) % 4)) / 4), (int32_t)((cy - ((cy % 4 + 4) % 4)) / 4),
+                                  (int32_t)((cz - ((cz % 4 + 4) % 4)) / 4)};
+                if (vid[0] < -1023 || vid[0] > 1023 || vid[1] < -511 || vid[1] > 511 || vid[2] < -1023 || vid[2] > 1023)
+                    continue;
+                uint32_t blk = hmap_find(hm, pack_vid(vid));
+                if (blk == NONE) continue;
+                int64_t node = 64 * (int64_t)blk + ((cx % 4 + 4) % 4) + 4 * ((cy % 4 + 4) % 4) + 16 * ((cz % 4 + 4) % 4);
+                v3 x = vmul(V3((REAL)cx, (REAL)cy, (REAL)cz), h);
+                v3 ap = vsub(x, a), bp = vsub(x, b), cpv = vsub(x, c);
+                if (vdot(vcross(ab, n), ap) <= K(0.0) && vdot(vcross(bc, n), bp) <= K(0.0) &&
+                    vdot(vcross(ac, n), cpv) >= K(0.0)) {
+                    REAL sd = vdot(n, ap) / nl;
+                    REAL d = RFABS(sd);
+                    naff[node] |= (sd < K(0.0) ? 0x00010001u : 0x00000001u) << cid;
+                    if (d < mdist[node] || (d == mdist[node] && cid < mid[node])) { mdist[node] = d; mid[node] = cid; }
+                }
+            }
+        }
+    }
+    for (int64_t i = 0; i < 64 * nb; ++i)
+        if (mid[i] != NONE && mdist[i] < ndist[i]) { ndist[i] = mdist[i]; nclosest[i] = mid[i]; }
+    free(mdist);
+    free(mid);
+}
+
 /* 4x4 determinant and inverse through the 2x2 minors of the first two and the last two
  * rows (Laplace expansion); m is row-major here (symmetric inputs only). */
 static REAL m4_minors(const REAL m[16], REAL s[6], REAL c[6])
@@ -1044,8 +1229,8 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
         }
     }
 
-    /* --- CPIC: node CDFs (grid_update_cdf.rs:60-132), then each primary particle's CDF
-     *     (g2p_cdf.rs:331-407); no mesh colliders, so p2g_cdf adds nothing --- */
+    /* --- CPIC: node CDFs (grid_update_cdf.rs:60-132 for the cuboids, p2g_cdf.rs:60-259
+     *     for the meshes), then each primary particle's CDF (g2p_cdf.rs:331-407) --- */
     const int64_t nbn = nb > 0 ? nb : 1;
     REAL *ndist = NULL;
     uint32_t *naff = NULL, *nclosest = NULL;
@@ -1059,8 +1244,9 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
             int t = (int)(i % 64);
             const int32_t *vid = bvid + 3 * b;
             v3 pt = vmul(V3((REAL)(vid[0] * 4 + (t & 3)), (REAL)(vid[1] * 4 + ((t >> 2) & 3)), (REAL)(vid[2] * 4 + (t >> 4))), h);
-            node_cdf(cols, ncol, h, pt, ndist + i, naff + i, nclosest + i);
+            node_cdf(cols, ncol - cp->nmesh, h, pt, ndist + i, naff + i, nclosest + i);
         }
+        if (cp->nmesh > 0) mesh_cdf(cp, cols, ncol, h, &hm, bvid, nb, ndist, naff, nclosest);
         const REAL none_dist = MAX_FLT_REF;
         const uint32_t none_aff = 0u;
 #pragma omp parallel

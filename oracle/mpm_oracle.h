@@ -74,15 +74,38 @@ int orc_step(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ma
  * velocity at the particle (rigid_vel, n*3).  boundary_friction (n) scales the
  * colliders' friction per particle.  At most 16 colliders.
  */
+typedef struct orc_trimesh orc_trimesh;
 typedef struct {
     uint32_t *affinity;
     REAL *normal;
     REAL *signed_distance;
     REAL *rigid_vel;
     const REAL *boundary_friction;
+    const orc_trimesh *meshes;  /* the last nmesh colliders are these meshes' bodies */
+    int nmesh;
 } orc_cpic;
 
-/* orc_step with the CPIC transfers when cp != NULL: node contact distance field,
+/* A triangle-mesh collider (src_mpm/solver/particle.rs:295-345): vertices and triangles
+ * in the body frame and its rigid particles, the surface samples of orc_sample_trimesh
+ * with the triangle each belongs to.  Its body (pose, velocities, boundary condition,
+ * mass properties) is an orc_cuboid entry of the collider array, half extents unused. */
+struct orc_trimesh {
+    const float *vertices;          /* nv*3 */
+    const uint32_t *triangles;      /* nt*3 */
+    const float *samples;           /* ns*3 */
+    const uint32_t *sample_triangle;/* ns */
+    int64_t nv, nt, ns;
+};
+
+/* sample_mesh (src_mpm/sampling/sample_trimesh.rs:60-231): points on every triangle
+ * (rows spaced spacing / sqrt 2 along the longest edge and its height) and along every
+ * edge once, in f32.  Writes up to cap samples (points*3, triangle ids) and returns how
+ * many there are. */
+int64_t orc_sample_trimesh(const float *vertices, int64_t nv, const uint32_t *triangles, int64_t nt,
+                           float spacing, float *points, uint32_t *triangle_of, int64_t cap);
+
+/* orc_step with the CPIC transfers when cp != NULL: node contact distance field (the
+ * cuboids' by projection, the meshes' from their rigid particles),
  * particle CDF, compatible / incompatible P2G, both grid fields (no collider pass),
  * CPIC G2P and the particle update's reprojection and penalty.  Returns -3 for more
  * than 16 colliders. */

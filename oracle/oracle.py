@@ -45,6 +45,8 @@ def lib(bits: int = 32) -> ct.CDLL:
     L.orc_particle_cdf.argtypes = [P, ct.c_float, P, P, ct.c_uint32, P, P, P]
     L.orc_m4_solve.restype = ct.c_double if bits == 64 else ct.c_float
     L.orc_m4_solve.argtypes = [P, P, P]
+    L.orc_sample_trimesh.restype = ct.c_int64
+    L.orc_sample_trimesh.argtypes = [P, ct.c_int64, P, ct.c_int64, ct.c_float, P, P, ct.c_int64]
     for f in ("orc_ws_nblocks", "orc_ws_nprimary_blocks", "orc_ws_nsorted"):
         getattr(L, f).restype = ct.c_int64
         getattr(L, f).argtypes = [P]
@@ -71,9 +73,28 @@ def real(bits: int):
     return np.float32 if bits == 32 else np.float64
 
 
+class _TriMesh(ct.Structure):
+    _fields_ = [("vertices", ct.c_void_p), ("triangles", ct.c_void_p), ("samples", ct.c_void_p),
+                ("sample_triangle", ct.c_void_p), ("nv", ct.c_int64), ("nt", ct.c_int64), ("ns", ct.c_int64)]
+
+
 class _Cpic(ct.Structure):
     _fields_ = [("affinity", ct.c_void_p), ("normal", ct.c_void_p), ("signed_distance", ct.c_void_p),
-                ("rigid_vel", ct.c_void_p), ("boundary_friction", ct.c_void_p)]
+                ("rigid_vel", ct.c_void_p), ("boundary_friction", ct.c_void_p), ("meshes", ct.c_void_p),
+                ("nmesh", ct.c_int)]
+
+
+def sample_trimesh(vertices, triangles, spacing: float):
+    """The oracle's sample_mesh (src_mpm/sampling/sample_trimesh.rs:60-231): (points (k, 3)
+    f32, triangle of each point (k,) u32)."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    L = lib(32)
+    n = L.orc_sample_trimesh(_p(v), v.shape[0], _p(t), t.shape[0], ct.c_float(spacing), None, None, 0)
+    pts = np.zeros((max(n, 1), 3), np.float32)
+    tri = np.zeros(max(n, 1), np.uint32)
+    L.orc_sample_trimesh(_p(v), v.shape[0], _p(t), t.shape[0], ct.c_float(spacing), _p(pts), _p(tri), n)
+    return pts[:n], tri[:n]
 
 
 @dataclass
@@ -101,6 +122,7 @@ class OracleState:
     signed_distance: np.ndarray | None = None
     rigid_vel: np.ndarray | None = None
     boundary_friction: np.ndarray | None = None
+    meshes: list | None = None                   # per mesh collider: its arrays (vertices, ...)
 
     @staticmethod
     def from_scene(scene, bits: int = 32) -> "OracleState":
@@ -113,6 +135,22 @@ class OracleState:
         pstate[sand] = words[sand, 0:3]
         pstate[snow, 0] = words[snow, 0]
         sim = np.array([*scene.gravity, scene.dt, scene.cell_width], np.float32)
+        # Mesh colliders (CPIC only): their bodies follow the cuboids in the collider
+        # array; their rigid particles are sampled at the cell width
+        # (src_mpm/pipeline.rs:310-316).
+        meshes = list(getattr(scene, "meshes", None) or [])
+        cols = np.array(scene.colliders)
+        mesh_arrays = []
+        if meshes:
+            if not getattr(scene, "cpic", False):
+                raise ValueError("mesh colliders act through CPIC only")
+            cols = np.concatenate([cols, np.concatenate([m.body for m in meshes])])
+            for m in meshes:
+                v = np.ascontiguousarray(m.vertices, np.float32).reshape(-1, 3)
+                t = np.ascontiguousarray(m.triangles, np.uint32).reshape(-1, 3)
+                pts, tri = sample_trimesh(v, t, float(np.float32(scene.cell_width)))
+                mesh_arrays.append(dict(vertices=v, triangles=t, samples=np.ascontiguousarray(pts),
+                                        sample_triangle=np.ascontiguousarray(tri)))
         return OracleState(
             bits=bits,
             x=np.array(scene.x, R), v=np.array(scene.v, R),
@@ -122,8 +160,9 @@ class OracleState:
             enabled=np.ascontiguousarray(scene.enabled, np.uint32).copy(),
             fixed=np.array(scene.fixed, np.uint32), tag=np.array(scene.tag, np.uint32),
             params=np.array(words, np.float32), sim=sim,
-            colliders=np.array(scene.colliders),
+            colliders=cols,
             cpic=bool(getattr(scene, "cpic", False)),
+            meshes=mesh_arrays,
             affinity=np.zeros(scene.n, np.uint32), normal=np.zeros((scene.n, 3), R),
             signed_distance=np.zeros(scene.n, R), rigid_vel=np.zeros((scene.n, 3), R),
             boundary_friction=(np.ones(scene.n, R) if getattr(scene, "boundary_friction", None) is None
@@ -154,8 +193,11 @@ class Oracle:
         s = self.s
         cp = None
         if s.cpic:
+            self._meshes = (_TriMesh * max(len(s.meshes), 1))(*[
+                _TriMesh(_p(m["vertices"]), _p(m["triangles"]), _p(m["samples"]), _p(m["sample_triangle"]),
+                         m["vertices"].shape[0], m["triangles"].shape[0], m["samples"].shape[0]) for m in s.meshes])
             cp = _Cpic(_p(s.affinity), _p(s.normal), _p(s.signed_distance), _p(s.rigid_vel),
-                       _p(s.boundary_friction))
+                       _p(s.boundary_friction), ct.cast(self._meshes, ct.c_void_p), len(s.meshes))
         for _ in range(count):
             rc = self.L.orc_step_ex(
                 self.ws, s.n, _p(s.x), _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.damping),
