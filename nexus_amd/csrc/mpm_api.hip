@@ -1060,7 +1060,8 @@ class World {
         (void)b;
         // Buffers a step in flight may still use are reallocated only after it finished;
         // otherwise the new tables are copied behind it on the stream.
-        const uint32_t scan_stride = std::max<uint32_t>(hsum, hsum) / 1024u + 2u;
+        // One look-back word per scan tile (k_scan), for each of the two scans.
+        const uint32_t scan_stride = (hsum + SCAN_TILE - 1u) / SCAN_TILE + 2u;
         const uint32_t nreg = (uint32_t)region_start.size();
         const bool grow = std::max<size_t>(slots_.size(), 1) > scenes_.n || std::max<size_t>(slots_.size(), 1) > ctr_.n ||
                           std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(ncols, 1) > cols_[0].n || std::max<size_t>(ncols, 1) > cols_[1].n ||

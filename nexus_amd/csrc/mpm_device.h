@@ -15,6 +15,9 @@ constexpr int TILE = 6;                   // 6^3 node tile: a block plus its +1 
 constexpr int TILE_NODES = TILE * TILE * TILE;
 constexpr int NODES_PER_BLOCK = 64;       // 4^3 (src_mpm_shaders/grid/grid.rs:25)
 constexpr float MAX_FLT_REF = 1.0e18f;    // src_rbd_shaders/lib.rs:283
+// Elements per tile of the single-pass scans (k_scan).  The look-back state array holds
+// one word per tile for each of the two scans: the host sizes it by this constant.
+constexpr int SCAN_TILE = 1024;
 
 // Particle flags word: bit 0 enabled, bit 1 fixed, bits 2..15 material class,
 // bits 16..31 scene slot.

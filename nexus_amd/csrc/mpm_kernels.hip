@@ -208,10 +208,9 @@ __global__ void k_keys_touch(WorldDev w)
     }
 }
 
-// Exclusive scans, 1024 elements per tile.  MODE_SLOTS counts occupied hash slots
+// Exclusive scans, SCAN_TILE elements per tile.  MODE_SLOTS counts occupied hash slots
 // (dense block ids); MODE_COUNTS sums the particles of each block (first sorted
 // position of each block).
-constexpr int SCAN_TILE = 2048;
 enum ScanMode { MODE_SLOTS = 0, MODE_COUNTS = 1 };
 
 template <int MODE>
@@ -299,7 +298,9 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
     __shared__ uint32_t sh[4];
     __shared__ uint32_t s_tile, s_prefix;
     const uint32_t len = MODE == MODE_SLOTS ? w.h_total : min(w.g->n_blocks, w.b_total);
-    const uint32_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    // The state array has scan_stride words per scan (the host sizes it for len); the
+    // bound keeps every tile's word inside it whatever the host passed.
+    const uint32_t ntiles = min((len + SCAN_TILE - 1) / SCAN_TILE, w.scan_stride - 1u);
     unsigned long long *state = w.scan_state + (MODE == MODE_SLOTS ? 0u : w.scan_stride);
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(&w.g->ticket[MODE], 1u);
