@@ -434,6 +434,7 @@ class World {
     nxm_status upload(nxm_scene *sc, size_t first, size_t count)
     {
         if (count == 0) return NXM_OK;
+        clean_ = false;   // the new particles have no keys yet
         if (p_in_ + count > p_cap_) {
             nxm_status st = sync();   // the arenas are reallocated: nothing may use them
             if (st != NXM_OK) return st;
@@ -647,14 +648,17 @@ class World {
             if (st != NXM_OK) return st;
             if (tev[0]) HIP_TRY(hipEventRecord(tev[0], stream_));
         }
-        if (!clean_) {
-            // Hash regions, counters and scan state must start empty; each substep
-            // leaves them so for the next.
+        // clean_: the input arena's block keys and insertions are in place, left by the
+        // previous substep's G2P, and the counters and scan state are empty.  Otherwise
+        // (first substep, arena or tables changed on the host) everything is emptied and
+        // the substep starts with the key pass.
+        const bool keys = !clean_;
+        if (keys) {
             launch_clear(w, lc, stream_);
             HIP_TRY(hipGetLastError());
         }
         clean_ = false;
-        if (graph_on_ && !evs && !tuning_) {
+        if (graph_on_ && !evs && !tuning_ && !keys) {
             // Replays the substep's launches as one graph; recaptured when the launch
             // shapes or any buffer address change.  The graph holds kernels only.
             const GraphKey k{p_in_, b_total_, h_total_, gen_, cfg.p2g_groups, cfg.p2g_block_cost, cfg.grid_groups};
@@ -662,7 +666,7 @@ class World {
                 if (gexec_[cur_]) { HIP_TRY(hipGraphExecDestroy(gexec_[cur_])); gexec_[cur_] = nullptr; }
                 hipGraph_t graph = nullptr;
                 HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-                launch_substep(w, lc, stream_, nullptr);
+                launch_substep(w, lc, stream_, nullptr, false);
                 hipError_t le = hipGetLastError();
                 hipError_t ce = hipStreamEndCapture(stream_, &graph);
                 if (le != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return hip_fail(le, "launch_substep (capture)"); }
@@ -674,7 +678,7 @@ class World {
             }
             HIP_TRY(hipGraphLaunch(gexec_[cur_], stream_));
         } else {
-            launch_substep(w, lc, stream_, evs);
+            launch_substep(w, lc, stream_, evs, keys);
         }
         HIP_TRY(hipGetLastError());
         clean_ = true;

@@ -67,7 +67,9 @@ struct LaunchConfig {
     int p2g_blocks;                // workgroups of the per-block kernels
 };
 
-void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev);
+// keys: run k_keys_touch first (the hash and counters emptied by launch_clear); without
+// it the substep uses the keys and block insertions of the previous substep's G2P.
+void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys);
 void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st);
 void launch_set_particles(Globals *g, uint32_t p_in, hipStream_t st);
 void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st);

@@ -130,81 +130,95 @@ __global__ void k_clear(WorldDev w)
     }
 }
 
-// Hash keys and marks are left empty by the previous substep's G2P kernel (k_clear
-// does it before the first substep and after the tables change); the counters used
-// later in this substep are cleared here.
-__global__ void k_keys_touch(WorldDev w)
+// Block key of particle slot i at position (x, y, z) of active scene s (written to
+// pkey[i]), and the wave's block insertions: adjacent lanes with the same (scene, key)
+// form a run; each run inserts its block (marked primary) and the block's 7 +1
+// neighbours (the active set is P + {0,1}^3, src_mpm/grid/grid.rs:193-200).  Called by
+// every lane of a wave together (invalid lanes with valid = false), so that the lanes'
+// neighbours in the wave are the particles' neighbours in the arena.
+__device__ inline void touch_keys(const WorldDev &w, uint32_t i, bool valid, uint32_t s, float x, float y, float z)
 {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
-    for (uint32_t i = gid; i < w.b_total; i += gsz) w.bnp[i] = 0u;
-    for (uint32_t i = gid; i < 2u * w.scan_stride; i += gsz) w.scan_state[i] = 0ull;
-    if (gid == 0) { w.g->p_out = 0u; w.g->n_blocks = 0u; w.g->ticket[0] = 0u; w.g->ticket[1] = 0u; }
-    // The virtual overflow slot closing each active scene's region.
-    for (uint32_t s = gid; s < w.s_count; s += gsz) {
+    const uint32_t l = lane_id();
+    uint32_t key = EMPTY_KEY;
+    if (valid) {
+        const SceneDev sc = w.scenes[s];
+        int32_t bx, by, bz;
+        uint32_t lx, ly, lz;
+        cell_of(x, sc.h, sc.inv_h, bx, lx);
+        cell_of(y, sc.h, sc.inv_h, by, ly);
+        cell_of(z, sc.h, sc.inv_h, bz, lz);
+        if (vid_in_range(bx, by, bz)) {
+            key = pack_vid(bx, by, bz);
+        } else {
+            key = OVERFLOW_KEY;
+            atomicOr(&w.ctr[s].err, ERR_DOMAIN);
+        }
+        w.pkey[i] = key;
+    }
+    const uint32_t pk = __shfl_up(key, 1, WAVE);
+    const uint32_t ps = __shfl_up(s, 1, WAVE);
+    const bool pv = __shfl_up((int)valid, 1, WAVE) != 0;
+    const bool head = valid && key != OVERFLOW_KEY && (l == 0 || !pv || pk != key || ps != s);
+    // Up to 8 runs per round: lane l inserts neighbour (l mod 8) of run (l / 8), so a
+    // wave's insertions overlap instead of waiting for each other's atomics.
+    uint64_t heads = __ballot(head);
+    while (heads) {
+        uint32_t hsrc[8];
+        #pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            hsrc[k] = heads ? (uint32_t)__ffsll((unsigned long long)heads) - 1u : 64u;
+            heads = heads ? heads & (heads - 1ull) : 0ull;
+        }
+        const uint32_t r = l >> 3, j = l & 7u;
+        uint32_t src = hsrc[0];
+        #pragma unroll
+        for (int k = 1; k < 8; ++k) src = r == (uint32_t)k ? hsrc[k] : src;
+        const uint32_t hk = __shfl(key, (int)(src & 63u), WAVE);
+        const uint32_t hs = __shfl(s, (int)(src & 63u), WAVE);
+        if (src < 64u) {
+            const SceneDev hsc = w.scenes[hs];
+            const int4 v = unpack_vid(hk);
+            uint32_t slot;
+            hash_insert(w.hkeys + hsc.h_base, hsc.h_cap - 1u,
+                        pack_vid(v.x + ASSOC[j][0], v.y + ASSOC[j][1], v.z + ASSOC[j][2]), slot);
+            if (slot == NONE) atomicOr(&w.ctr[hs].err, ERR_GRID_CAPACITY);
+            else if (j == 0) w.hmark[hsc.h_base + slot] = 1u;
+        }
+    }
+}
+
+// The virtual overflow slot closing each active scene's hash region.
+__device__ inline void mark_overflow_slots(const WorldDev &w)
+{
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.s_count; s += gridDim.x * blockDim.x) {
         const SceneDev sc = w.scenes[s];
         if (sc.active) w.hkeys[sc.h_base + sc.h_cap] = OVERFLOW_KEY;
     }
+}
+
+// Keys and block insertions of every particle of the input arena.  Used for a substep
+// whose keys the previous substep's G2P did not insert (the first one, and after the
+// host changed the arena or the tables): k_clear has emptied the hash and the counters
+// before it.
+__global__ void k_keys_touch(WorldDev w)
+{
+    mark_overflow_slots(w);
     const uint32_t n = w.g->p_in;
     const uint32_t nr = (n + WAVE - 1) / WAVE * WAVE;
-    const uint32_t l = lane_id();
     const float *X = w.in.plane(PX), *Y = w.in.plane(PY), *Z = w.in.plane(PZ);
     const uint32_t *FL = w.in.uplane(PFLAGS);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
         bool valid = i < n;
-        uint32_t s = 0, key = EMPTY_KEY;
-        SceneDev sc;
+        uint32_t s = 0;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
         if (valid) {
             s = flags_scene(FL[i]);
-            sc = w.scenes[s];
-            valid = sc.active != 0u;
+            valid = w.scenes[s].active != 0u;
+            x = X[i];
+            y = Y[i];
+            z = Z[i];
         }
-        if (valid) {
-            int32_t bx, by, bz;
-            uint32_t lx, ly, lz;
-            cell_of(X[i], sc.h, sc.inv_h, bx, lx);
-            cell_of(Y[i], sc.h, sc.inv_h, by, ly);
-            cell_of(Z[i], sc.h, sc.inv_h, bz, lz);
-            if (vid_in_range(bx, by, bz)) {
-                key = pack_vid(bx, by, bz);
-            } else {
-                key = OVERFLOW_KEY;
-                atomicOr(&w.ctr[s].err, ERR_DOMAIN);
-            }
-            w.pkey[i] = key;
-        }
-        // Run heads: adjacent lanes with the same (scene, key) insert once.  For each
-        // run, lanes 0..7 insert the block and its 7 +1 neighbours (the active set is
-        // P + {0,1}^3, src_mpm/grid/grid.rs:193-200); lane 0 marks the block primary.
-        uint32_t pk = __shfl_up(key, 1, WAVE);
-        uint32_t ps = __shfl_up(s, 1, WAVE);
-        bool pv = __shfl_up((int)valid, 1, WAVE) != 0;
-        bool head = valid && key != OVERFLOW_KEY && (l == 0 || !pv || pk != key || ps != s);
-        // Up to 8 runs per round: lane l inserts neighbour (l mod 8) of run (l / 8), so a
-        // wave's insertions overlap instead of waiting for each other's atomics.
-        uint64_t heads = __ballot(head);
-        while (heads) {
-            uint32_t hsrc[8];
-            #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                hsrc[k] = heads ? (uint32_t)__ffsll((unsigned long long)heads) - 1u : 64u;
-                heads = heads ? heads & (heads - 1ull) : 0ull;
-            }
-            const uint32_t r = l >> 3, j = l & 7u;
-            uint32_t src = hsrc[0];
-            #pragma unroll
-            for (int k = 1; k < 8; ++k) src = r == (uint32_t)k ? hsrc[k] : src;
-            const uint32_t hk = __shfl(key, (int)(src & 63u), WAVE);
-            const uint32_t hs = __shfl(s, (int)(src & 63u), WAVE);
-            if (src < 64u) {
-                const SceneDev hsc = w.scenes[hs];
-                const int4 v = unpack_vid(hk);
-                uint32_t slot;
-                hash_insert(w.hkeys + hsc.h_base, hsc.h_cap - 1u,
-                            pack_vid(v.x + ASSOC[j][0], v.y + ASSOC[j][1], v.z + ASSOC[j][2]), slot);
-                if (slot == NONE) atomicOr(&w.ctr[hs].err, ERR_GRID_CAPACITY);
-                else if (j == 0) w.hmark[hsc.h_base + slot] = 1u;
-            }
-        }
+        touch_keys(w, i, valid, s, x, y, z);
     }
 }
 
@@ -301,6 +315,11 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
     // The state array has scan_stride words per scan (the host sizes it for len); the
     // bound keeps every tile's word inside it whatever the host passed.
     const uint32_t ntiles = min((len + SCAN_TILE - 1) / SCAN_TILE, w.scan_stride - 1u);
+    if (MODE == MODE_SLOTS) {
+        // The block counts of this substep start at zero (k_block_count adds to them; the
+        // previous substep's stay readable until here).
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.b_total; i += gridDim.x * blockDim.x) w.bnp[i] = 0u;
+    }
     unsigned long long *state = w.scan_state + (MODE == MODE_SLOTS ? 0u : w.scan_stride);
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(&w.g->ticket[MODE], 1u);
@@ -848,6 +867,16 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
     }
 }
 
+// Empties the hash regions (keys and primary marks) for the next substep's insertions.
+__device__ inline void clear_hash(const WorldDev &w)
+{
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < w.h_total; i += gsz) {
+        w.hkeys[i] = EMPTY_KEY;
+        w.hmark[i] = 0u;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Grid update: gpu_grid_update + gpu_grid_update_collide (grid_update.rs:21-88,
 // grid_update_collide.rs:105-183) on the summed tiles.  CPIC scenes update both velocity
@@ -855,6 +884,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
 template <bool CPIC>
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
+    clear_hash(w);   // not read after the block tables (k_block_count, k_mesh_cdf) are built
     uint32_t nbegin, nend;
     xcd_range(w.g->n_blocks * NODES_PER_BLOCK, nbegin, nend);
     for (uint32_t n = nbegin + threadIdx.x; n < nend; n += blockDim.x) {
@@ -1174,17 +1204,6 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
     }
 }
 
-// The hash regions are not read after the block tables are built: the G2P kernels leave
-// them empty for the next substep.
-__device__ inline void clear_hash(const WorldDev &w)
-{
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
-    for (uint32_t i = gid; i < w.h_total; i += gsz) {
-        w.hkeys[i] = EMPTY_KEY;
-        w.hmark[i] = 0u;
-    }
-}
-
 // Body integration at the end of a substep (the reference's "Integrate bodies" pass,
 // src_mpm/pipeline.rs:510-520, gpu_rigid_impulses_update in rigid_impulses.rs:66-140):
 // MPM integrates its own copy of every coupled body (gpu_rigid_impulses_update,
@@ -1251,9 +1270,23 @@ __device__ inline bool contact_particle(const WorldDev &w, uint32_t pos, uint32_
     return w.scenes[w.bscene[d]].cpic && (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
 }
 
+// After this substep's scans: their look-back state and tickets start empty for the next
+// substep, whose keys the G2P kernels insert into the hash (emptied by the grid update).
+__device__ inline void prepare_next_sort(const WorldDev &w)
+{
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < 2u * w.scan_stride; i += gsz) w.scan_state[i] = 0ull;
+    if (gid == 0) { w.g->ticket[0] = 0u; w.g->ticket[1] = 0u; }
+    mark_overflow_slots(w);
+}
+
 // One thread per sorted particle, nodes read from the grid.  In CPIC worlds this kernel
 // takes the plain particles and k_g2p_contact the contact ones, so that the contact code
-// does not weigh on the register budget of the common path.
+// does not weigh on the register budget of the common path.  Then the new positions give
+// the next substep's block keys and insertions (touch_keys): the output arena is the next
+// input arena, in the same order, so the next substep starts at its scan.  That second
+// loop runs whole waves (lanes past the range are invalid) over the positions this same
+// thread has just written.
 template <bool CPIC>
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
@@ -1265,11 +1298,27 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
         if (CPIC && contact_particle(w, pos, d)) continue;
         g2p_particle<GatherGrid, false>(w, pos, d, grid);
     }
-    clear_hash(w);
+    const uint32_t l = lane_id();
+    for (uint32_t base = pbegin + (threadIdx.x & ~(uint32_t)(WAVE - 1)); base < pend; base += blockDim.x) {
+        const uint32_t pos = base + l;
+        bool valid = pos < pend;
+        uint32_t s = 0;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (valid) {
+            const uint32_t d = w.sblock[pos];
+            s = w.bscene[d];
+            valid = !(CPIC && contact_particle(w, pos, d));   // keyed by k_g2p_contact
+            x = w.out.plane(PX)[pos];
+            y = w.out.plane(PY)[pos];
+            z = w.out.plane(PZ)[pos];
+        }
+        touch_keys(w, pos, valid, s, x, y, z);
+    }
+    prepare_next_sort(w);
     if (!CPIC) integrate_colliders_part(w);   // CPIC worlds: after k_g2p_contact, which reads them
 }
 
-// The contact particles listed by k_particle_cdf.
+// The contact particles listed by k_particle_cdf, then their keys (as in k_g2p_update).
 __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
 {
     const uint32_t n = w.g->n_contact;
@@ -1277,6 +1326,22 @@ __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t pos = w.clist[i];
         g2p_particle<GatherGrid, true>(w, pos, w.sblock[pos], grid);
+    }
+    const uint32_t l = lane_id();
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(WAVE - 1)); base < n;
+         base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + l;
+        const bool valid = i < n;
+        uint32_t pos = 0, s = 0;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (valid) {
+            pos = w.clist[i];
+            s = w.bscene[w.sblock[pos]];
+            x = w.out.plane(PX)[pos];
+            y = w.out.plane(PY)[pos];
+            z = w.out.plane(PZ)[pos];
+        }
+        touch_keys(w, pos, valid, s, x, y, z);
     }
 }
 
@@ -1660,12 +1725,12 @@ static inline int grid_for(uint64_t n, int threads, int cap)
     return (int)g;
 }
 
-void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev)
+void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys)
 {
     const int T = 256;
     const int cap = lc.persistent_blocks;
     if (ev) (void)hipEventRecord(ev[0], st);
-    k_keys_touch<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    if (keys) k_keys_touch<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     const int hscan = std::min((int)((w.h_total + SCAN_TILE - 1) / SCAN_TILE), cap);
     k_scan<MODE_SLOTS><<<std::max(hscan, 1), 256, 0, st>>>(w);
     k_block_count<<<grid_for(std::max<uint64_t>(lc.p_hint, (uint64_t)w.b_total * 16), T, cap), T, 0, st>>>(w);
