@@ -962,7 +962,7 @@ class World {
         w.g = g_.p; w.scan_state = scan_state_.p; w.scan_stride = scan_stride_;
         w.p2g_start = p2g_start_.p; w.p2g_groups = cfg_.p2g_groups; w.p2g_block_cost = cfg_.p2g_block_cost;
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
-        w.ncdf = ncdf_.p; w.nbody = nbody_.p; w.binc = binc_.p; w.bany = bany_.p; w.clist = clist_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
+        w.ncdf = ncdf_.p; w.nbody = nbody_.p; w.binc = binc_.p; w.bany = bany_.p; w.breach = any_cpic_ ? breach_.p : nullptr; w.clist = clist_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
         w.mesh_v = mesh_v_.p; w.mesh_t = mesh_t_.p; w.mesh_rp = mesh_rp_.p; w.mkey = mkey_.p; w.maff = maff_.p;
         w.n_rp = n_rp_;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
@@ -1143,6 +1143,7 @@ class World {
             HIP_TRY(nbody_.reserve((size_t)NODES_PER_BLOCK * nb));
             HIP_TRY(binc_.reserve(nb));
             HIP_TRY(bany_.reserve(nb));
+            HIP_TRY(breach_.reserve(nb));
             HIP_TRY(stage_i_.reserve((size_t)4 * TILE_NODES * nb));
             HIP_TRY(gridv_i_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
@@ -1405,7 +1406,7 @@ class World {
     DevBuf<float4> gridv_;
     DevBuf<uint4> ncdf_;               // CPIC buffers, allocated once a CPIC scene is present
     DevBuf<float4> nbody_;
-    DevBuf<uint32_t> binc_, bany_, clist_;
+    DevBuf<uint32_t> binc_, bany_, breach_, clist_;
     DevBuf<float> stage_i_;
     DevBuf<float4> gridv_i_;
     bool any_cpic_ = false;

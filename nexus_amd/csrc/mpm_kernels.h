@@ -45,6 +45,7 @@ struct WorldDev {
     float4 *nbody;                 // per node with a closest collider: its velocity there, its friction
     uint32_t *binc;                // per block: its P2G tile has incompatible contributions
     uint32_t *bany;                // per block: some node is in reach of a collider
+    uint32_t *breach;              // per primary block: some node of its particles' stencils is (null: no CPIC)
     uint32_t *clist;               // sorted positions of the contact particles (n_contact of them)
     float *stage_i;
     float4 *gridv_i;

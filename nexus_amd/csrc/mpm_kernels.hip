@@ -401,6 +401,7 @@ __device__ inline void block_info_part(const WorldDev &w)
     const uint64_t n = (uint64_t)w.g->n_blocks * 16;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t d = (uint32_t)(t >> 4), j = (uint32_t)(t & 15);
+        if (j == 0 && w.breach) w.breach[d] = 0u;   // set by k_grid_cdf (CPIC worlds)
         const int4 v = w.bvid[d];
         if (v.x == OVERFLOW_VID) continue;
         const SceneDev sc = w.scenes[w.bscene[d]];
@@ -816,7 +817,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                 if (on) {
                     p = p2g_load(w, slot);
                     p2g_bound(p, h, bmom, bm);
-                    if (CPIC && sc.cpic) pa = w.out.uplane(PAFF)[first + l];
+                    if (CPIC && w.breach[d]) pa = w.out.uplane(PAFF)[first + l];
                 }
             }
             bmom = wave_max(bmom);
@@ -840,7 +841,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
                 const uint32_t slot = w.perm[pos];
                 if (FL[slot] & FLAG_ENABLED) {
                     p2g_bound(p2g_load(w, slot), h, bmom, bm);
-                    if (CPIC && sc.cpic) anyc |= (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
+                    if (CPIC && w.breach[d]) anyc |= (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
                 }
             }
         }
@@ -1062,10 +1063,11 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                     // g2p.rs:206-233: a node the particle is not compatible with gives its
                     // incompatible velocity, seen through its closest collider's boundary
                     // condition (friction scaled by the particle's factor) relative to the
-                    // body's velocity there.  Unrolled by 9: enough loads in flight without
-                    // the register count of a full unroll.
+                    // body's velocity there.  Fully unrolled: the contact particles are few,
+                    // so their kernel runs about one wave per SIMD and needs every load in
+                    // flight rather than occupancy.
                     const f3 pn = mk3(w.out.plane(PNX)[pos], w.out.plane(PNY)[pos], w.out.plane(PNZ)[pos]);
-                    #pragma unroll 9
+                    #pragma unroll
                     for (int q = 0; q < 27; ++q) {
                         const int sx = NBH[q][0], sy = NBH[q][1], sz = NBH[q][2];
                         const uint32_t ni = gather.index(bx + sx, by + sy, bz + sz);
@@ -1267,7 +1269,8 @@ __global__ void __launch_bounds__(256) k_integrate_colliders(WorldDev w) { integ
 // with every node and have a zero contact field, so the plain path is exact for them).
 __device__ inline bool contact_particle(const WorldDev &w, uint32_t pos, uint32_t d)
 {
-    return w.scenes[w.bscene[d]].cpic && (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
+    // breach is zero for the blocks of scenes without CPIC.
+    return w.breach[d] && (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
 }
 
 // After this substep's scans: their look-back state and tickets start empty for the next
@@ -1457,6 +1460,12 @@ __global__ void __launch_bounds__(256) k_grid_cdf(WorldDev w)
         w.ncdf[n] = make_uint4(__float_as_uint(dist), aff, closest, bc);
         const uint64_t any = __ballot(aff != 0u);
         if (l == 0) w.bany[d] = any != 0ull ? 1u : 0u;
+        // The primary blocks whose particles' stencils include nodes of this block (block d
+        // is their +e neighbour, e in {0,1}^3) have particles in reach of a collider.
+        if (any && l < 8u) {
+            const uint32_t src = w.bdn[8 * (size_t)d + l];
+            if (src != NONE) w.breach[src] = 1u;
+        }
     }
 }
 
@@ -1491,23 +1500,15 @@ __device__ __forceinline__ uint32_t particle_cdf_one(const WorldDev &w, uint32_t
                 wy[0] = 0.5f * (1.5f - fy) * (1.5f - fy); wy[1] = 0.75f - (fy - 1.0f) * (fy - 1.0f); wy[2] = 0.5f * (fy - 0.5f) * (fy - 0.5f);
                 wz[0] = 0.5f * (1.5f - fz) * (1.5f - fz); wz[1] = 0.75f - (fz - 1.0f) * (fz - 1.0f); wz[2] = 0.5f * (fz - 0.5f) * (fz - 0.5f);
             }
-            GatherGrid gather{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
-            gather.prepare(d);
-            // No node of the 8 blocks of the stencil is in reach of a collider: every
-            // affinity is empty, the fit has no node and the field is zero.
-            uint32_t reach = 0u;
-            reach |= gather.n0 != NONE ? w.bany[gather.n0] : 0u;
-            reach |= gather.n1 != NONE ? w.bany[gather.n1] : 0u;
-            reach |= gather.n2 != NONE ? w.bany[gather.n2] : 0u;
-            reach |= gather.n3 != NONE ? w.bany[gather.n3] : 0u;
-            reach |= gather.n4 != NONE ? w.bany[gather.n4] : 0u;
-            reach |= gather.n5 != NONE ? w.bany[gather.n5] : 0u;
-            reach |= gather.n6 != NONE ? w.bany[gather.n6] : 0u;
-            reach |= gather.n7 != NONE ? w.bany[gather.n7] : 0u;
-            if (reach == 0u) {
+            // No node of the 8 blocks of the stencil (block d and its +1 neighbours) is in
+            // reach of a collider: every affinity is empty, the fit has no node and the
+            // field is zero.
+            if (!w.breach[d]) {
                 w.out.uplane(PAFF)[pos] = 0u;
                 return 0u;
             }
+            GatherGrid gather{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
+            gather.prepare(d);
             float nd[27], wq[27];
             uint32_t na[27];
             uint32_t pa = 0u;
@@ -1574,29 +1575,32 @@ __device__ __forceinline__ uint32_t particle_cdf_one(const WorldDev &w, uint32_t
     return aff;
 }
 
+// Appends pos to a list (one counter atomic per wave; the order does not matter).
+__device__ inline void wave_append(uint32_t *list, uint32_t *count, bool take, uint32_t pos)
+{
+    const uint32_t l = lane_id();
+    const uint64_t active = __ballot(1), m = __ballot(take);
+    if (!m) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)active) - 1u;
+    uint32_t base = 0u;
+    if (l == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader, WAVE);
+    if (take) list[base + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = pos;
+}
+
+// Per particle of a CPIC scene: its contact field (particle_cdf_one); the particles left
+// with affinity bits go to the contact list that k_g2p_contact processes densely.
 __global__ void __launch_bounds__(256) k_particle_cdf(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
-    const uint32_t l = lane_id();
     for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
         const uint32_t d = w.sblock[pos];
         const SceneDev sc = w.scenes[w.bscene[d]];
         const bool contact = sc.cpic && (particle_cdf_one(w, pos, d, sc) & 0xFFFFu) != 0u;
-        // Contact particles go to a list (one counter atomic per wave) that k_g2p_contact
-        // processes densely; the order of the list does not matter.
-        const uint64_t active = __ballot(1), m = __ballot(contact);
-        if (m) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)active) - 1u;
-            uint32_t base = 0u;
-            if (l == leader) base = atomicAdd(&w.g->n_contact, (uint32_t)__popcll(m));
-            base = __shfl(base, (int)leader, WAVE);
-            if (contact) w.clist[base + (uint32_t)__popcll(m & ((1ull << l) - 1ull))] = pos;
-        }
+        wave_append(w.clist, &w.g->n_contact, contact, pos);
     }
 }
-
-
 
 // ------------------------------------------------------------------------------------
 // CFL timestep bound (timestep_bound.rs:61-111, models/default.rs:268-330,
