@@ -237,6 +237,34 @@ def test_swap_remove_then_step(pipeline):
     ref.close()
 
 
+def test_append_then_step(pipeline):
+    """GpuParticles::append on a resident scene mid-run (src_mpm/solver/particle.rs:533):
+    the appended particles join the next substep (whose block keys the last G2P did not
+    make for them), bit for bit as a scene built from the same state with them, and in
+    graph replay as eagerly."""
+    sc = small_sand(6000, seed=52)
+    extra = small_sand(1500, seed=53)
+    for graph in (False, True):
+        st = mpm.MpmState.from_scene(pipeline, sc)
+        st.set_graph(graph)
+        for _ in range(3):
+            pipeline.step(st)
+        p = st.read_particles()
+        st.add_particles(extra)
+        for _ in range(4):
+            pipeline.step(st)
+        keys = ("x", "v", "C", "F", "mass", "vol0", "radius", "damping", "enabled", "fixed", "tag", "model")
+        both = {k: np.concatenate([np.array(p[k]), np.array(getattr(extra, k))]) for k in keys}
+        ref_scene = scenes.Scene(cell_width=sc.cell_width, dt=sc.dt, gravity=sc.gravity, colliders=sc.colliders, **both)
+        ref = run_gpu(pipeline, ref_scene, 4)
+        a, b = st.read_particles(), ref.read_particles()
+        assert len(st) == sc.n + extra.n
+        for f in ("x", "v", "C", "F", "enabled"):
+            assert np.array_equal(a[f], b[f]), (graph, f)
+        st.close()
+        ref.close()
+
+
 def test_launch_shapes_do_not_change_results(pipeline):
     """P2G sums in exact fixed point and the other stages work per particle or per node,
     so any launch shape gives the same bits: fixed extreme shapes and an autotuned run
