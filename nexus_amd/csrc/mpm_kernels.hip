@@ -1,10 +1,11 @@
 // mpm_kernels.hip — gfx950 kernels of the MLS-MPM substep.
 //
 // One substep over every active scene of a world (device data in WorldDev):
-//   k_clear         clear hash regions, block tables and counters (first substep and after
-//                   the tables change; afterwards k_keys_touch and k_g2p_update keep them)
+//   k_clear         clear hash regions, block tables and counters, then
 //   k_keys_touch    per particle: block key; per run of equal keys in a wave, insertion
-//                   of the block and its 7 +1 neighbours
+//                   of the block and its 7 +1 neighbours (both only for the first substep
+//                   and after the host changed the arena or the tables: otherwise the
+//                   previous substep's k_g2p_update made the keys and insertions)
 //   k_scan<SLOTS>   single-pass scan of occupied hash slots -> dense block ids
 //   k_block_count   per block: dense ids of the 7 +1 and 7 -1 neighbours, collider mask;
 //                   per particle: dense block id, wave-aggregated block counts and ranks
@@ -12,9 +13,12 @@
 //   k_scatter       per particle: sorted position (block-major permutation)
 //   k_p2g           per block: LDS-staged scatter of its particles into a 6^3 node tile
 //   k_grid_update   per node: sum of the tiles covering it, momentum -> velocity, gravity,
-//                   clamp, cuboid colliders
+//                   clamp, cuboid colliders; empties the hash
 //   k_g2p_update    per sorted particle: 27-node gather + deformation / constitutive update,
-//                   written in sorted order to the output arena
+//                   written in sorted order to the output arena; then the next substep's
+//                   block keys and insertions
+// CPIC worlds add the contact fields (k_mesh_cdf, k_grid_cdf, k_particle_cdf) before P2G,
+// the CPIC variants of the transfers, k_g2p_contact and k_integrate_colliders.
 // The reference runs the same stages as ~21 dispatches (src_mpm/pipeline.rs:385-526).
 #include <type_traits>
 
