@@ -225,6 +225,40 @@ nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders
  * integrated like the cuboids (nxm_scene_read_colliders lists them after the cuboids,
  * half extents zero). */
 nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *scene, const nxm_trimesh *meshes, size_t n);
+
+/* ReadbackData (src_mpm_shaders/solver/prep_readback.rs:84-92): a particle's render
+ * colour, its deformation (column-major 4x4, the 3x3 in the upper left) and position. */
+typedef struct {
+    float color[4];
+    float deformation[16];
+    float position[3];
+    float pad;
+} nxm_readback_data;
+
+/* RenderConfig::mode values (prep_readback.rs:16-21); any other value is the default
+ * (the particle group's colour). */
+#define NXM_RENDER_DEFAULT 0u
+#define NXM_RENDER_VOLUME 1u
+#define NXM_RENDER_VELOCITY 2u
+#define NXM_RENDER_PHASE 3u
+#define NXM_RENDER_CDF_NORMALS 4u
+#define NXM_RENDER_CDF_DISTANCES 5u
+#define NXM_RENDER_CDF_SIGNS 6u
+
+/* WgPrepReadback::launch (src_mpm/solver/prep_readback.rs:153-210, gpu_prep_readback):
+ * per particle, in id order, its render data computed on the device: the deformation
+ * diag(2 init_radius) F with F's entries clamped to [-4, 4], and the colour by mode from
+ * the group palette (group_colors: num_groups RGBA colours, group id modulo their count;
+ * NULL / 0: the reference's six-colour default palette), the velocity, the singular values
+ * of F (largest first), the phase, or the contact field (zero for scenes without CPIC);
+ * disabled particles in red.  out sized n >= the scene's particle count; synchronizes. */
+nxm_status nxm_scene_prep_readback(nxm_scene *scene, uint32_t mode, const float *group_colors, uint32_t num_groups,
+                                   nxm_readback_data *out, size_t n);
+/* gpu_prep_readback_rigid (prep_readback.rs:429-469): the mesh colliders' rigid particles
+ * at their bodies' current poses, rigid particle i coloured by palette entry i modulo its
+ * size, deformation diag(0.4 h).  Writes up to n; *count receives how many there are. */
+nxm_status nxm_scene_prep_readback_rigid(nxm_scene *scene, const float *group_colors, uint32_t num_groups,
+                                         nxm_readback_data *out, size_t n, size_t *count);
 /* sample_mesh (src_mpm/sampling/sample_trimesh.rs:60-231): the rigid particles of a
  * triangle mesh, points on every triangle and along every edge once, at most
  * spacing / sqrt 2 apart.  Writes up to capacity points (3 floats each) and the triangle

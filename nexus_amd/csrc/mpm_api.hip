@@ -784,6 +784,39 @@ class World {
         return NXM_OK;
     }
 
+    // Render data of the scene's particles (gpu_prep_readback), in id order.
+    nxm_status prep_readback(nxm_scene *sc, uint32_t mode, const float *palette, uint32_t npal, float *out)
+    {
+        HIP_TRY(hipSetDevice(ctx_->device));
+        if (dirty_) {
+            nxm_status st = rebuild_tables();
+            if (st != NXM_OK) return st;
+        }
+        const size_t n = sc->hp.n;
+        if (n == 0) return NXM_OK;
+        const HostParticles &hp = sc->hp;
+        DevBuf<uint32_t> group;
+        DevBuf<float> props, dout;
+        DevBuf<float4> pal;
+        HIP_TRY(group.reserve(n));
+        HIP_TRY(props.reserve(2 * n));
+        HIP_TRY(pal.reserve(npal));
+        HIP_TRY(dout.reserve(24 * n));
+        nxm_status st = put_async(group.p, hp.group.data(), n * 4);
+        if (st == NXM_OK) st = put_async(props.p, hp.phase.data(), n * 4);
+        if (st == NXM_OK) st = put_async(props.p + n, hp.radius.data(), n * 4);
+        if (st == NXM_OK) st = put_async(pal.p, palette, (size_t)npal * 16);
+        if (st != NXM_OK) return st;
+        for (const SceneRange &r : sc->ranges)
+            launch_prep_readback(arena(cur_), r.start, r.count, group.p, props.p, props.p + n, pal.p, npal, mode, sc->h,
+                                 sc->dt, sc->cpic, dout.p, (int)ctx_->num_cus * 8, stream_);
+        HIP_TRY(hipGetLastError());
+        st = sync();
+        if (st != NXM_OK) return st;
+        HIP_TRY(hipMemcpy(out, dout.p, 24 * n * sizeof(float), hipMemcpyDeviceToHost));
+        return NXM_OK;
+    }
+
     // ---- inspection of the last step ----
     nxm_status sort_size(nxm_scene *sc, size_t *nb)
     {
@@ -1941,6 +1974,78 @@ nxm_status nxm_scene_timestep_bound(nxm_scene *sc, float *max_dt)
     if (st != NXM_OK) return st;
     *max_dt = (float)units / 1.0e12f;
     return latched(sc, NXM_OK);
+}
+
+// GpuReadbackData::DEFAULT_GROUP_COLORS (src_mpm/solver/prep_readback.rs:58-65): the
+// palette used when the caller gives none.
+static const float DEFAULT_GROUP_COLORS[6][4] = {
+    {124.0f / 255.0f, 144.0f / 255.0f, 1.0f, 1.0f},         {8.0f / 255.0f, 144.0f / 255.0f, 1.0f, 1.0f},
+    {124.0f / 255.0f, 7.0f / 255.0f, 1.0f, 1.0f},           {124.0f / 255.0f, 144.0f / 255.0f, 7.0f / 255.0f, 1.0f},
+    {200.0f / 255.0f, 37.0f / 255.0f, 1.0f, 1.0f},          {124.0f / 255.0f, 230.0f / 255.0f, 25.0f / 255.0f, 1.0f},
+};
+
+nxm_status nxm_scene_prep_readback(nxm_scene *sc, uint32_t mode, const float *group_colors, uint32_t num_groups,
+                                   nxm_readback_data *out, size_t n)
+{
+    if (!sc || (!out && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_groups && !group_colors) return fail(NXM_ERR_INVALID_ARGUMENT, "null palette");
+    if (n < sc->hp.n) return fail(NXM_ERR_INVALID_ARGUMENT, "output too small");
+    static_assert(sizeof(nxm_readback_data) == 96, "ReadbackData layout");
+    const float *pal = num_groups ? group_colors : &DEFAULT_GROUP_COLORS[0][0];
+    const uint32_t npal = num_groups ? num_groups : 6u;
+    if (!sc->batch) {
+        nxm_status st = ensure_own_world(sc);
+        if (st != NXM_OK) return st;
+    }
+    nxm_status st = sc->world->prep_readback(sc, mode, pal, npal, reinterpret_cast<float *>(out));
+    if (st != NXM_OK) return st;
+    return latched(sc, NXM_OK);
+}
+
+nxm_status nxm_scene_prep_readback_rigid(nxm_scene *sc, const float *group_colors, uint32_t num_groups,
+                                         nxm_readback_data *out, size_t n, size_t *count)
+{
+    if (!sc || !count || (!out && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_groups && !group_colors) return fail(NXM_ERR_INVALID_ARGUMENT, "null palette");
+    const float *pal = num_groups ? group_colors : &DEFAULT_GROUP_COLORS[0][0];
+    const uint32_t npal = num_groups ? num_groups : 6u;
+    if (sc->world) {
+        nxm_status st = sc->world->pull_colliders(sc);
+        if (st != NXM_OK) return st;
+    }
+    // The rigid particles, posed by their bodies (gpu_transform_sample_points), in the
+    // order of the meshes and their samples: gpu_prep_readback_rigid (prep_readback.rs:
+    // 429-469) with rigid particle i coloured by palette entry i mod the palette size.
+    const size_t ncub = sc->cols.size() - sc->meshes.size();
+    size_t total = 0;
+    for (size_t k = 0; k < sc->meshes.size(); ++k) {
+        HostMesh &m = sc->meshes[k];
+        if (m.sampled_h != sc->h) {
+            m.samples = sample_mesh_surface(m.v.data(), m.tri.data(), (uint32_t)(m.tri.size() / 3), sc->h);
+            m.sampled_h = sc->h;
+        }
+        const ColliderDev &b = sc->cols[ncub + k];
+        for (size_t q = 0; q < m.samples.tri.size(); ++q, ++total) {
+            if (total >= n) continue;
+            float *o = reinterpret_cast<float *>(out + total);
+            const float px = m.samples.pts[3 * q], py = m.samples.pts[3 * q + 1], pz = m.samples.pts[3 * q + 2];
+            // rotation * p + translation, the quaternion rotation v (w^2 - b.b) + 2 b (b.v) + 2 w (b x v)
+            const float bx = b.q[0], by = b.q[1], bz = b.q[2], bw = b.q[3];
+            const float b2 = bx * bx + by * by + bz * bz, bv = bx * px + by * py + bz * pz;
+            const float k0 = bw * bw - b2, k2 = 2.0f * bw;
+            o[20] = px * k0 + bx * (2.0f * bv) + (by * pz - bz * py) * k2 + b.t[0];
+            o[21] = py * k0 + by * (2.0f * bv) + (bz * px - bx * pz) * k2 + b.t[1];
+            o[22] = pz * k0 + bz * (2.0f * bv) + (bx * py - by * px) * k2 + b.t[2];
+            o[23] = 0.0f;
+            const float *c = pal + 4 * (total % npal);
+            o[0] = c[0]; o[1] = c[1]; o[2] = c[2]; o[3] = c[3];
+            const float sdiag = sc->h * 0.4f;
+            for (int e = 0; e < 16; ++e) o[4 + e] = 0.0f;
+            o[4] = sdiag; o[9] = sdiag; o[14] = sdiag; o[19] = 1.0f;
+        }
+    }
+    *count = total;
+    return NXM_OK;
 }
 
 nxm_status nxm_scene_sort_size(nxm_scene *sc, size_t *nb)

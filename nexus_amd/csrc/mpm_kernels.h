@@ -72,6 +72,10 @@ struct LaunchConfig {
 // it the substep uses the keys and block insertions of the previous substep's G2P.
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys);
 void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st);
+// Render data (24 floats per particle, at its id) of arena positions [start, start + count).
+void launch_prep_readback(const Arena &a, uint32_t start, uint32_t count, const uint32_t *group, const float *phase,
+                          const float *radius, const float4 *palette, uint32_t npal, uint32_t mode, float h, float dt,
+                          bool cdf, float *out, int cap, hipStream_t st);
 void launch_set_particles(Globals *g, uint32_t p_in, hipStream_t st);
 void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st);
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,

@@ -1793,4 +1793,87 @@ void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uin
     k_canonical<<<grid_for(p_hint, 256, 4096), 256, 0, st>>>(w, prev, pcell, buckets);
 }
 
+// ------------------------------------------------------------------------------------
+// Render readback: gpu_prep_readback (src_mpm_shaders/solver/prep_readback.rs:245-300)
+// over arena positions [start, start + count) of one scene, written at each particle's id
+// as 24 floats (ReadbackData: colour, deformation as a column-major 4x4 with the 3x3 in
+// its upper left, position, padding).  group / phase / radius are the particles' constant
+// properties by id; cdf says whether the scene's contact field is live (CPIC).
+__global__ void __launch_bounds__(256) k_prep_readback(Arena a, uint32_t start, uint32_t count, const uint32_t *group,
+                                                       const float *phase, const float *radius, const float4 *palette,
+                                                       uint32_t npal, uint32_t mode, float h, float dt, uint32_t cdf,
+                                                       float *out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = start + i;
+        const uint32_t id = a.uplane(PID)[slot];
+        const uint32_t fl = a.uplane(PFLAGS)[slot];
+        m3 F;
+        F.c0 = mk3(a.plane(PF0)[slot], a.plane(PF1)[slot], a.plane(PF2)[slot]);
+        F.c1 = mk3(a.plane(PF3)[slot], a.plane(PF4)[slot], a.plane(PF5)[slot]);
+        F.c2 = mk3(a.plane(PF6)[slot], a.plane(PF7)[slot], a.plane(PF8)[slot]);
+        const float4 base = palette[group[id] % npal];
+        // compute_deformation (:97-118): diag(2 r) times F with entries clamped to [-4, 4].
+        const float d = radius[id] * 2.0f;
+        float *o = out + 24 * (size_t)id;
+        const f3 cols[3] = {F.c0, F.c1, F.c2};
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o[4 + 4 * c + 0] = d * fminf(fmaxf(cols[c].x, -4.0f), 4.0f);
+            o[4 + 4 * c + 1] = d * fminf(fmaxf(cols[c].y, -4.0f), 4.0f);
+            o[4 + 4 * c + 2] = d * fminf(fmaxf(cols[c].z, -4.0f), 4.0f);
+            o[4 + 4 * c + 3] = 0.0f;
+        }
+        o[16] = 0.0f; o[17] = 0.0f; o[18] = 0.0f; o[19] = 1.0f;
+        // compute_color (:170-243).
+        float4 col = base;
+        const uint32_t aff = cdf ? a.uplane(PAFF)[slot] : 0u;
+        if (mode == 2u) {           // velocity
+            const f3 v = mk3(a.plane(PVX)[slot], a.plane(PVY)[slot], a.plane(PVZ)[slot]);
+            col = make_float4(fabsf(v.x) * dt * 100.0f + 0.2f, fabsf(v.y) * dt * 100.0f + 0.2f,
+                              fabsf(v.z) * dt * 100.0f + 0.2f, base.w);
+        } else if (mode == 1u) {    // volume: singular values of F, largest first
+            m3 U, V;
+            f3 sv;
+            svd3(F, U, sv, V);
+            float s0 = sv.x, s1 = sv.y, s2 = sv.z, t;
+            if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+            if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
+            if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+            col = make_float4((1.0f - s0) / 0.005f + 0.2f, (1.0f - s1) / 0.005f + 0.2f, (1.0f - s2) / 0.005f + 0.2f,
+                              base.w);
+        } else if (mode == 3u) {    // phase
+            const float ph = phase[id];
+            col = make_float4(0.0f, 0.4f * ph, 0.4f * (1.0f - ph), base.w);
+        } else if (mode == 4u) {    // contact normals
+            const f3 n = aff ? mk3(a.plane(PNX)[slot], a.plane(PNY)[slot], a.plane(PNZ)[slot]) : mk3(0, 0, 0);
+            col = n.x == 0.0f && n.y == 0.0f && n.z == 0.0f
+                      ? make_float4(0.0f, 0.0f, 0.0f, base.w)
+                      : make_float4((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f, base.w);
+        } else if (mode == 5u) {    // contact signed distances
+            const float sd = aff ? a.plane(PSD)[slot] : 0.0f;
+            const float dd = sd / (h * 1.5f);
+            col = dd > 0.0f ? make_float4(0.0f, fabsf(dd), 0.0f, base.w) : make_float4(fabsf(dd), 0.0f, 0.0f, base.w);
+        } else if (mode == 6u) {    // contact sides
+            const uint32_t inside = (aff >> 16) & (aff & 0xFFFFu);
+            col = aff == 0u ? make_float4(0.0f, 0.0f, 0.0f, base.w)
+                            : (inside == 0u ? make_float4(0.0f, 1.0f, 0.0f, base.w)
+                                            : make_float4(1.0f, 0.0f, 0.0f, base.w));
+        }
+        if (!(fl & FLAG_ENABLED)) col = make_float4(1.0f, 0.0f, 0.0f, 1.0f);   // failed particles in red
+        o[0] = col.x; o[1] = col.y; o[2] = col.z; o[3] = col.w;
+        o[20] = a.plane(PX)[slot]; o[21] = a.plane(PY)[slot]; o[22] = a.plane(PZ)[slot]; o[23] = 0.0f;
+    }
+}
+
+void launch_prep_readback(const Arena &a, uint32_t start, uint32_t count, const uint32_t *group, const float *phase,
+                          const float *radius, const float4 *palette, uint32_t npal, uint32_t mode, float h, float dt,
+                          bool cdf, float *out, int cap, hipStream_t st)
+{
+    if (count == 0) return;
+    const int T = 256;
+    k_prep_readback<<<grid_for(count, T, cap), T, 0, st>>>(a, start, count, group, phase, radius, palette, npal, mode,
+                                                          h, dt, cdf ? 1u : 0u, out);
+}
+
 }  // namespace nxm

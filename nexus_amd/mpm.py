@@ -45,6 +45,7 @@ EXPORTS = [
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
     "nxm_scene_stage_upload", "nxm_scene_set_trimesh_colliders", "nxm_sample_trimesh",
+    "nxm_scene_prep_readback", "nxm_scene_prep_readback_rigid",
     "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
@@ -136,6 +137,8 @@ def library() -> ct.CDLL:
     L.nxm_scene_add_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_set_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_set_trimesh_colliders.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_prep_readback.argtypes = [P, ct.c_uint32, P, ct.c_uint32, P, ct.c_size_t]
+    L.nxm_scene_prep_readback_rigid.argtypes = [P, P, ct.c_uint32, P, ct.c_size_t, ct.POINTER(ct.c_size_t)]
     L.nxm_sample_trimesh.argtypes = [P, ct.c_uint32, P, ct.c_uint32, ct.c_float, P, P, ct.c_size_t,
                                      ct.POINTER(ct.c_size_t)]
     L.nxm_scene_len.argtypes = [P]
@@ -332,6 +335,33 @@ class MpmState:
         self.num_cuboids = int(c.size)
         self.num_colliders = self.num_cuboids + int(getattr(self, "num_meshes", 0))
 
+    def prep_readback(self, mode: int = 0, group_colors=None) -> dict:
+        """WgPrepReadback: per particle (id order) render colour (n, 4), deformation (n, 4, 4)
+        column-major, and position (n, 3)."""
+        n = len(self)
+        out = np.zeros((max(n, 1), 24), np.float32)
+        pal = None if group_colors is None else np.ascontiguousarray(group_colors, np.float32).reshape(-1, 4)
+        _check(self.lib.nxm_scene_prep_readback(self.handle, int(mode), _ptr(pal) if pal is not None else None,
+                                                0 if pal is None else pal.shape[0], _ptr(out), n))
+        out = out[:n]
+        return {"color": out[:, 0:4], "deformation": out[:, 4:20].reshape(n, 4, 4).transpose(0, 2, 1),
+                "position": out[:, 20:23]}
+
+    def prep_readback_rigid(self, group_colors=None) -> dict:
+        """The mesh colliders' rigid particles as render data (see prep_readback)."""
+        pal = None if group_colors is None else np.ascontiguousarray(group_colors, np.float32).reshape(-1, 4)
+        cnt = ct.c_size_t(0)
+        _check(self.lib.nxm_scene_prep_readback_rigid(self.handle, _ptr(pal) if pal is not None else None,
+                                                      0 if pal is None else pal.shape[0], None, 0, ct.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 24), np.float32)
+        _check(self.lib.nxm_scene_prep_readback_rigid(self.handle, _ptr(pal) if pal is not None else None,
+                                                      0 if pal is None else pal.shape[0], _ptr(out), cnt.value,
+                                                      ct.byref(cnt)))
+        out = out[:cnt.value]
+        k = cnt.value
+        return {"color": out[:, 0:4], "deformation": out[:, 4:20].reshape(k, 4, 4).transpose(0, 2, 1),
+                "position": out[:, 20:23]}
+
     def set_trimesh_colliders(self, meshes) -> None:
         """Mesh colliders (scenes.TriMesh) of a CPIC scene; their ids follow the cuboids'."""
         keep = []
@@ -398,6 +428,10 @@ class MpmState:
         bf = getattr(scene, "boundary_friction", None)
         if bf is not None:
             arrs["boundary_friction"] = np.ascontiguousarray(bf, np.float32)
+        if getattr(scene, "group_id", None) is not None:
+            arrs["group_id"] = np.ascontiguousarray(scene.group_id, np.uint32)
+        if getattr(scene, "phase", None) is not None:
+            arrs["phase"] = np.ascontiguousarray(scene.phase, np.float32)
         soa = _SoA(**{k: _ptr(v) for k, v in arrs.items()})
         _check(self.lib.nxm_scene_add_particles_soa(self.handle, ct.byref(soa), scene.n))
 
