@@ -363,3 +363,38 @@ def test_moving_colliders_survive_batch_rebuilds(pipeline):
     for f in ("x", "v", "C", "F", "enabled"):
         assert np.array_equal(a[f], b[f]), f
     batch.close()
+
+
+def test_particle_chunks_on_the_device(pipeline):
+    """NexusState's particle chunks (src/state.rs:665-774) over a resident scene: chunks
+    added in separate regions, one removed whole and one in part between substeps; every
+    remaining particle's chunk (followed through the library's swap-remove moves) is the
+    region it lies in, and the counts add up."""
+    from nexus_amd.chunks import ParticleChunks
+    h = 1.0 / 64
+    model = scenes.model_neo_hookean(1e5, 0.3)
+
+    def block(x0, seed, n_side=6):
+        rng = np.random.default_rng(seed)
+        pos = np.array([x0, 0.5, 0.5]) + rng.random((n_side ** 3, 3)) * 0.06
+        return scenes.make_scene(scenes.particles(pos, h / 4, 1000.0, model), h, 1e-4)
+
+    base = block(0.1, 1)
+    st = mpm.MpmState.from_scene(pipeline, base)
+    ch = ParticleChunks(st)
+    a = ch.add_particles(block(0.3, 2))
+    b = ch.add_particles(block(0.5, 3))
+    pipeline.step(st)
+    ch.extend_chunk(a, block(0.3, 4, 4))
+    c = ch.add_particles(block(0.7, 5))
+    pipeline.step(st)
+    ch.remove_chunk(b)
+    assert ch.remove_particles_from_chunk(a, 50) == 50
+    pipeline.step(st)
+    x = st.read_particles()["x"]
+    region = {-1: 0.1, a: 0.3, c: 0.7}
+    assert len(st) == base.n + 216 + 64 - 50 + 216 == len(ch.slot2chunk)
+    for slot, chunk in enumerate(ch.slot2chunk):
+        assert chunk in region and region[chunk] - 0.01 < x[slot, 0] < region[chunk] + 0.07, (slot, chunk)
+    assert ch.sizes == {a: 216 + 64 - 50, c: 216}
+    st.close()
