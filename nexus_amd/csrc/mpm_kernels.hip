@@ -592,10 +592,17 @@ __device__ inline float wave_max(float v)
 // (only for suspect particles, so the common path carries none of the tests).
 // Words of the non-finite mask (one bit per tile word); the word after them flags any.
 // With CPIC the tile holds a second (incompatible) momentum and mass after the first.
-template <bool CPIC> constexpr int tile_fields() { return CPIC ? 8 : 4; }
+// The tile holds 4 fixed-point fields per node (momentum, mass).  CPIC blocks with
+// particles that carry affinity bits fill it twice: the compatible contributions, flushed
+// to the staging tiles, then the incompatible ones, flushed to the second staging tiles.
+template <bool CPIC> constexpr int tile_fields() { return 4; }
 template <bool CPIC> constexpr int bad_any() { return (tile_fields<CPIC>() * TILE_NODES + 31) / 32; }
 
-template <bool CHECK, bool CPIC>
+// Which contributions a scatter pass adds: all (no affinity in the block), the compatible
+// ones, or the incompatible ones.
+constexpr int PART_ALL = 0, PART_COMPATIBLE = 1, PART_INCOMPATIBLE = 2;
+
+template <bool CHECK, bool CPIC, int PART>
 __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int rot,
                                             const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wxy)[3][3],
                                             float wz0, float wz1, float wz2, float smom, float msm,
@@ -622,8 +629,12 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
                 const float mw = msm * wgt;
                 const int tn = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
                 // CPIC (p2g.rs:268-302): a particle not compatible with the node adds to
-                // the node's incompatible fields.
-                const int t = CPIC && (pa & 0xFFFFu) != 0u && !aff_compatible(pa, naff[tn]) ? tn + 4 * TILE_NODES : tn;
+                // the node's incompatible fields (the second pass of the block).
+                if (PART != PART_ALL) {
+                    const bool incomp = (pa & 0xFFFFu) != 0u && !aff_compatible(pa, naff[tn]);
+                    if (incomp != (PART == PART_INCOMPATIBLE)) continue;
+                }
+                const int t = tn;
                 atomicAdd(&tile[t], Fixed::to(vc.x, 1.0f));
                 atomicAdd(&tile[TILE_NODES + t], Fixed::to(vc.y, 1.0f));
                 atomicAdd(&tile[2 * TILE_NODES + t], Fixed::to(vc.z, 1.0f));
@@ -640,46 +651,43 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
     }
 }
 
-// Flush: after the block's scatter, node t of the 6^3 tile goes to the region of the block
-// it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
-// block's node order, so the consumer reads each region as one run of float4.  Ends with
-// the tile free for the next block.
-// With CPIC, inc (wave-uniform) says whether the block has particles with affinity bits:
-// only then does the incompatible half of the tile hold anything, and only then is it
-// written (the grid update reads w.binc before it).
-template <bool CPIC>
-__device__ __forceinline__ void p2g_flush(const WorldDev &w, const unsigned long long *tile, const uint32_t *bad,
-                                          uint32_t d, float smom, float sm, uint32_t l, bool inc)
+// Flush: after the block's scatter pass, node t of the 6^3 tile goes to the region of the
+// block it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
+// block's node order in the staging array st, so the consumer reads each region as one run
+// of float4.  Ends with the tile free for the next pass.
+__device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const uint32_t *bad, float4 *stage,
+                                          uint32_t d, float smom, float sm, uint32_t l)
 {
     __syncthreads();
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
-    const bool any_bad = bad[bad_any<CPIC>()] != 0u;
-    if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
-    #pragma unroll
-    for (int f = 0; f < (CPIC ? 2 : 1); ++f) {
-        if (f == 1 && !inc) break;
-        float4 *st = reinterpret_cast<float4 *>(f ? w.stage_i : w.stage) + (size_t)d * TILE_NODES;
-        for (int t = l; t < TILE_NODES; t += WAVE) {
-            const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
-            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-            const int e = 4 * ex + 2 * ey + ez;
-            const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
-            const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
-            float v4[4];
+    const bool any_bad = bad[bad_any<false>()] != 0u;
+    float4 *st = stage + (size_t)d * TILE_NODES;
+    for (int t = l; t < TILE_NODES; t += WAVE) {
+        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+        const int e = 4 * ex + 2 * ey + ez;
+        const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+        const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+        float v4[4];
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) v4[c] = Fixed::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
+        if (any_bad) {
             #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                v4[c] = Fixed::back(tile[(4 * f + c) * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
-            if (any_bad) {
-                #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int k = (4 * f + c) * TILE_NODES + t;
-                    if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
-                }
+            for (int c = 0; c < 4; ++c) {
+                const int k = c * TILE_NODES + t;
+                if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
             }
-            st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
         }
+        st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
     }
     __syncthreads();
+}
+
+// Empties the tile and its non-finite marks for the next scatter pass.
+__device__ __forceinline__ void p2g_clear(unsigned long long *tile, uint32_t *bad, uint32_t l)
+{
+    for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
+    if (l <= (uint32_t)bad_any<false>()) bad[l] = 0u;
 }
 
 // One particle's P2G inputs.
@@ -713,7 +721,7 @@ __device__ inline void p2g_bound(const P2GIn &p, float h, float &bmom, float &bm
 }
 
 // Scatter of one enabled particle of block vid into the tile.
-template <bool CPIC>
+template <bool CPIC, int PART>
 __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
                                         const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t l,
                                         const uint32_t *naff, uint32_t pa)
@@ -769,9 +777,9 @@ __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad,
     const int rot = (int)(l % 3u);
     if (inside) {
         if (suspect)
-            p2g_scatter<true, CPIC>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
+            p2g_scatter<true, CPIC, PART>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
         else
-            p2g_scatter<false, CPIC>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
+            p2g_scatter<false, CPIC, PART>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
     }
 }
 
@@ -788,7 +796,7 @@ __device__ inline void load_tile_affinity(const WorldDev &w, uint32_t d, bool cp
 }
 
 template <bool CPIC>
-__global__ void __launch_bounds__(64) k_p2g(WorldDev w)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_p2g(WorldDev w)
 {
     using FX = Fixed;
     constexpr int NF = tile_fields<CPIC>();
@@ -829,12 +837,23 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
             const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
             const bool inc = CPIC && __ballot((pa & 0xFFFFu) != 0u) != 0ull;
-            for (int k = l; k < (inc ? 8 : 4) * TILE_NODES; k += WAVE) tile[k] = 0;
-            if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
+            p2g_clear(tile, bad, l);
             if (inc) load_tile_affinity(w, d, true, naff, l);
             __syncthreads();
-            if (on) p2g_one<CPIC>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
-            p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l, inc);
+            if (on) {
+                if (inc) p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+                else p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+            }
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
+            if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
+            if (CPIC && inc) {
+                // Second pass: the incompatible contributions of the particles with affinity.
+                p2g_clear(tile, bad, l);
+                __syncthreads();
+                if (on && (pa & 0xFFFFu) != 0u)
+                    p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
+            }
             continue;
         }
         // Pass 1: bounds over the block; pass 2: scatter, 64 particles at a time.
@@ -854,8 +873,7 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
         const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
         const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
         const bool inc = CPIC && __ballot(anyc) != 0ull;
-        for (int k = l; k < (inc ? 8 : 4) * TILE_NODES; k += WAVE) tile[k] = 0;
-        if (l <= (uint32_t)bad_any<CPIC>()) bad[l] = 0u;
+        p2g_clear(tile, bad, l);
         if (inc) load_tile_affinity(w, d, true, naff, l);
         __syncthreads();
         for (uint32_t c = 0; c < np; c += WAVE) {
@@ -863,12 +881,32 @@ __global__ void __launch_bounds__(64) k_p2g(WorldDev w)
             if (pos < first + np) {
                 const uint32_t slot = w.perm[pos];
                 if (FL[slot] & FLAG_ENABLED) {
-                    const uint32_t pa = inc ? w.out.uplane(PAFF)[pos] : 0u;
-                    p2g_one<CPIC>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
+                    if (inc) {
+                        const uint32_t pa = w.out.uplane(PAFF)[pos];
+                        p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
+                    } else {
+                        p2g_one<CPIC, PART_ALL>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, 0u);
+                    }
                 }
             }
         }
-        p2g_flush<CPIC>(w, tile, bad, d, smom, sm, l, inc);
+        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
+        if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
+        if (CPIC && inc) {
+            // Second pass: the incompatible contributions of the particles with affinity.
+            p2g_clear(tile, bad, l);
+            __syncthreads();
+            for (uint32_t c = 0; c < np; c += WAVE) {
+                const uint32_t pos = first + c + l;
+                if (pos < first + np) {
+                    const uint32_t slot = w.perm[pos];
+                    const uint32_t pa = w.out.uplane(PAFF)[pos];
+                    if ((FL[slot] & FLAG_ENABLED) && (pa & 0xFFFFu) != 0u)
+                        p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
+                }
+            }
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
+        }
     }
 }
 
