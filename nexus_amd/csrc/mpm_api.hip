@@ -372,6 +372,12 @@ class World {
                 (void)hipHostFree(c.p);
             }
             for (auto &e : ring_) (void)hipEventDestroy(e);
+            if (side_) {
+                (void)hipStreamSynchronize(side_);
+                (void)hipEventDestroy(fork_);
+                (void)hipEventDestroy(join_);
+                (void)hipStreamDestroy(side_);
+            }
             (void)hipStreamDestroy(stream_);
         }
     }
@@ -382,6 +388,9 @@ class World {
         cfg_ = default_config();
         HIP_TRY(p2g_start_.reserve(max_p2g_groups() + 1));
         HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
         for (auto &e : ev_) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(g_.reserve(1));
         HIP_TRY(hipMemsetAsync(g_.p, 0, sizeof(Globals), stream_));
@@ -636,6 +645,9 @@ class World {
         lc.p_hint = p_in_;
         lc.persistent_blocks = (int)cfg.grid_groups;
         lc.p2g_blocks = (int)cfg.p2g_groups;
+        lc.side = side_;
+        lc.fork = fork_;
+        lc.join = join_;
         hipEvent_t *evs = times ? ev_ : nullptr;
         if (!times && profiling_ && profile_seen_++ % profile_every_ == 0) {
             nxm_status st = ring_events(&evs);
@@ -1401,6 +1413,8 @@ class World {
 
     nxm_ctx *ctx_;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;       // CPIC: the contact particles' G2P (default priority)
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
     nxm_launch_config cfg_{};
     bool tune_on_ = false, tuning_ = false;
     int tune_stage_ = 0, tune_rep_ = 0;

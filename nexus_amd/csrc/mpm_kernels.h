@@ -66,6 +66,10 @@ struct LaunchConfig {
     uint64_t p_hint;               // upper bound of particles (grid sizing only)
     int persistent_blocks;         // cap on workgroups of grid-stride kernels
     int p2g_blocks;                // workgroups of the per-block kernels
+    // CPIC worlds: a second stream (default priority) on which the contact particles' G2P
+    // runs beside the plain particles' one, forked and joined with these two events.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // keys: run k_keys_touch first (the hash and counters emptied by launch_clear); without

@@ -1794,8 +1794,20 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (ev) (void)hipEventRecord(ev[2], st);
         k_grid_update<true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
+        // The contact particles' G2P touches other particles than the plain one and is
+        // latency-bound on few waves: it runs beside it on the side stream when there is one.
+        hipStream_t cs = st;
+        if (lc.side && lc.fork && lc.join) {
+            (void)hipEventRecord(lc.fork, st);
+            (void)hipStreamWaitEvent(lc.side, lc.fork, 0);
+            cs = lc.side;
+        }
+        k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
         k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
-        k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        if (cs != st) {
+            (void)hipEventRecord(lc.join, cs);
+            (void)hipStreamWaitEvent(st, lc.join, 0);
+        }
         k_integrate_colliders<<<std::max(1, (int)((w.s_count * 16u + T - 1) / T)), T, 0, st>>>(w);
     } else {
         k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
