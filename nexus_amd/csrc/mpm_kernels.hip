@@ -1350,9 +1350,8 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
         uint32_t s = 0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
         if (valid) {
-            const uint32_t d = w.sblock[pos];
-            s = w.bscene[d];
-            valid = !(CPIC && contact_particle(w, pos, d));   // keyed by k_g2p_contact
+            s = flags_scene(w.out.uplane(PFLAGS)[pos]);
+            if (CPIC) valid = !contact_particle(w, pos, w.sblock[pos]);   // keyed by k_g2p_contact
             x = w.out.plane(PX)[pos];
             y = w.out.plane(PY)[pos];
             z = w.out.plane(PZ)[pos];
