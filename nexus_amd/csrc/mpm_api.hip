@@ -388,9 +388,6 @@ class World {
         cfg_ = default_config();
         HIP_TRY(p2g_start_.reserve(max_p2g_groups() + 1));
         HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
         for (auto &e : ev_) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(g_.reserve(1));
         HIP_TRY(hipMemsetAsync(g_.p, 0, sizeof(Globals), stream_));
@@ -1193,6 +1190,13 @@ class World {
             HIP_TRY(gridv_i_.reserve((size_t)NODES_PER_BLOCK * nb));
         }
         any_cpic_ = any_cpic;
+        if (any_cpic && !side_) {
+            // Created with the first CPIC scene only: streams are spread over the device's
+            // hardware queues in creation order, and worlds without CPIC keep one stream each.
+            HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+        }
         if (any_mesh) {
             // k_grid_cdf empties every node k_mesh_cdf wrote in the same substep; fresh
             // allocations are emptied here, outside any captured graph.
