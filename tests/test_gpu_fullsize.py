@@ -6,6 +6,10 @@
 - Drucker-Prager sand column, 1,048,576 particles, and clumped fluid, 8,388,608 particles:
   one substep against the oracle, the sort bit-exact in canonical form and x, v, C, F
   within the parity bar.
+- The same two workloads over many substeps (the block keys then come from the previous
+  substep's G2P): two runs are bit-identical, and so is the run of the scene inside a batch
+  beside another scene (P2G sums in exact fixed point, every other stage works per particle
+  or per node, so neither the atomics' order nor the batch changes a bit).
 """
 import numpy as np
 import pytest
@@ -52,3 +56,31 @@ def test_one_substep_full_size(pipeline, make):
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
         assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+
+
+@pytest.mark.parametrize("make,substeps", [(scenes.sand_column, 20), (scenes.fluid_clumps, 4)],
+                         ids=["sand_1M", "fluid_8M"])
+def test_full_size_runs_are_bit_identical(pipeline, make, substeps):
+    sc = make()
+    runs = []
+    for _ in range(2):
+        st = mpm.MpmState.from_scene(pipeline, sc)
+        for _ in range(substeps):
+            pipeline.step(st)
+        runs.append(st.read_particles())
+        st.close()
+    other = scenes.elastic_cube(12)
+    batch = mpm.Batch(pipeline)
+    states = [mpm.MpmState.from_scene(pipeline, s) for s in (other, sc)]
+    for s in states:
+        batch.add(s)
+    batch.run(substeps)
+    batch.synchronize()
+    runs.append(states[1].read_particles())
+    batch.close()
+    for s in states:
+        s.close()
+    assert runs[0]["enabled"].sum() > 0.99 * sc.n
+    for got in runs[1:]:
+        for f in ("x", "v", "C", "F", "enabled"):
+            assert np.array_equal(got[f], runs[0][f]), f
