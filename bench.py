@@ -289,7 +289,7 @@ def main(argv=None):
 
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (scenes shard across GPUs "
-              f"without communication; see DESIGN.md)", file=sys.stderr)
+              f"without communication, one process per GPU, so N GPUs step N times the scenes)", file=sys.stderr)
         return 2
 
     from nexus_amd import mpm
