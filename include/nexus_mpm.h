@@ -338,6 +338,12 @@ nxm_status nxm_scene_synchronize(nxm_scene *scene);
 /* Keep the per-node momentum after P2G for nxm_scene_read_grid (costs one extra
  * write per node per step).  No counterpart in the reference: a test hook. */
 nxm_status nxm_scene_set_debug_grid(nxm_scene *scene, int enabled);
+/* Test hook, no counterpart in the reference: the next `count` particle uploads into a
+ * world (a scene's first step, joining a batch, particles appended to a resident scene)
+ * fail with NXM_ERR_DEVICE before any copy is queued, as a failed device copy would.  The
+ * scene is then left on the host, detached, and every object can still be stepped and
+ * destroyed in any order.  0 ends it. */
+nxm_status nxm_debug_fail_uploads(uint32_t count);
 
 /* ---- pipeline step (MpmPipeline::step, src_mpm/pipeline.rs:385) ---- */
 /* Enqueues one substep of a standalone scene.  times != NULL synchronizes and

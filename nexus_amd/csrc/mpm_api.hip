@@ -28,6 +28,9 @@
 namespace nxm {
 
 static thread_local std::string g_last_error;
+// Uploads still to fail on purpose (nxm_debug_fail_uploads): reaches the error paths of a
+// world's bookkeeping from the host.
+static std::atomic<int> g_fail_uploads{0};
 
 static nxm_status fail(nxm_status code, const std::string &msg)
 {
@@ -429,9 +432,23 @@ class World {
         sc->ranges.clear();
         dirty_ = true;
         nxm_status st = upload(sc, 0, sc->hp.n);
-        if (st != NXM_OK) return st;
+        if (st != NXM_OK) {
+            // Nothing of the scene entered the arena (the particle count advances only
+            // after its copies are queued): the slot is freed and the scene stays on the
+            // host, detached, so that no table, step or destroy reaches it through here.
+            slots_[slot].scene = nullptr;
+            slots_[slot].state = SLOT_FREE;
+            sc->world = nullptr;
+            sc->ranges.clear();
+            return st;
+        }
         sc->on_host = false;
         return NXM_OK;
+    }
+
+    bool holds(const nxm_scene *sc) const
+    {
+        return sc->world == this && sc->slot < slots_.size() && slots_[sc->slot].scene == sc;
     }
 
     // Appends particles [first, first + count) of sc's host arrays to the input arena of
@@ -440,6 +457,8 @@ class World {
     nxm_status upload(nxm_scene *sc, size_t first, size_t count)
     {
         if (count == 0) return NXM_OK;
+        if (g_fail_uploads.load() > 0 && g_fail_uploads.fetch_sub(1) > 0)
+            return fail(NXM_ERR_DEVICE, "injected upload failure (nxm_debug_fail_uploads)");
         clean_ = false;   // the new particles have no keys yet
         if (p_in_ + count > p_cap_) {
             nxm_status st = sync();   // the arenas are reallocated: nothing may use them
@@ -1571,8 +1590,10 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
             bfr = a.boundary_friction; grp = a.group_id; en = a.enabled; fx = a.fixed; tag = a.model_tag;
             words = a.model_data;
         } else {
-            if (!p->position || !p->init_volume || !p->mass || !p->model_tag || !p->model_data)
+            if (!p->position || !p->init_volume || !p->mass || !p->model_tag || !p->model_data) {
+                hp.resize(base);
                 return fail(NXM_ERR_INVALID_ARGUMENT, "position, init_volume, mass, model_tag and model_data are required");
+            }
             std::memcpy(pos, p->position + 3 * k, 12);
             if (p->velocity) std::memcpy(vel, p->velocity + 3 * k, 12);
             if (p->def_grad) std::memcpy(F, p->def_grad + 9 * k, 36);
@@ -1590,7 +1611,10 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
             words = p->model_data + 24 * k;
         }
         // The reference's model constructors only produce tags 0-5 (particle_model.rs:282-325).
-        if (tag > NXM_MODEL_SNOW) return fail(NXM_ERR_UNSUPPORTED_MODEL, "unknown model tag");
+        if (tag > NXM_MODEL_SNOW) {
+            hp.resize(base);   // the scene keeps the particles it had
+            return fail(NXM_ERR_UNSUPPORTED_MODEL, "unknown model tag");
+        }
         std::memcpy(&hp.x[3 * i], pos, 12);
         std::memcpy(&hp.v[3 * i], vel, 12);
         std::memcpy(&hp.F[9 * i], F, 36);
@@ -1611,9 +1635,17 @@ static nxm_status append_host(nxm_scene *sc, const nxm_particles_soa *p, const n
         hp.aff[i] = 0u;
         hp.cdf[4 * i] = hp.cdf[4 * i + 1] = hp.cdf[4 * i + 2] = hp.cdf[4 * i + 3] = 0.0f;
         nxm_status st = intern_class(sc, tag, words, damp, bfr, &hp.cls[i]);
-        if (st != NXM_OK) return st;
+        if (st != NXM_OK) {
+            hp.resize(base);
+            return st;
+        }
     }
-    if (sc->world) return sc->world->upload(sc, base, n);
+    if (sc->world) {
+        nxm_status st = sc->world->upload(sc, base, n);
+        // Not uploaded: the scene keeps the particles it had, on both sides.
+        if (st != NXM_OK) hp.resize(base);
+        return st;
+    }
     return NXM_OK;
 }
 
@@ -1683,8 +1715,11 @@ void nxm_scene_destroy(nxm_scene *sc)
     if (sc->batch) {
         auto &m = sc->batch->members;
         m.erase(std::remove(m.begin(), m.end(), sc), m.end());
-        if (sc->world) sc->world->drop(sc);   // the state is discarded with the scene
     }
+    // Resident in a world it does not own (a batch's): the slot is dropped, the state
+    // discarded with the scene.  Its own world goes with it below.
+    if (sc->world && sc->world != sc->own.get() && sc->world->holds(sc)) sc->world->drop(sc);
+    sc->world = nullptr;
     sc->own.reset();
     nxm_ctx *ctx = sc->ctx;
     if (sc->staged_chunk >= 0) (void)ctx->pool.give(sc->staged_chunk, nullptr);
@@ -2133,6 +2168,12 @@ nxm_status nxm_scene_read_cdf(nxm_scene *sc, uint32_t *affinity, float *normal, 
     return latched(sc, NXM_OK);
 }
 
+nxm_status nxm_debug_fail_uploads(uint32_t count)
+{
+    g_fail_uploads.store((int)std::min<uint32_t>(count, 1u << 30));
+    return NXM_OK;
+}
+
 nxm_status nxm_scene_set_debug_grid(nxm_scene *sc, int enabled)
 {
     if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
@@ -2238,8 +2279,12 @@ nxm_status nxm_batch_add_scene(nxm_batch *b, nxm_scene *sc)
 nxm_status nxm_batch_remove_scene(nxm_batch *b, nxm_scene *sc)
 {
     if (!b || !sc || sc->batch != b) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not in this batch");
-    nxm_status st = b->world->remove(sc);
-    if (st != NXM_OK) return st;
+    // A member whose re-entry failed (nxm_scene_swap_remove, nxm_scene_set_cpic) is on the
+    // host already and holds no slot.
+    if (b->world->holds(sc)) {
+        nxm_status st = b->world->remove(sc);
+        if (st != NXM_OK) return st;
+    }
     sc->batch = nullptr;
     auto &m = b->members;
     m.erase(std::remove(m.begin(), m.end(), sc), m.end());

@@ -43,6 +43,7 @@ EXPORTS = [
     "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
+    "nxm_debug_fail_uploads",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
     "nxm_scene_stage_upload", "nxm_scene_set_trimesh_colliders", "nxm_sample_trimesh",
     "nxm_scene_prep_readback", "nxm_scene_prep_readback_rigid",
@@ -164,6 +165,7 @@ def library() -> ct.CDLL:
     L.nxm_scene_read_grid.argtypes = [P, P, P, P]
     L.nxm_scene_synchronize.argtypes = [P]
     L.nxm_scene_set_debug_grid.argtypes = [P, I]
+    L.nxm_debug_fail_uploads.argtypes = [ct.c_uint32]
     L.nxm_scene_set_cpic.argtypes = [P, I]
     L.nxm_scene_stage_upload.argtypes = [P]
     L.nxm_scene_read_colliders.argtypes = [P, P, ct.c_size_t]
@@ -188,6 +190,12 @@ def library() -> ct.CDLL:
         getattr(L, f).argtypes = [P, P, ct.POINTER(ct.c_uint32)]
     _LIB = L
     return L
+
+
+def debug_fail_uploads(count: int) -> None:
+    """Test hook: the next `count` particle uploads into a world fail with NXM_ERR_DEVICE
+    before any copy (nxm_debug_fail_uploads); 0 ends it."""
+    _check(library().nxm_debug_fail_uploads(int(count)))
 
 
 def _check(code: int) -> None:

@@ -1020,6 +1020,68 @@ static v3 body_point_velocity(const orc_cuboid *c, v3 pt)
     return vadd(lin, vcross(ang, vsub(pt, com)));
 }
 
+/* gpu_particle_update for one particle (solver/particle_update.rs:60-201): from the G2P
+ * velocity gv, velocity gradient gg and divergence div, and the contact field (signed
+ * distance psd, normal pn, the bodies' velocity prv; psd = 0 outside CPIC).  Reads and
+ * writes x, v, C, F, the plastic state ps (NULL for tags without one), mass and enabled.
+ * Returns nonzero for an unknown model tag. */
+static int particle_update_one(uint32_t tag, const float *params, REAL h, REAL dt, REAL damping, uint32_t fixed,
+                               REAL vol0, const REAL *gv, const REAL *gg, REAL div, REAL psd, const REAL *pn,
+                               const REAL *prv, REAL *x, REAL *v, REAL *C, REAL *F, REAL *ps, REAL *mass,
+                               uint32_t *enabled)
+{
+    v3 vel = vload(gv);
+    m3 gradv = mload(gg);
+    const REAL inv_d = qk_inv_d(h);
+    /* CPIC reprojection of a penetrating particle (particle_update.rs:92-102) */
+    const int penetrating = psd < K(-0.05) * h;
+    if (penetrating) {
+        v3 rv = vload(prv);
+        vel = vadd(rv, bc_project(1u, K(0.0), vsub(vel, rv), vload(pn)));
+    }
+    REAL vl = vlen(vel);
+    if (vl > h / dt) vel = vdiv(vmul(vdiv(vel, vl), h), dt);
+    vel = vdiv(vel, K(1.0) + damping * dt);
+    if (fixed) vel = V3(0, 0, 0);
+    v3 pos = vload(x);
+    v3 npos = vadd(pos, vmul(vel, dt));
+    if (penetrating) {
+        /* penalty impulse (particle_update.rs:126-136) */
+        REAL corrected = RFMAX(psd, K(-0.3) * h);
+        vel = vadd(vel, vmul(vload(pn), dt * -corrected * K(1.0e3)));
+    }
+    m3 Fm = mload(F);
+    if (tag != 4) {
+        Fm = madd(Fm, mmul(mscale(gradv, dt), Fm));
+    } else {
+        REAL f0 = Fm.c0.x;
+        REAL rate = div * dt / K(3.0);
+        rate = rclamp(rate, K(-0.1), K(0.1));
+        REAL nd = rclamp(f0 + rate * f0, K(0.5), K(2.0));
+        Fm = mdiag(V3(nd, nd, nd));
+    }
+    REAL st[3] = {0, 0, 0};
+    if (ps) { st[0] = ps[0]; st[1] = ps[1]; st[2] = ps[2]; }
+    m3 tau;
+    const int bad = model_update(tag, params, st, &Fm, gradv, &tau) != 0;
+    if (ps) { ps[0] = st[0]; ps[1] = st[1]; ps[2] = st[2]; }
+    m3 aff = msub(mscale(gradv, *mass), mscale(tau, vol0 * inv_d * dt));
+    int nan_pos = npos.x != npos.x || npos.y != npos.y || npos.z != npos.z;
+    if (!nan_pos && mdet(Fm) > K(0.0)) {
+        vstore(x, npos);
+        mstore(C, aff);
+    } else {
+        *enabled = 0;
+        vel = V3(0, 0, 0);
+        Fm = mident();
+        mstore(C, mzero());
+        *mass = K(0.0);
+    }
+    vstore(v, vel);
+    mstore(F, Fm);
+    return bad;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Workspace. */
 
@@ -1477,55 +1539,15 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
     int bad_model = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad_model)
     for (int64_t p = 0; p < n; ++p) {
-        v3 vel = vload(pv + 3 * p);
-        m3 gradv = mload(pg + 9 * p);
-        /* CPIC reprojection of a penetrating particle (particle_update.rs:92-102) */
-        const int penetrating = cp && cp->signed_distance[p] < K(-0.05) * h;
-        if (penetrating) {
-            v3 rv = vload(cp->rigid_vel + 3 * p);
-            vel = vadd(rv, bc_project(1u, K(0.0), vsub(vel, rv), vload(cp->normal + 3 * p)));
-        }
-        REAL vl = vlen(vel);
-        if (vl > h / dt) vel = vdiv(vmul(vdiv(vel, vl), h), dt);
-        vel = vdiv(vel, K(1.0) + damping[p] * dt);
-        if (fixed[p]) vel = V3(0, 0, 0);
-        v3 pos = vload(x + 3 * p);
-        v3 npos = vadd(pos, vmul(vel, dt));
-        if (penetrating) {
-            /* penalty impulse (particle_update.rs:126-136) */
-            REAL corrected = RFMAX(cp->signed_distance[p], K(-0.3) * h);
-            vel = vadd(vel, vmul(vload(cp->normal + 3 * p), dt * -corrected * K(1.0e3)));
-        }
-        m3 Fm = mload(F + 9 * p);
-        if (tag[p] != 4) {
-            Fm = madd(Fm, mmul(mscale(gradv, dt), Fm));
-        } else {
-            REAL f0 = Fm.c0.x;
-            REAL rate = pdiv[p] * dt / K(3.0);
-            rate = rclamp(rate, K(-0.1), K(0.1));
-            REAL nd = rclamp(f0 + rate * f0, K(0.5), K(2.0));
-            Fm = mdiag(V3(nd, nd, nd));
-        }
-        REAL st[3] = {0, 0, 0};
-        const int has_state = tag[p] == 2 || tag[p] == 3 || tag[p] == 5;
-        if (has_state) { st[0] = pstate[3 * p]; st[1] = pstate[3 * p + 1]; st[2] = pstate[3 * p + 2]; }
-        m3 tau;
-        if (model_update(tag[p], params + 24 * (size_t)p, st, &Fm, gradv, &tau) != 0) bad_model = 1;
-        if (has_state) { pstate[3 * p] = st[0]; pstate[3 * p + 1] = st[1]; pstate[3 * p + 2] = st[2]; }
-        m3 aff = msub(mscale(gradv, mass[p]), mscale(tau, vol0[p] * inv_d * dt));
-        int nan_pos = npos.x != npos.x || npos.y != npos.y || npos.z != npos.z;
-        if (!nan_pos && mdet(Fm) > K(0.0)) {
-            vstore(x + 3 * p, npos);
-            mstore(C + 9 * p, aff);
-        } else {
-            enabled[p] = 0;
-            vel = V3(0, 0, 0);
-            Fm = mident();
-            mstore(C + 9 * p, mzero());
-            mass[p] = K(0.0);
-        }
-        vstore(v + 3 * p, vel);
-        mstore(F + 9 * p, Fm);
+        const REAL zero3[3] = {0, 0, 0};
+        const REAL psd = cp ? cp->signed_distance[p] : K(0.0);
+        const REAL *pn = cp ? cp->normal + 3 * p : zero3;
+        const REAL *prv = cp ? cp->rigid_vel + 3 * p : zero3;
+        REAL *ps = (tag[p] == 2 || tag[p] == 3 || tag[p] == 5) ? pstate + 3 * p : NULL;
+        if (particle_update_one(tag[p], params + 24 * (size_t)p, h, dt, damping[p], fixed[p], vol0[p], pv + 3 * p,
+                                pg + 9 * p, pdiv[p], psd, pn, prv, x + 3 * p, v + 3 * p, C + 9 * p, F + 9 * p, ps,
+                                &mass[p], &enabled[p]) != 0)
+            bad_model = 1;
     }
     if (bad_model) rc = -2;
 
@@ -1592,6 +1614,16 @@ int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F, c
     mstore(F, Fm);
     mstore(tau, t);
     return rc;
+}
+
+int orc_particle_update(uint32_t tag, const float *params, float cell_width, float dt, REAL damping, uint32_t fixed,
+                        REAL vol0, const REAL *gv, const REAL *gradv, REAL div, REAL psd, const REAL *normal,
+                        const REAL *rigid_vel, REAL *x, REAL *v, REAL *C, REAL *F, REAL *pstate, REAL *mass,
+                        uint32_t *enabled)
+{
+    const int has_state = tag == 2 || tag == 3 || tag == 5;
+    return particle_update_one(tag, params, (REAL)cell_width, (REAL)dt, damping, fixed, vol0, gv, gradv, div, psd,
+                               normal, rigid_vel, x, v, C, F, has_state ? pstate : NULL, mass, enabled);
 }
 
 int orc_node_cdf(const orc_cuboid *cols, int ncol, float cell_width, const REAL *pt, REAL *dist, uint32_t *aff,

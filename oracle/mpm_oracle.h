@@ -141,6 +141,15 @@ int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim,
                      const REAL *node_pos, REAL *vel);
 int orc_block_keys(int64_t n, const REAL *x, float cell_width, int32_t *vid,
                    uint32_t *local);
+/* gpu_particle_update for one particle (solver/particle_update.rs:60-201) from its G2P
+ * outputs: velocity gv (3), velocity gradient gradv (9) and divergence div, and its contact
+ * field (signed distance psd, normal and the bodies' velocity rigid_vel, 3 each; psd = 0
+ * without CPIC).  Updates x, v, C, F, pstate (3, sand and snow tags), mass and enabled in
+ * place.  Returns nonzero for an unknown model tag. */
+int orc_particle_update(uint32_t tag, const float *params, float cell_width, float dt, REAL damping, uint32_t fixed,
+                        REAL vol0, const REAL *gv, const REAL *gradv, REAL div, REAL psd, const REAL *normal,
+                        const REAL *rigid_vel, REAL *x, REAL *v, REAL *C, REAL *F, REAL *pstate, REAL *mass,
+                        uint32_t *enabled);
 /* Node CDF at pt (one collider pass of grid_update_cdf); particle CDF from the 27 stencil
  * nodes in NBH_SHIFTS order (refp = base node - particle); 4x4 solve m x = b (m
  * row-major), returning det m. */

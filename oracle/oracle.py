@@ -14,7 +14,9 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BUILD = os.path.join(HERE, "build")
+# ORACLE_BUILD selects another build of the same sources (oracle/Makefile's sanitizer
+# build, build-asan/).
+BUILD = os.path.abspath(os.environ.get("ORACLE_BUILD", os.path.join(HERE, "build")))
 
 _LIBS: dict[int, ct.CDLL] = {}
 
@@ -61,6 +63,10 @@ def lib(bits: int = 32) -> ct.CDLL:
     L.orc_block_keys.argtypes = [ct.c_int64, P, ct.c_float, P, P]
     L.orc_timestep_bound.argtypes = [ct.c_int64] + [P] * 8 + [ct.c_float]
     L.orc_timestep_bound.restype = ct.c_uint32
+    R = ct.c_double if bits == 64 else ct.c_float
+    L.orc_particle_update.restype = ct.c_int
+    L.orc_particle_update.argtypes = [ct.c_uint32, P, ct.c_float, ct.c_float, R, ct.c_uint32, R, P, P, R, R, P, P,
+                                      P, P, P, P, P, P, P]
     _LIBS[bits] = L
     return L
 
@@ -362,3 +368,28 @@ def timestep_bound(state: "OracleState") -> tuple[int, float]:
     u = lib(s.bits).orc_timestep_bound(s.n, _p(s.v), _p(s.C), _p(s.F), _p(s.mass), _p(s.vol0), _p(s.enabled),
                                        _p(s.tag), _p(params), ct.c_float(float(s.sim[4])))
     return int(u), float(np.float32(u) / np.float32(1.0e12))
+
+
+def particle_update(tag: int, words, cell_width: float, dt: float, *, x, gv, gradv=None, div=0.0, F=None, C=None,
+                    mass=1.0, vol0=1.0, damping=0.0, fixed=0, pstate=None, psd=0.0, normal=(0, 0, 0),
+                    rigid_vel=(0, 0, 0), enabled=1, bits: int = 64):
+    """One particle through the oracle's particle update (particle_update.rs:60-201) from
+    given G2P outputs (velocity gv, gradient gradv as a 3x3 array, divergence div) and
+    contact field.  Returns a dict of the updated x, v, C (3x3), F (3x3), pstate, mass,
+    enabled and the model rc."""
+    R = real(bits)
+    params = np.ascontiguousarray(np.asarray(words, np.uint32).view(np.float32), np.float32)
+    cm = lambda M: np.ascontiguousarray(np.asarray(M, R).T.reshape(9))   # column-major
+    xx = np.array(x, R)
+    vv = np.zeros(3, R)
+    CC = cm(np.zeros((3, 3)) if C is None else C)
+    FF = cm(np.eye(3) if F is None else F)
+    g = cm(np.zeros((3, 3)) if gradv is None else gradv)
+    st = np.zeros(3, R) if pstate is None else np.array(pstate, R)
+    m = np.array([mass], R)
+    en = np.array([enabled], np.uint32)
+    rc = lib(bits).orc_particle_update(tag, _p(params), ct.c_float(cell_width), ct.c_float(dt), damping, fixed, vol0,
+                                       _p(np.array(gv, R)), _p(g), div, psd, _p(np.array(normal, R)),
+                                       _p(np.array(rigid_vel, R)), _p(xx), _p(vv), _p(CC), _p(FF), _p(st), _p(m), _p(en))
+    return dict(x=xx, v=vv, C=CC.reshape(3, 3).T, F=FF.reshape(3, 3).T, pstate=st, mass=float(m[0]),
+                enabled=int(en[0]), rc=rc)

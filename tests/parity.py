@@ -1,16 +1,24 @@
 """Shared comparison helpers for the GPU-vs-oracle parity tests."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 import oracle as O
 
-# One FP32 substep must match within 1e-5 relative (norm-wise over the field), or be
-# no further from the FP64 oracle than the reference algorithm run in FP32 is (times
-# a small factor): the GPU result is then exactly as good as the reference's own FP32
-# arithmetic allows.
+# One FP32 substep must match the FP64 oracle within 1e-5 of the field's largest value
+# (max-norm), or be no further from it than the reference algorithm itself run in FP32
+# (the op-for-op FP32 mirror) times a small factor: the GPU result is then as good as the
+# reference's own FP32 arithmetic allows.  The second clause matters for fields whose
+# FP32 floor lies above 1e-5: C on Drucker-Prager sand, where the stress term is a
+# difference of large nearly equal values.
 REL_TOL = 1e-5
-FP32_FACTOR = 8.0
+FP32_FACTOR = 2.0
+
+# Every comparison made in this process: (test, field, |gpu - fp64|, |fp32 mirror - fp64|,
+# scale, clause) -- printed at the end of the session (tests/conftest.py).
+RECORDS: list = []
 
 
 def field_errors(gpu: np.ndarray, o32: np.ndarray, o64: np.ndarray):
@@ -24,10 +32,17 @@ def field_errors(gpu: np.ndarray, o32: np.ndarray, o64: np.ndarray):
 
 
 def assert_field_close(name: str, gpu, o32, o64):
+    """Asserts the one-substep bar and returns (|gpu - fp64|, |fp32 - fp64|, scale, clause):
+    clause "rel" when the 1e-5 relative bound holds, "fp32" when only the mirror bound does."""
     e_gpu, e_32, scale = field_errors(gpu, o32, o64)
-    bound = max(REL_TOL * scale, FP32_FACTOR * e_32, 1e-30)
-    assert e_gpu <= bound, f"{name}: |gpu - fp64| = {e_gpu:.3e} > {bound:.3e} (fp32 mirror {e_32:.3e}, scale {scale:.3e})"
-    return e_gpu, e_32, scale
+    rel_bound = REL_TOL * scale
+    clause = "rel" if e_gpu <= rel_bound else ("fp32" if e_gpu <= FP32_FACTOR * e_32 else "FAIL")
+    test = os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0]
+    RECORDS.append((test, name, e_gpu, e_32, scale, clause))
+    bound = max(rel_bound, FP32_FACTOR * e_32, 1e-30)
+    assert e_gpu <= bound, (f"{name}: |gpu - fp64| = {e_gpu:.3e} > {bound:.3e} (fp32 mirror {e_32:.3e}, "
+                            f"scale {scale:.3e}, relative {e_gpu / max(scale, 1e-30):.2e})")
+    return e_gpu, e_32, scale, clause
 
 
 def oracle_pair(scene, steps: int = 1):
