@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s at 1M FP32 MLS-MPM; batched scenes/s + p50/p99 step latency"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak without packed FP32: one wave64 VALU instruction per CU per cycle (four
+# 16-lane SIMDs, four cycles each), 256 CUs at 2.4 GHz = 614.4 G wave-instructions/s.
+VALU_PEAK_G = 256 * 2.4
 SEEDS = {"cube": 0x4E584D01, "sand": 0x4E584D02, "fluid": 0x4E584D03}
 
 
@@ -60,6 +63,19 @@ def g2p_algorithmic_bytes(n_particles: int, n_blocks: int, plastic_words: int) -
     and written (4 bytes a word: 3 for Drucker-Prager sand), the next substep's block key
     written (4); per node of every active block its velocity read (16 bytes)."""
     return (52 + 96 + 8 * plastic_words + 4) * n_particles + 16 * 64 * n_blocks
+
+
+def finish_tuning(obj, step, limit: int = 400) -> dict:
+    """Steps ``obj`` (a scene or a batch, through ``step``) until its launch-shape tuner
+    has kept a shape, so that no trial shape and no tuner synchronization falls inside a
+    timed region.  Returns the tuner's state for the JSON line."""
+    extra = 0
+    tuning, used = obj.autotune_state()
+    while tuning and extra < limit:
+        step()
+        extra += 1
+        tuning, used = obj.autotune_state()
+    return {"finished": not tuning, "substeps": used, "extra_warmup_substeps": extra}
 
 
 def cpu_baseline(scene, budget_s: float = 15.0, max_steps: int = 30):
@@ -103,6 +119,8 @@ def bench_cpic(pipe, steps: int, warmup: int = 40, autotune: bool = True) -> dic
             st.set_autotune(True)
         for _ in range(warmup):
             pipe.step(st)
+        if autotune:
+            finish_tuning(st, lambda: pipe.step(st))
         st.synchronize()
         st.profile_begin(every=10)
         t0 = time.perf_counter()
@@ -180,7 +198,7 @@ def bench_batched(pipe, count: int, substeps: int, loop_substeps: int) -> dict:
     # autotuned launch shapes (the tuner runs on live substeps), then graph replay
     batch.set_graph(False)
     batch.set_autotune(True)
-    batch.run(40)
+    tuner = finish_tuning(batch, lambda: batch.run(1))
     batch.set_graph(True)
     batch.run(2)
     batch.synchronize()
@@ -207,6 +225,7 @@ def bench_batched(pipe, count: int, substeps: int, loop_substeps: int) -> dict:
         "autotuned_graph_particle_substeps_per_s": rate_tuned,
         "autotuned_launch_config": {"p2g_groups": tuned_cfg[0], "p2g_block_cost": tuned_cfg[1],
                                     "grid_groups": tuned_cfg[2]},
+        "tuner": tuner,
         "stages_ms": {"sort": stages.sort_ms, "p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
                       "g2p_update": stages.g2p_update_ms},
         "plain_loop": {"particle_substeps_per_s": loop_rate, "substeps": loop_substeps,
@@ -235,15 +254,19 @@ def bench_served(pipe, seconds: float, rate: float, budget: float) -> dict:
     return out
 
 
-def traffic_from_profiles(kernel: str, workload: str):
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.json")
+def counters_from_profiles(workload: str) -> dict:
+    """Per-kernel counter figures recorded by rocprofv3 for this workload
+    (profiles/counters_<workload>.json: HBM bytes per launch from FETCH_SIZE / WRITE_SIZE
+    scaled by a copy calibration, VALU wave-instructions per launch from SQ_INSTS_VALU),
+    keyed by kernel name; empty when no profile is committed."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
         with open(path) as f:
-            return float(json.load(f)["bytes_per_launch"])
-    except Exception:
-        return None
+            return json.load(f).get("kernels", {})
+    except (OSError, ValueError):
+        return {}
 
 
 def calibrate_copy(nbytes: int = 1 << 30, reps: int = 4):
@@ -298,9 +321,12 @@ def main(argv=None):
     pipe = mpm.MpmPipeline(0)
     st = mpm.MpmState.from_scene(pipe, scene)
     if not args.no_autotune:
-        st.set_autotune(True)       # the tuner runs on the warm-up substeps (28 of them)
+        st.set_autotune(True)       # the tuner runs on live substeps (28 of them)
     for _ in range(args.warmup):
         pipe.step(st)
+    # Whatever --warmup is, the clock starts only once the tuner has kept a shape.
+    tuner = finish_tuning(st, lambda: pipe.step(st)) if not args.no_autotune else {"finished": False,
+                                                                                   "substeps": 0, "off": True}
     st.synchronize()
 
     # Stage events on one substep in ten: an event record between kernels costs GPU time of
@@ -318,19 +344,28 @@ def main(argv=None):
     nblocks = st.sort_size()
     launch_cfg = st.launch_config()
 
-    # The roofline of the dominant kernel (the longer of P2G and G2P + update, each timed
-    # with the same sampled events), the other beside it.
+    # The roofline of the dominant kernel (the longer of P2G and G2P + update), the other
+    # beside it.  Each kernel's launch time is the average of its own start/end timestamps
+    # over the sampled timed substeps.
     plastic = 3 if args.workload == "sand" else 0
-    figures = {}
-    for kernel, stage_ms, nbytes in (("k_p2g", stages.p2g_ms, p2g_algorithmic_bytes(n, nblocks)),
-                                     ("k_g2p_update", stages.g2p_update_ms,
-                                      g2p_algorithmic_bytes(n, nblocks, plastic))):
-        achieved = nbytes / (stage_ms * 1e-3) / 1e9
+    counters = counters_from_profiles(args.workload)
+    figures, valu = {}, {}
+    for kernel, kname, kernel_ms, stage_ms, nbytes in (
+            ("k_p2g", "k_p2g<false>", stages.p2g_kernel_ms, stages.p2g_ms, p2g_algorithmic_bytes(n, nblocks)),
+            ("k_g2p_update", "k_g2p_update<false>", stages.g2p_update_kernel_ms, stages.g2p_update_ms,
+             g2p_algorithmic_bytes(n, nblocks, plastic))):
+        achieved = nbytes / (kernel_ms * 1e-3) / 1e9
+        prof = counters.get(kname, {})
         figures[kernel] = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": traffic_from_profiles(kernel.replace("k_", "").replace("_update", ""),
-                                                            args.workload),
-                           "algorithmic_bytes": nbytes, "avg_launch_ms": stage_ms, "launches_timed": nprof}
+                           "traffic": prof.get("traffic_bytes"),
+                           "algorithmic_bytes": nbytes, "avg_launch_ms": kernel_ms, "stage_ms": stage_ms,
+                           "launches_timed": nprof}
+        if prof.get("valu_insts"):
+            rate = prof["valu_insts"] / (kernel_ms * 1e-3) / 1e9
+            valu[kernel] = {"kernel": kernel, "bound": "valu", "achieved": rate, "peak": VALU_PEAK_G,
+                            "unit": "G VALU wave-instructions/s", "frac": rate / VALU_PEAK_G,
+                            "valu_insts_per_launch": prof["valu_insts"], "avg_launch_ms": kernel_ms}
     dominant = max(figures, key=lambda k: figures[k]["avg_launch_ms"])
     out = {
         "metric": METRIC,
@@ -349,11 +384,14 @@ def main(argv=None):
                    "colliders": int(scene.colliders.size), "active_blocks": nblocks, "parallelism": "single GPU",
                    "seed": SEEDS[args.workload], "generator": "numpy PCG64 (nexus_amd/scenes.py)"},
         "launch_config": dict(zip(("p2g_groups", "p2g_block_cost", "grid_groups"), launch_cfg)),
+        "tuner": tuner,
         "stages_ms": {"sort": stages.sort_ms, "p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
                       "g2p_update": stages.g2p_update_ms},
         "roofline": figures[dominant],
         "roofline_other": figures[[k for k in figures if k != dominant][0]],
     }
+    if valu:
+        out["roofline_valu"] = valu
     st.close()
     if not args.no_cpic:
         out["cpic"] = bench_cpic(pipe, args.cpic_steps, autotune=not args.no_autotune)

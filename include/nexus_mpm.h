@@ -166,6 +166,10 @@ typedef struct {
     float grid_update_ms;  /* "[MPM] Grid update" */
     float g2p_update_ms;   /* "[MPM] G2P" + "[MPM] Particle update" (fused) */
     float total_ms;
+    /* The P2G and the G2P + update kernels alone, from the kernels' own start and end
+     * timestamps (the stage times above also hold the launch gap before the kernel). */
+    float p2g_kernel_ms;
+    float g2p_update_kernel_ms;
 } nxm_stage_times;
 
 /* Launch shapes of a substep (no reference counterpart: wgpu picks none of these).
@@ -392,6 +396,11 @@ nxm_status nxm_scene_set_autotune(nxm_scene *scene, int enabled);
 nxm_status nxm_batch_get_launch_config(nxm_batch *batch, nxm_launch_config *out);
 nxm_status nxm_batch_set_launch_config(nxm_batch *batch, const nxm_launch_config *cfg);
 nxm_status nxm_scene_get_launch_config(nxm_scene *scene, nxm_launch_config *out);
+/* Tuner state: *tuning = 1 while the tuner is still trying shapes (0 once it has kept
+ * one, or when it is off); *substeps = the substeps its current or last run has used.
+ * Either pointer may be null.  No reference counterpart. */
+nxm_status nxm_batch_get_autotune(nxm_batch *batch, int *tuning, uint32_t *substeps);
+nxm_status nxm_scene_get_autotune(nxm_scene *scene, int *tuning, uint32_t *substeps);
 nxm_status nxm_batch_profile_begin(nxm_batch *batch);
 nxm_status nxm_batch_profile_end(nxm_batch *batch, nxm_stage_times *avg, uint32_t *steps);
 

@@ -712,6 +712,7 @@ class World {
         clean_ = true;
         if (tuning_step) {
             if (tev[1]) HIP_TRY(hipEventRecord(tev[1], stream_));
+            ++tune_substeps_;
             nxm_status st = tune_advance();
             if (st != NXM_OK) return st;
         }
@@ -738,16 +739,8 @@ class World {
         stepped_ = true;
         if (times) {
             HIP_TRY(hipEventSynchronize(ev_[4]));
-            float a = 0, b = 0, c = 0, d = 0;
-            HIP_TRY(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-            HIP_TRY(hipEventElapsedTime(&b, ev_[1], ev_[2]));
-            HIP_TRY(hipEventElapsedTime(&c, ev_[2], ev_[3]));
-            HIP_TRY(hipEventElapsedTime(&d, ev_[3], ev_[4]));
-            times->sort_ms = a;
-            times->p2g_ms = b;
-            times->grid_update_ms = c;
-            times->g2p_update_ms = d;
-            times->total_ms = a + b + c + d;
+            nxm_status st = stage_times(ev_, times);
+            if (st != NXM_OK) return st;
             return collect_errors();
         }
         return NXM_OK;
@@ -764,19 +757,36 @@ class World {
         profile_seen_ = 0;
     }
 
+    // Stage and kernel times of one substep from its SUBSTEP_EVENTS events.
+    static nxm_status stage_times(hipEvent_t *e, nxm_stage_times *t)
+    {
+        float ms[6] = {};
+        for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms[4], e[5], e[6]));
+        HIP_TRY(hipEventElapsedTime(&ms[5], e[7], e[8]));
+        t->sort_ms = ms[0];
+        t->p2g_ms = ms[1];
+        t->grid_update_ms = ms[2];
+        t->g2p_update_ms = ms[3];
+        t->total_ms = ms[0] + ms[1] + ms[2] + ms[3];
+        t->p2g_kernel_ms = ms[4];
+        t->g2p_update_kernel_ms = ms[5];
+        return NXM_OK;
+    }
+
     nxm_status profile_end(nxm_stage_times *avg, uint32_t *steps)
     {
         profiling_ = false;
         nxm_status st = sync();
         if (st != NXM_OK) return st;
-        double acc[4] = {0, 0, 0, 0};
+        double acc[7] = {};
         for (size_t k = 0; k < ring_used_; ++k) {
-            hipEvent_t *e = &ring_[5 * k];
-            for (int i = 0; i < 4; ++i) {
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, e[i], e[i + 1]));
-                acc[i] += ms;
-            }
+            nxm_stage_times t{};
+            st = stage_times(&ring_[SUBSTEP_EVENTS * k], &t);
+            if (st != NXM_OK) return st;
+            const float v[7] = {t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms,
+                                t.p2g_kernel_ms, t.g2p_update_kernel_ms};
+            for (int i = 0; i < 7; ++i) acc[i] += v[i];
         }
         const double n = ring_used_ ? (double)ring_used_ : 1.0;
         if (avg) {
@@ -784,7 +794,9 @@ class World {
             avg->p2g_ms = (float)(acc[1] / n);
             avg->grid_update_ms = (float)(acc[2] / n);
             avg->g2p_update_ms = (float)(acc[3] / n);
-            avg->total_ms = (float)((acc[0] + acc[1] + acc[2] + acc[3]) / n);
+            avg->total_ms = (float)(acc[4] / n);
+            avg->p2g_kernel_ms = (float)(acc[5] / n);
+            avg->g2p_update_kernel_ms = (float)(acc[6] / n);
         }
         if (steps) *steps = (uint32_t)ring_used_;
         ring_used_ = 0;
@@ -994,15 +1006,15 @@ class World {
 
     nxm_status ring_events(hipEvent_t **out)
     {
-        if (5 * (ring_used_ + 1) > ring_.size()) {
-            size_t add = std::max<size_t>(5 * 64, ring_.size());
+        if (SUBSTEP_EVENTS * (ring_used_ + 1) > ring_.size()) {
+            size_t add = std::max<size_t>(SUBSTEP_EVENTS * 64, ring_.size());
             for (size_t i = 0; i < add; ++i) {
                 hipEvent_t e;
                 HIP_TRY(hipEventCreate(&e));
                 ring_.push_back(e);
             }
         }
-        *out = &ring_[5 * ring_used_];
+        *out = &ring_[SUBSTEP_EVENTS * ring_used_];
         ++ring_used_;
         return NXM_OK;
     }
@@ -1350,6 +1362,7 @@ class World {
         tune_rep_ = -1;   // one untimed substep first
         tune_times_.assign(tune_cands_.size(), 1e30f);
         tune_used_ = 0;
+        tune_substeps_ = 0;
     }
 
     nxm_launch_config tune_candidate() const { return tune_cands_[tune_idx_]; }
@@ -1413,6 +1426,13 @@ class World {
 
     nxm_launch_config launch_config() const { return cfg_; }
 
+    // Whether the tuner is still trying shapes, and the substeps of its current (or last) run.
+    void autotune_state(int *tuning, uint32_t *substeps) const
+    {
+        if (tuning) *tuning = tuning_ ? 1 : 0;
+        if (substeps) *substeps = tune_substeps_;
+    }
+
     nxm_status set_launch_config(const nxm_launch_config &c)
     {
         if (c.p2g_groups == 0 || c.p2g_groups > max_p2g_groups() || c.p2g_groups % 8 != 0)
@@ -1442,7 +1462,7 @@ class World {
     bool tune_on_ = false, tuning_ = false;
     int tune_stage_ = 0, tune_rep_ = 0;
     size_t tune_idx_ = 0, tune_used_ = 0;
-    uint32_t tuned_p_ = 0;
+    uint32_t tuned_p_ = 0, tune_substeps_ = 0;
     nxm_launch_config tune_best_{};
     std::vector<nxm_launch_config> tune_cands_;
     std::vector<float> tune_times_;
@@ -1453,7 +1473,7 @@ class World {
     uint64_t gen_ = 0;                 // bumped whenever a buffer address or table changes
     hipGraphExec_t gexec_[2] = {};     // one per arena parity
     GraphKey gkey_[2] = {};
-    hipEvent_t ev_[5] = {};
+    hipEvent_t ev_[SUBSTEP_EVENTS] = {};
     std::vector<Slot> slots_;
     std::vector<SceneDev> layout_;
     bool dirty_ = true, pin_dirty_ = true, stepped_ = false, debug_ = false, profiling_ = false, clean_ = false;
@@ -2354,6 +2374,21 @@ nxm_status nxm_scene_get_launch_config(nxm_scene *sc, nxm_launch_config *out)
     if (!sc || !out) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
     if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
     *out = sc->world->launch_config();
+    return NXM_OK;
+}
+
+nxm_status nxm_batch_get_autotune(nxm_batch *b, int *tuning, uint32_t *substeps)
+{
+    if (!b) return fail(NXM_ERR_INVALID_ARGUMENT, "null batch");
+    b->world->autotune_state(tuning, substeps);
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_get_autotune(nxm_scene *sc, int *tuning, uint32_t *substeps)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (!sc->world) return fail(NXM_ERR_INVALID_ARGUMENT, "scene is not resident");
+    sc->world->autotune_state(tuning, substeps);
     return NXM_OK;
 }
 

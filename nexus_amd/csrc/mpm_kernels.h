@@ -74,6 +74,11 @@ struct LaunchConfig {
 
 // keys: run k_keys_touch first (the hash and counters emptied by launch_clear); without
 // it the substep uses the keys and block insertions of the previous substep's G2P.
+// ev (null, or SUBSTEP_EVENTS events): ev[0..4] are recorded on the stream between the
+// stages (sort | P2G | grid update | G2P + update); ev[5..6] and ev[7..8] take the start
+// and end of the P2G and the G2P + update kernels themselves (hipExtLaunchKernelGGL's
+// kernel events, without the launch gaps the stage events include).
+constexpr int SUBSTEP_EVENTS = 9;
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys);
 void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st);
 // Render data (24 floats per particle, at its id) of arena positions [start, start + count).

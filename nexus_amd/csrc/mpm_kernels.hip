@@ -22,6 +22,8 @@
 // The reference runs the same stages as ~21 dispatches (src_mpm/pipeline.rs:385-526).
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "mpm_kernels.h"
 
 namespace nxm {
@@ -1789,7 +1791,8 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (w.n_rp) k_mesh_cdf<<<grid_for(w.n_rp, T, cap), T, 0, st>>>(w);
         k_grid_cdf<<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);   // one wave per block
         k_particle_cdf<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
-        k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(k_p2g<true>, dim3(lc.p2g_blocks), dim3(64), 0, st, ev[5], ev[6], 0, w);
+        else k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);
         k_grid_update<true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
@@ -1802,18 +1805,23 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
             cs = lc.side;
         }
         k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
-        k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(k_g2p_update<true>, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st,
+                                      ev[7], ev[8], 0, w);
+        else k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
         if (cs != st) {
             (void)hipEventRecord(lc.join, cs);
             (void)hipStreamWaitEvent(st, lc.join, 0);
         }
         k_integrate_colliders<<<std::max(1, (int)((w.s_count * 16u + T - 1) / T)), T, 0, st>>>(w);
     } else {
-        k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(k_p2g<false>, dim3(lc.p2g_blocks), dim3(64), 0, st, ev[5], ev[6], 0, w);
+        else k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);
         k_grid_update<false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
-        k_g2p_update<false><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(k_g2p_update<false>, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st,
+                                      ev[7], ev[8], 0, w);
+        else k_g2p_update<false><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
 }

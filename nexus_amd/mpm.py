@@ -41,7 +41,8 @@ EXPORTS = [
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
     "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
-    "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
+    "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config",
+    "nxm_batch_get_autotune", "nxm_scene_get_autotune", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
     "nxm_debug_fail_uploads",
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
@@ -105,7 +106,8 @@ class LaunchConfig(ct.Structure):
 
 class _StageTimes(ct.Structure):
     _fields_ = [("sort_ms", ct.c_float), ("p2g_ms", ct.c_float), ("grid_update_ms", ct.c_float),
-                ("g2p_update_ms", ct.c_float), ("total_ms", ct.c_float)]
+                ("g2p_update_ms", ct.c_float), ("total_ms", ct.c_float), ("p2g_kernel_ms", ct.c_float),
+                ("g2p_update_kernel_ms", ct.c_float)]
 
 
 class _Canonical(ct.Structure):
@@ -154,6 +156,9 @@ def library() -> ct.CDLL:
     L.nxm_scene_swap_remove.restype = ct.c_int
     for f in ("nxm_batch_set_autotune", "nxm_scene_set_autotune"):
         getattr(L, f).argtypes = [P, ct.c_int]
+        getattr(L, f).restype = ct.c_int
+    for f in ("nxm_batch_get_autotune", "nxm_scene_get_autotune"):
+        getattr(L, f).argtypes = [P, ct.POINTER(ct.c_int), ct.POINTER(ct.c_uint32)]
         getattr(L, f).restype = ct.c_int
     for f in ("nxm_batch_get_launch_config", "nxm_scene_get_launch_config", "nxm_batch_set_launch_config"):
         getattr(L, f).argtypes = [P, ct.POINTER(LaunchConfig)]
@@ -214,6 +219,13 @@ class StageTimes:
     grid_update_ms: float
     g2p_update_ms: float
     total_ms: float
+    p2g_kernel_ms: float = 0.0          # the kernels alone (their own start/end timestamps)
+    g2p_update_kernel_ms: float = 0.0
+
+    @staticmethod
+    def of(t: "_StageTimes") -> "StageTimes":
+        return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms, t.p2g_kernel_ms,
+                          t.g2p_update_kernel_ms)
 
 
 class MpmPipeline:
@@ -235,7 +247,7 @@ class MpmPipeline:
         if timings:
             t = _StageTimes()
             _check(self.lib.nxm_pipeline_step(self.handle, state.handle, ct.byref(t)))
-            return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms)
+            return StageTimes.of(t)
         _check(self.lib.nxm_pipeline_step(self.handle, state.handle, None))
         return None
 
@@ -416,6 +428,13 @@ class MpmState:
         _check(self.lib.nxm_scene_get_launch_config(self.handle, ct.byref(c)))
         return c.as_tuple()
 
+    def autotune_state(self) -> tuple[bool, int]:
+        """(tuning, substeps): whether the tuner is still trying launch shapes, and the
+        substeps its current or last run has used."""
+        t, k = ct.c_int(0), ct.c_uint32(0)
+        _check(self.lib.nxm_scene_get_autotune(self.handle, ct.byref(t), ct.byref(k)))
+        return bool(t.value), int(k.value)
+
     def timestep_bound(self) -> float:
         """CFL bound on the substep length of the current state (WgTimestepBounds)."""
         dt = ct.c_float(0.0)
@@ -458,7 +477,7 @@ class MpmState:
         t = _StageTimes()
         n = ct.c_uint32(0)
         _check(self.lib.nxm_scene_profile_end(self.handle, ct.byref(t), ct.byref(n)))
-        return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms), int(n.value)
+        return StageTimes.of(t), int(n.value)
 
     def read_positions(self) -> np.ndarray:
         n = len(self)
@@ -556,7 +575,7 @@ class Batch:
         if timings:
             t = _StageTimes()
             _check(self.lib.nxm_batch_step(self.handle, ct.byref(t)))
-            return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms)
+            return StageTimes.of(t)
         _check(self.lib.nxm_batch_step(self.handle, None))
         return None
 
@@ -578,6 +597,12 @@ class Batch:
         _check(self.lib.nxm_batch_get_launch_config(self.handle, ct.byref(c)))
         return c.as_tuple()
 
+    def autotune_state(self) -> tuple[bool, int]:
+        """(tuning, substeps) of the batch's launch-shape tuner."""
+        t, k = ct.c_int(0), ct.c_uint32(0)
+        _check(self.lib.nxm_batch_get_autotune(self.handle, ct.byref(t), ct.byref(k)))
+        return bool(t.value), int(k.value)
+
     def set_launch_config(self, p2g_groups: int, p2g_block_cost: int, grid_groups: int) -> None:
         c = LaunchConfig(p2g_groups, p2g_block_cost, grid_groups)
         _check(self.lib.nxm_batch_set_launch_config(self.handle, ct.byref(c)))
@@ -589,7 +614,7 @@ class Batch:
         t = _StageTimes()
         n = ct.c_uint32(0)
         _check(self.lib.nxm_batch_profile_end(self.handle, ct.byref(t), ct.byref(n)))
-        return StageTimes(t.sort_ms, t.p2g_ms, t.grid_update_ms, t.g2p_update_ms, t.total_ms), int(n.value)
+        return StageTimes.of(t), int(n.value)
 
 
 def default_grid_capacity(scene: "_scenes.Scene") -> int:

@@ -36,15 +36,17 @@ def pytest_terminal_summary(terminalreporter):
     tr.write_sep("-", "parity: |gpu - fp64| / max|fp64| per field (one-substep bar)")
     by = {}
     for test, field, e_gpu, e_32, scale, clause in recs:
-        r = by.setdefault(field, dict(n=0, rel=0, fp32=0, worst=0.0, worst32=0.0, where=""))
+        r = by.setdefault(field, dict(n=0, rel=0, fp32=0, ulp=0, worst=0.0, worst32=0.0, where=""))
         r["n"] += 1
-        r[clause if clause in ("rel", "fp32") else "fp32"] += clause in ("rel", "fp32")
+        if clause in ("rel", "fp32", "ulp"):
+            r[clause] += 1
         rel = e_gpu / scale if scale > 0 else 0.0
         if rel >= r["worst"]:
             r["worst"], r["worst32"], r["where"] = rel, (e_32 / scale if scale > 0 else 0.0), test
     for field, r in sorted(by.items()):
         tr.write_line(f"  {field:>6}: {r['n']:4d} comparisons, {r['rel']:4d} within 1e-5, {r['fp32']:4d} by the "
-                      f"FP32 mirror bound; worst {r['worst']:.2e} (mirror {r['worst32']:.2e}) in {r['where']}")
+                      f"FP32 mirror bound, {r['ulp']:4d} by C's FP32 resolution; worst {r['worst']:.2e} "
+                      f"(mirror {r['worst32']:.2e}) in {r['where']}")
     path = os.environ.get("PARITY_REPORT")
     if path:
         import json

@@ -57,7 +57,7 @@ def check_against_oracle(st, sc, steps):
     assert np.array_equal(p["enabled"], o32.s.enabled)
     assert np.array_equal(cdf["affinity"], o64.s.affinity)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     assert_field_close("cdf normal", cdf["normal"], o32.s.normal, o64.s.normal)
     assert_field_close("cdf signed distance", cdf["signed_distance"], o32.s.signed_distance, o64.s.signed_distance)
     return p, cdf, o64

@@ -47,7 +47,7 @@ def test_inverted_particle_is_disabled(pipeline):
     assert p["enabled"][3] == 0 and p["mass"][3] == 0.0
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     st.close()
 
 
@@ -71,7 +71,7 @@ def test_boundary_conditions_in_contact(pipeline, bc, friction):
     o32, o64 = oracle_pair(sc, 6)
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     st.close()
 
 
@@ -91,7 +91,7 @@ def test_snow_compaction_matches_oracle(pipeline):
     o32, o64 = oracle_pair(sc, 10)
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     pdet = p["model"][:, 0].view(np.float32)
     assert np.abs(pdet - 1.0).max() > 1e-3                      # plasticity acted
     assert_field_close("plastic det", pdet, o32.s.pstate[:, 0], o64.s.pstate[:, 0])
@@ -127,7 +127,7 @@ def test_all_models_in_one_scene(pipeline):
     o32, o64 = oracle_pair(sc, 40)
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     words = p["model"].view(np.float32)
     sand = np.isin(p["tag"], (2, 3))
     snow = p["tag"] == 5

@@ -52,7 +52,7 @@ def test_failed_batch_add_leaves_scene_detached(pipeline):
     p = a.read_particles()
     o32, o64 = oracle_pair(a_sc, 3)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     batch.remove(a)
     batch.close()
     a.close()
@@ -86,7 +86,7 @@ def test_failed_reentry_after_swap_remove(pipeline):
     p = other.read_particles()
     o32, o64 = oracle_pair(other_sc, 2)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     batch.close()
     st.close()
     other.close()
@@ -109,7 +109,7 @@ def test_failed_append_keeps_particle_count(pipeline):
     p = st.read_particles()
     o32, o64 = oracle_pair(sc, 2)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
     st.close()
 
 

@@ -55,7 +55,7 @@ def test_one_substep_full_size(pipeline, make):
     compare_sort(can, o32, x0, sc.cell_width)
     assert np.array_equal(p["enabled"], o32.s.enabled)
     for f in ("x", "v", "C", "F"):
-        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f))
+        assert_field_close(f, p[f], getattr(o32.s, f), getattr(o64.s, f), o64.s)
 
 
 @pytest.mark.parametrize("make,substeps", [(scenes.sand_column, 20), (scenes.fluid_clumps, 4)],

@@ -29,7 +29,7 @@ def check_state(gp, o32, o64):
     out = {}
     for name, g, a, b in (("x", gp["x"], s32.x, s64.x), ("v", gp["v"], s32.v, s64.v),
                           ("C", gp["C"], s32.C, s64.C), ("F", gp["F"], s32.F, s64.F)):
-        out[name] = assert_field_close(name, g, a, b)
+        out[name] = assert_field_close(name, g, a, b, s64)
     assert np.array_equal(gp["enabled"], s32.enabled)
     return out
 
