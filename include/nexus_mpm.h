@@ -355,6 +355,72 @@ nxm_status nxm_debug_fail_uploads(uint32_t count);
 nxm_status nxm_pipeline_step(nxm_ctx *ctx, nxm_scene *scene, nxm_stage_times *times);
 /* Enqueues n substeps (the NexusPipeline::simulate substep loop, src/pipeline.rs:84-86). */
 nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *scene, uint32_t substeps);
+
+/* ---- one substep without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * nxm_pipeline_step_noalloc runs one substep (the kernels of nxm_pipeline_step, non-CPIC
+ * path) of a scene whose particles live in caller-owned device arrays, updated in place
+ * in particle-id order.  Everything else a substep needs (particle arenas, sort scratch,
+ * hash, block tables, staging tiles, counters) lives in one static device array of
+ * NXM_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from the first
+ * nxm_ctx_create on that GPU until the process ends.  A scene needs about
+ * 308 B per particle + 4.6 KB per hash slot (2 x next_pow2(grid_capacity) + 1 slots)
+ * + 60 KB; a larger one is rejected with NXM_ERR_INVALID_ARGUMENT before anything is
+ * enqueued.  The call allocates nothing (no hipMalloc, no pinned memory, no library
+ * allocation), enqueues on `stream` and returns without synchronizing.
+ *
+ * Every call on one GPU uses that GPU's single scratch array, which keeps the last
+ * call's contents (each call sets up what it reads): calls that overlap on one GPU
+ * (different streams, or different threads without ordering) race and must not.
+ * Colliders are the bodies' poses and velocities at the start of the substep; the
+ * substep does not write back their integrated poses (a caller with moving bodies
+ * advances them itself between calls).  CPIC and mesh colliders are not supported here. */
+#define NXM_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define NXM_NOALLOC_MAX_MATERIALS 16
+
+/* A material: model tag and words (GpuParticleModel) with the particle damping and
+ * boundary friction factor it is used with. */
+typedef struct {
+    uint32_t model_tag;
+    uint32_t model_data[NXM_MODEL_WORDS];
+    float damping;
+    float boundary_friction;
+} nxm_material;
+
+/* Device arrays (n particles, id order), read and written by the substep.  plastic may
+ * be NULL: the sand tags' (det Fp, q, log volume gain) and snow's det Fp then start from
+ * the material's leading words and are not written back.  flags: bit 0 enabled, bit 1
+ * fixed, bits 2..15 the material index (into nxm_noalloc_scene.materials); the substep
+ * clears bit 0 (and the mass) of a particle that fails (NaN position or det F <= 0). */
+typedef struct {
+    float *position;     /* n*3 */
+    float *velocity;     /* n*3 */
+    float *def_grad;     /* n*9 */
+    float *affine;       /* n*9 */
+    float *mass;         /* n */
+    float *init_volume;  /* n */
+    float *plastic;      /* n*3, or NULL */
+    uint32_t *flags;     /* n */
+} nxm_device_particles;
+
+/* The scene of one no-allocation substep (host memory, read during the call). */
+typedef struct {
+    float cell_width;
+    uint32_t grid_capacity;          /* blocks, rounded up to a power of two */
+    nxm_sim_params params;           /* gravity and the substep length */
+    const nxm_material *materials;   /* 1..NXM_NOALLOC_MAX_MATERIALS */
+    uint32_t num_materials;
+    const nxm_cuboid *colliders;     /* 0..NXM_MAX_COLLIDERS */
+    uint32_t num_colliders;
+} nxm_noalloc_scene;
+
+/* errors: optional device word that receives the substep's error bits (1 grid capacity
+ * exceeded, 2 a block outside the packable domain, 4 a material index out of range; 0
+ * when none).  stream: a hipStream_t (NULL: the null stream). */
+nxm_status nxm_pipeline_step_noalloc(nxm_ctx *ctx, const nxm_noalloc_scene *scene,
+                                     const nxm_device_particles *particles, uint32_t n,
+                                     uint32_t *errors, void *stream);
 /* Per-stage device timestamps of every step from begin to end, recorded on the
  * scene's stream without synchronizing and resolved at end (the reference's
  * GpuTimestamps passed to step and resolved afterwards, src_mpm/pipeline.rs:389,

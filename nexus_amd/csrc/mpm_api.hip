@@ -153,12 +153,39 @@ struct PinnedPool {
 struct nxm_ctx {
     int device = 0;
     int num_cus = 256;
+    void *scratch = nullptr;     // this device's address of the no-allocation scratch array
     // Scenes and batches created from this context; nxm_ctx_destroy defers the delete
     // until the last of them is destroyed.
     std::atomic<int> refs{0};
     std::atomic<bool> closed{false};
     PinnedPool pool;
 };
+
+// Scratch of nxm_pipeline_step_noalloc: one per GPU (each device holds its own copy of
+// this array at its own address).  Host code never uses the name itself (it is a host-side
+// stand-in, not a device address): nxm_ctx_create looks the device address up once per
+// device with hipGetSymbolAddress and every use goes through that pointer.
+static __device__ __align__(256) unsigned char scratch[NXM_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;   // device -> address of its scratch array
+
+// The calling thread's current device must be `device`.
+static hipError_t scratch_address(int device, void **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr.emplace(device, p);
+    *out = p;
+    return hipSuccess;
+}
 
 static void ctx_release(nxm_ctx *ctx)
 {
@@ -1689,6 +1716,20 @@ size_t scene_device_bytes(size_t n, uint32_t grid_cap)
 // ======================================================================================
 // C ABI
 
+// Carves the no-allocation scratch: sizes and offsets of a one-scene world.
+namespace {
+struct Carver {
+    size_t at = 0;
+    template <typename T>
+    size_t take(size_t count)
+    {
+        const size_t off = (at + 255u) & ~(size_t)255u;
+        at = off + count * sizeof(T);
+        return off;
+    }
+};
+}  // namespace
+
 extern "C" {
 
 const char *nxm_last_error(void) { return g_last_error.c_str(); }
@@ -1703,9 +1744,12 @@ nxm_status nxm_ctx_create(int device, nxm_ctx **out)
     HIP_TRY(hipSetDevice(device));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
+    void *scr = nullptr;
+    HIP_TRY(scratch_address(device, &scr));
     auto *c = new nxm_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    c->scratch = scr;
     *out = c;
     return NXM_OK;
 }
@@ -2217,6 +2261,122 @@ nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *sc, uint32_t substeps)
         nxm_status st = nxm_pipeline_step(ctx, sc, nullptr);
         if (st != NXM_OK) return st;
     }
+    return NXM_OK;
+}
+
+nxm_status nxm_pipeline_step_noalloc(nxm_ctx *ctx, const nxm_noalloc_scene *scene,
+                                     const nxm_device_particles *p, uint32_t n, uint32_t *errors, void *stream)
+{
+    using namespace nxm;
+    if (!ctx || !scene || !p) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->scratch) return fail(NXM_ERR_DEVICE, "no scratch address for this device");
+    if (!(scene->cell_width > 0.0f) || !std::isfinite(scene->cell_width))
+        return fail(NXM_ERR_INVALID_ARGUMENT, "cell_width must be positive and finite");
+    if (!(scene->params.dt > 0.0f) || !std::isfinite(scene->params.dt))
+        return fail(NXM_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
+    if (scene->grid_capacity == 0 || scene->grid_capacity > (1u << 26))
+        return fail(NXM_ERR_INVALID_ARGUMENT, "grid_capacity must be in [1, 2^26]");
+    if (!scene->materials || scene->num_materials == 0 || scene->num_materials > (uint32_t)NOALLOC_MAX_CLASSES)
+        return fail(NXM_ERR_INVALID_ARGUMENT, "1 to 16 materials");
+    if (scene->num_colliders > (uint32_t)NOALLOC_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
+    if (scene->num_colliders && !scene->colliders) return fail(NXM_ERR_INVALID_ARGUMENT, "null colliders");
+    for (uint32_t i = 0; i < scene->num_materials; ++i)
+        if (scene->materials[i].model_tag > NXM_MODEL_SNOW)
+            return fail(NXM_ERR_UNSUPPORTED_MODEL, "model tag above NXM_MODEL_SNOW (material " + std::to_string(i) + ")");
+    if (n == 0) return NXM_OK;
+    if (!p->position || !p->velocity || !p->def_grad || !p->affine || !p->mass || !p->init_volume || !p->flags)
+        return fail(NXM_ERR_INVALID_ARGUMENT, "null particle array");
+
+    // The world of one scene: the same tables and sizes a scene alone gets (its hash
+    // region of 2 x capacity slots plus the overflow slot; block ids sized by slots).
+    const uint32_t cap = next_pow2(scene->grid_capacity);
+    const uint32_t h_cap = next_pow2(2 * cap);
+    const uint32_t h_total = h_cap + 1u;
+    const uint32_t b_total = h_total;
+    const uint32_t p_cap = (n + 63u) & ~63u;
+    const uint32_t cu = (uint32_t)ctx->num_cus;
+    const uint32_t p2g_groups = cu * 32u;
+    const uint32_t scan_stride = (h_total + SCAN_TILE - 1u) / SCAN_TILE + 2u;
+    Carver c;
+    const size_t o_arena0 = c.take<float>((size_t)NUM_PLANES * p_cap);
+    const size_t o_arena1 = c.take<float>((size_t)NUM_PLANES * p_cap);
+    const size_t o_pkey = c.take<uint32_t>(p_cap), o_pdense = c.take<uint32_t>(p_cap), o_prank = c.take<uint32_t>(p_cap);
+    const size_t o_perm = c.take<uint32_t>(p_cap), o_sblock = c.take<uint32_t>(p_cap);
+    const size_t o_bscene = c.take<uint32_t>(b_total), o_bvid = c.take<int4>(b_total), o_bnp = c.take<uint32_t>(b_total);
+    const size_t o_bfirst = c.take<uint32_t>(b_total), o_bup = c.take<uint32_t>(8 * (size_t)b_total);
+    const size_t o_bdn = c.take<uint32_t>(8 * (size_t)b_total);
+    const size_t o_stage = c.take<float>((size_t)4 * TILE_NODES * b_total);
+    const size_t o_gridv = c.take<float4>((size_t)NODES_PER_BLOCK * b_total);
+    const size_t o_hkeys = c.take<uint32_t>(h_total), o_hvals = c.take<uint32_t>(h_total), o_hmark = c.take<uint32_t>(h_total);
+    const size_t o_bprim = c.take<uint32_t>(b_total), o_bmask = c.take<uint32_t>(b_total);
+    const size_t o_rstart = c.take<uint32_t>(1), o_rscene = c.take<uint32_t>(1);
+    const size_t o_scenes = c.take<SceneDev>(1), o_ctr = c.take<SceneCtr>(1);
+    const size_t o_classes = c.take<ClassDev>(NOALLOC_MAX_CLASSES), o_cols = c.take<ColliderDev>(NOALLOC_MAX_COLLIDERS);
+    const size_t o_g = c.take<Globals>(1), o_p2g = c.take<uint32_t>((size_t)p2g_groups + 1);
+    const size_t o_scan = c.take<unsigned long long>(2 * (size_t)scan_stride);
+    if (c.at > (size_t)NXM_NOALLOC_SCRATCH_BYTES)
+        return fail(NXM_ERR_INVALID_ARGUMENT, "the scene needs " + std::to_string(c.at) + " bytes of scratch; "
+                    "nxm_pipeline_step_noalloc has " + std::to_string((size_t)NXM_NOALLOC_SCRATCH_BYTES) +
+                    " (NXM_NOALLOC_SCRATCH_BYTES): use fewer particles or a smaller grid capacity");
+
+    char *base = static_cast<char *>(ctx->scratch);
+    auto at = [&](size_t off) { return static_cast<void *>(base + off); };
+    WorldDev w{};
+    w.in = Arena{static_cast<float *>(at(o_arena0)), p_cap};
+    w.out = Arena{static_cast<float *>(at(o_arena1)), p_cap};
+    w.pkey = static_cast<uint32_t *>(at(o_pkey)); w.pdense = static_cast<uint32_t *>(at(o_pdense));
+    w.prank = static_cast<uint32_t *>(at(o_prank)); w.perm = static_cast<uint32_t *>(at(o_perm));
+    w.sblock = static_cast<uint32_t *>(at(o_sblock));
+    w.bscene = static_cast<uint32_t *>(at(o_bscene)); w.bvid = static_cast<int4 *>(at(o_bvid));
+    w.bnp = static_cast<uint32_t *>(at(o_bnp)); w.bfirst = static_cast<uint32_t *>(at(o_bfirst));
+    w.bup = static_cast<uint32_t *>(at(o_bup)); w.bdn = static_cast<uint32_t *>(at(o_bdn));
+    w.stage = static_cast<float *>(at(o_stage)); w.gridv = static_cast<float4 *>(at(o_gridv));
+    w.hkeys = static_cast<uint32_t *>(at(o_hkeys)); w.hvals = static_cast<uint32_t *>(at(o_hvals));
+    w.hmark = static_cast<uint32_t *>(at(o_hmark));
+    w.bprim = static_cast<uint32_t *>(at(o_bprim)); w.bmask = static_cast<uint32_t *>(at(o_bmask));
+    w.region_start = static_cast<const uint32_t *>(at(o_rstart)); w.region_scene = static_cast<const uint32_t *>(at(o_rscene));
+    w.n_regions = 1;
+    w.scenes = static_cast<const SceneDev *>(at(o_scenes)); w.ctr = static_cast<SceneCtr *>(at(o_ctr));
+    w.classes = static_cast<const ClassDev *>(at(o_classes)); w.cols = static_cast<ColliderDev *>(at(o_cols));
+    w.g = static_cast<Globals *>(at(o_g));
+    w.p2g_start = static_cast<uint32_t *>(at(o_p2g)); w.p2g_groups = p2g_groups; w.p2g_block_cost = 16u;
+    w.scan_state = static_cast<unsigned long long *>(at(o_scan)); w.scan_stride = scan_stride;
+    w.p_cap = p_cap; w.b_total = b_total; w.h_total = h_total; w.s_count = 1;
+
+    NoallocTables t{};
+    SceneDev &L = t.scene;
+    const float h = scene->cell_width, dt = scene->params.dt;
+    L.h = h; L.inv_h = 1.0f / h; L.dt = dt; L.vlim = h / dt;
+    L.gx = scene->params.gravity[0]; L.gy = scene->params.gravity[1]; L.gz = scene->params.gravity[2];
+    L.inv_d = 4.0f / (h * h);
+    L.active = 1; L.b_cap = cap; L.h_base = 0; L.h_cap = h_cap;
+    L.col_start = 0; L.col_count = scene->num_colliders; L.class_base = 0; L.cpic = 0;
+    for (uint32_t i = 0; i < scene->num_materials; ++i) {
+        const nxm_material &m = scene->materials[i];
+        ClassDev &k = t.classes[i];
+        k.tag = m.model_tag;
+        k.damping = m.damping;
+        k.bfric = m.boundary_friction;
+        std::memcpy(k.w, m.model_data, sizeof(k.w));
+    }
+    for (uint32_t i = 0; i < scene->num_colliders; ++i) t.cols[i] = collider_record(scene->colliders[i], SHAPE_CUBOID);
+    t.n_classes = scene->num_materials;
+    t.n_cols = scene->num_colliders;
+    t.n = n;
+    NoallocParticles dp{p->position, p->velocity, p->def_grad, p->affine, p->mass, p->init_volume, p->plastic, p->flags,
+                        errors};
+
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    LaunchConfig lc{};
+    lc.p_hint = n;
+    lc.persistent_blocks = (int)(cu * 8u);
+    lc.p2g_blocks = (int)p2g_groups;
+    launch_noalloc_setup(w, t, dp, lc.persistent_blocks, st);
+    launch_clear(w, lc, st);
+    launch_substep(w, lc, st, nullptr, true);
+    launch_noalloc_unpack(w, dp, n, lc.persistent_blocks, st);
+    HIP_TRY(hipGetLastError());
     return NXM_OK;
 }
 

@@ -90,4 +90,28 @@ void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,
                       hipStream_t st);
 
+// ---- substep without allocation (nxm_pipeline_step_noalloc) ----
+// The scene's small tables travel as one kernel argument (under 4 KB), so that a captured
+// launch carries them by value and no host buffer has to outlive the call.
+constexpr int NOALLOC_MAX_CLASSES = 16;
+constexpr int NOALLOC_MAX_COLLIDERS = 16;
+struct NoallocTables {
+    SceneDev scene;
+    ClassDev classes[NOALLOC_MAX_CLASSES];
+    ColliderDev cols[NOALLOC_MAX_COLLIDERS];
+    uint32_t n_classes, n_cols, n;
+};
+static_assert(sizeof(NoallocTables) <= 4000, "kernel arguments are limited to 4 KB");
+// The caller's particles: device structure-of-arrays in particle-id order, read before
+// the substep and written back after it.
+struct NoallocParticles {
+    float *x, *v, *F, *C, *mass, *vol0, *pstate;   // n*3, n*3, n*9, n*9, n, n, n*3 (pstate may be null)
+    uint32_t *flags;                               // n: bit 0 enabled, bit 1 fixed, bits 2..15 material
+    uint32_t *errors;                              // 1 word (may be null): the substep's error bits
+};
+// Writes the tables, the counters and the input arena (from the caller's arrays).
+void launch_noalloc_setup(const WorldDev &w, const NoallocTables &t, const NoallocParticles &p, int cap, hipStream_t st);
+// Copies the output arena back to the caller's arrays by particle id.
+void launch_noalloc_unpack(const WorldDev &w, const NoallocParticles &p, uint32_t n, int cap, hipStream_t st);
+
 }  // namespace nxm

@@ -1933,4 +1933,97 @@ void launch_prep_readback(const Arena &a, uint32_t start, uint32_t count, const 
                                                           h, dt, cdf ? 1u : 0u, out);
 }
 
+// ------------------------------------------------------------------------------------
+// Substep without allocation: the world's tables and arenas live in a buffer the host
+// carved up (the library's static scratch array); the particles come from and return to
+// the caller's arrays in id order.
+
+// Tables and counters (one workgroup).
+__global__ void k_noalloc_tables(WorldDev w, NoallocTables t)
+{
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        *const_cast<SceneDev *>(w.scenes) = t.scene;
+        w.ctr[0] = SceneCtr{0u, 0u, 0u, 0u};
+        const_cast<uint32_t *>(w.region_start)[0] = 0u;
+        const_cast<uint32_t *>(w.region_scene)[0] = 0u;
+        w.g->p_in = t.n;
+        w.g->n_contact = 0u;
+    }
+    if (tid < t.n_classes) const_cast<ClassDev *>(w.classes)[tid] = t.classes[tid];
+    if (tid < t.n_cols) w.cols[tid] = t.cols[tid];
+}
+
+// The input arena from the caller's arrays (after k_noalloc_tables: it may report a bad
+// material index in the scene's error word).
+__global__ void k_noalloc_pack(WorldDev w, NoallocTables t, NoallocParticles p)
+{
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + tid; i < t.n; i += gridDim.x * blockDim.x) {
+        const Arena a = w.in;
+        for (int c = 0; c < 3; ++c) {
+            a.plane(PX + c)[i] = p.x[3 * (size_t)i + c];
+            a.plane(PVX + c)[i] = p.v[3 * (size_t)i + c];
+        }
+        for (int c = 0; c < 9; ++c) {
+            a.plane(PC0 + c)[i] = p.C[9 * (size_t)i + c];
+            a.plane(PF0 + c)[i] = p.F[9 * (size_t)i + c];
+        }
+        a.plane(PMASS)[i] = p.mass[i];
+        a.plane(PVOL0)[i] = p.vol0[i];
+        const uint32_t f = p.flags[i];
+        uint32_t cls = flags_class(f);
+        if (cls >= t.n_classes) {
+            // A material index outside the table: reported, and the particle takes material 0.
+            atomicOr(&w.ctr[0].err, ERR_MODEL);
+            cls = 0u;
+        }
+        // Plastic state: the caller's, or the material's initial words (the sand tags'
+        // three, snow's one), as an upload takes it from the particle's model words.
+        const uint32_t tag = t.classes[cls].tag;
+        const int pw = (tag == 2u || tag == 3u) ? 3 : tag == 5u ? 1 : 0;
+        for (int c = 0; c < 3; ++c)
+            a.plane(PS0 + c)[i] = c < pw ? (p.pstate ? p.pstate[3 * (size_t)i + c] : t.classes[cls].w[c]) : 0.0f;
+        for (int c = PAFF; c <= PSD; ++c) a.plane(c)[i] = 0.0f;
+        a.uplane(PFLAGS)[i] = make_flags((f & FLAG_ENABLED) != 0u, (f & FLAG_FIXED) != 0u, cls, 0u);
+        a.uplane(PID)[i] = i;
+    }
+}
+
+__global__ void k_noalloc_unpack(WorldDev w, NoallocParticles p, uint32_t n)
+{
+    const Arena a = w.out;
+    for (uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
+        const uint32_t id = a.uplane(PID)[pos];
+        if (id >= n) continue;
+        for (int c = 0; c < 3; ++c) {
+            p.x[3 * (size_t)id + c] = a.plane(PX + c)[pos];
+            p.v[3 * (size_t)id + c] = a.plane(PVX + c)[pos];
+        }
+        for (int c = 0; c < 9; ++c) {
+            p.C[9 * (size_t)id + c] = a.plane(PC0 + c)[pos];
+            p.F[9 * (size_t)id + c] = a.plane(PF0 + c)[pos];
+        }
+        p.mass[id] = a.plane(PMASS)[pos];
+        if (p.pstate)
+            for (int c = 0; c < 3; ++c) p.pstate[3 * (size_t)id + c] = a.plane(PS0 + c)[pos];
+        const uint32_t f = a.uplane(PFLAGS)[pos];
+        p.flags[id] = (f & (FLAG_ENABLED | FLAG_FIXED)) | (flags_class(f) << 2);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && p.errors) p.errors[0] = w.ctr[0].err;
+}
+
+void launch_noalloc_setup(const WorldDev &w, const NoallocTables &t, const NoallocParticles &p, int cap, hipStream_t st)
+{
+    const int T = 256;
+    k_noalloc_tables<<<1, 64, 0, st>>>(w, t);
+    k_noalloc_pack<<<grid_for(std::max<uint32_t>(t.n, 1u), T, cap), T, 0, st>>>(w, t, p);
+}
+
+void launch_noalloc_unpack(const WorldDev &w, const NoallocParticles &p, uint32_t n, int cap, hipStream_t st)
+{
+    const int T = 256;
+    k_noalloc_unpack<<<grid_for(std::max<uint32_t>(n, 1u), T, cap), T, 0, st>>>(w, p, n);
+}
+
 }  // namespace nxm

@@ -48,7 +48,7 @@ EXPORTS = [
     "nxm_scene_set_cpic", "nxm_scene_read_cdf", "nxm_scene_read_grid_cpic", "nxm_scene_read_colliders",
     "nxm_scene_stage_upload", "nxm_scene_set_trimesh_colliders", "nxm_sample_trimesh",
     "nxm_scene_prep_readback", "nxm_scene_prep_readback_rigid",
-    "nxm_pipeline_step", "nxm_pipeline_run", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
+    "nxm_pipeline_step", "nxm_pipeline_run", "nxm_pipeline_step_noalloc", "nxm_scene_profile_begin", "nxm_scene_profile_end", "nxm_scene_profile_sampled",
     "nxm_batch_create", "nxm_batch_destroy", "nxm_batch_add_scene", "nxm_batch_remove_scene", "nxm_batch_step",
     "nxm_batch_run", "nxm_batch_synchronize", "nxm_batch_len", "nxm_batch_set_graph",
     "nxm_batch_profile_begin", "nxm_batch_profile_end",
@@ -172,6 +172,8 @@ def library() -> ct.CDLL:
     L.nxm_scene_set_debug_grid.argtypes = [P, I]
     L.nxm_debug_fail_uploads.argtypes = [ct.c_uint32]
     L.nxm_scene_set_cpic.argtypes = [P, I]
+    L.nxm_pipeline_step_noalloc.argtypes = [P, P, P, ct.c_uint32, P, P]
+    L.nxm_pipeline_step_noalloc.restype = ct.c_int
     L.nxm_scene_stage_upload.argtypes = [P]
     L.nxm_scene_read_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_read_cdf.argtypes = [P, P, P, P, ct.c_size_t]
@@ -630,3 +632,37 @@ def default_grid_capacity(scene: "_scenes.Scene") -> int:
     while cap < 2 * est:
         cap *= 2
     return min(cap, 1 << 26)
+
+
+# ---- one substep without allocation (nxm_pipeline_step_noalloc) ----
+
+class Material(ct.Structure):
+    """nxm_material: model tag and words, damping and boundary friction factor."""
+    _fields_ = [("model_tag", ct.c_uint32), ("model_data", ct.c_uint32 * 24), ("damping", ct.c_float),
+                ("boundary_friction", ct.c_float)]
+
+
+class DeviceParticles(ct.Structure):
+    """nxm_device_particles: device addresses (integers) of the caller's arrays."""
+    _fields_ = [(name, ct.c_void_p) for name in (
+        "position", "velocity", "def_grad", "affine", "mass", "init_volume", "plastic", "flags")]
+
+
+class _NoallocScene(ct.Structure):
+    _fields_ = [("cell_width", ct.c_float), ("grid_capacity", ct.c_uint32), ("params", _SimParams),
+                ("materials", ct.c_void_p), ("num_materials", ct.c_uint32), ("colliders", ct.c_void_p),
+                ("num_colliders", ct.c_uint32)]
+
+
+def step_noalloc(pipeline: MpmPipeline, *, cell_width: float, grid_capacity: int, gravity, dt: float,
+                 materials, colliders, particles: DeviceParticles, n: int, errors: int | None = None,
+                 stream: int | None = None) -> None:
+    """One substep of particles held in caller-owned device arrays, with no allocation
+    (nxm_pipeline_step_noalloc).  ``materials``: a sequence of Material; ``colliders``: a
+    CUBOID_DTYPE array; ``errors`` / ``stream``: a device address / a hipStream_t (ints)."""
+    lib = pipeline.lib
+    mats = (Material * max(len(materials), 1))(*materials)
+    cols = np.ascontiguousarray(colliders, dtype=_scenes.CUBOID_DTYPE).reshape(-1)
+    sc = _NoallocScene(ct.c_float(cell_width), grid_capacity, _SimParams((ct.c_float * 3)(*gravity), dt),
+                       ct.cast(mats, ct.c_void_p), len(materials), _ptr(cols) if cols.size else None, cols.size)
+    _check(lib.nxm_pipeline_step_noalloc(pipeline.handle, ct.byref(sc), ct.byref(particles), n, errors, stream))
