@@ -356,6 +356,20 @@ nxm_status nxm_pipeline_step(nxm_ctx *ctx, nxm_scene *scene, nxm_stage_times *ti
 /* Enqueues n substeps (the NexusPipeline::simulate substep loop, src/pipeline.rs:84-86). */
 nxm_status nxm_pipeline_run(nxm_ctx *ctx, nxm_scene *scene, uint32_t substeps);
 
+/* Adaptive substeps (no reference counterpart; the reference computes the bound,
+ * WgTimestepBounds::compute_bounds, src_mpm/solver/timestep_bound.rs:25-54, and leaves the
+ * choice of dt to its caller).  When enabled, each substep's length is
+ * min(dt, the CFL bound of the scene's state at the start of the substep), with dt the
+ * scene's substep length (nxm_scene_set_simulation_params / write_substep_params) as the
+ * maximum and the bound as nxm_scene_timestep_bound computes it.  The bound is taken on
+ * the device (the G2P kernel min-reduces it for the next substep), so a batch of adaptive
+ * scenes steps without a host round trip, each scene at its own length. */
+nxm_status nxm_scene_set_adaptive_dt(nxm_scene *scene, int enabled);
+/* The scene's clock: simulated time (the sum of its substeps' lengths since creation),
+ * the length of its last substep and the number of substeps.  Synchronizes.  Any pointer
+ * may be NULL. */
+nxm_status nxm_scene_read_clock(nxm_scene *scene, double *time, float *last_dt, uint64_t *substeps);
+
 /* ---- one substep without allocation ----
  * No counterpart in the reference.  For callers that cannot allocate device memory:
  * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.

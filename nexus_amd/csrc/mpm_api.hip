@@ -376,6 +376,11 @@ struct nxm_scene {
     bool graph = false;                 // standalone steps replay a captured graph
     bool cpic = false;                  // MpmState::use_cpic
     std::vector<HostMesh> meshes;       // mesh colliders: the last meshes.size() entries of cols
+    // Adaptive substeps (nxm_scene_set_adaptive_dt) and the scene's clock: the world's copy
+    // is authoritative while the scene is resident, clock_host says the host copy is newer.
+    bool adaptive = false;
+    nxm::SceneClock clock{0xFFFFFFFFu, 0.0f, 0u, 0u, 0.0};
+    bool clock_host = true;
 };
 
 namespace nxm {
@@ -562,6 +567,7 @@ class World {
         sc->ranges.clear();
         sc->dev_col_start = NONE;
         sc->cols_host = true;
+        sc->clock_host = true;
         dirty_ = true;
     }
 
@@ -572,6 +578,7 @@ class World {
         if (st != NXM_OK) return st;
         sc->dev_col_start = NONE;
         sc->cols_host = true;
+        sc->clock_host = true;
         slots_[sc->slot].state = SLOT_RETIRING;
         slots_[sc->slot].scene = nullptr;
         sc->world = nullptr;
@@ -591,6 +598,8 @@ class World {
         if (!sc->cols_host && sc->dev_col_start != NONE && !sc->cols.empty())
             HIP_TRY(hipMemcpy(sc->cols.data(), cols_[cols_cur_].p + sc->dev_col_start, sc->cols.size() * sizeof(ColliderDev),
                               hipMemcpyDeviceToHost));
+        if (!sc->clock_host && sc->world == this && sc->slot < clock_.n)
+            HIP_TRY(hipMemcpy(&sc->clock, clock_.p + sc->slot, sizeof(SceneClock), hipMemcpyDeviceToHost));
         return NXM_OK;
     }
 
@@ -843,7 +852,7 @@ class World {
         DevBuf<uint32_t> out;
         HIP_TRY(out.reserve(slots_.size()));
         HIP_TRY(hipMemsetD32Async(out.p, 0xFFFFFFFF, slots_.size(), stream_));
-        launch_timestep_bound(view(), p_in_, out.p, ctx_->num_cus * 8, stream_);
+        launch_timestep_bound(view(), p_in_, out.p, 1u, ctx_->num_cus * 8, stream_);
         HIP_TRY(hipGetLastError());
         nxm_status st = sync();
         if (st != NXM_OK) return st;
@@ -1066,6 +1075,7 @@ class World {
         w.mesh_v = mesh_v_.p; w.mesh_t = mesh_t_.p; w.mesh_rp = mesh_rp_.p; w.mkey = mkey_.p; w.maff = maff_.p;
         w.n_rp = n_rp_;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
+        w.clock = clock_.p; w.any_adaptive = any_adaptive_ ? 1u : 0u;
         return w;
     }
 
@@ -1107,7 +1117,7 @@ class World {
         uint32_t ncols = 0;
         std::vector<uint32_t> region_start, region_scene;
         uint32_t b = 0, hsum = 0;
-        bool any_cpic = false;
+        bool any_cpic = false, any_adaptive = false;
         std::vector<float4> mv, mrp;
         std::vector<uint4> mt;
         for (uint32_t i = 0; i < slots_.size(); ++i) {
@@ -1135,6 +1145,9 @@ class World {
             cls.insert(cls.end(), sc->classes.begin(), sc->classes.end());
             L.cpic = sc->cpic ? 1u : 0u;
             any_cpic |= sc->cpic;
+            L.dt_max = sc->dt;
+            L.adaptive = sc->adaptive ? 1u : 0u;
+            any_adaptive |= sc->adaptive;
             // Mesh colliders (CPIC scenes only): rigid particles sampled at the cell width
             // (src_mpm/pipeline.rs:310-316), geometry appended to the world's tables.
             const uint32_t nmesh = (uint32_t)sc->meshes.size();
@@ -1168,6 +1181,7 @@ class World {
         const uint32_t scan_stride = (hsum + SCAN_TILE - 1u) / SCAN_TILE + 2u;
         const uint32_t nreg = (uint32_t)region_start.size();
         const bool grow = std::max<size_t>(slots_.size(), 1) > scenes_.n || std::max<size_t>(slots_.size(), 1) > ctr_.n ||
+                          std::max<size_t>(slots_.size(), 1) > clock_.n ||
                           std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(ncols, 1) > cols_[0].n || std::max<size_t>(ncols, 1) > cols_[1].n ||
                           hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
                           std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
@@ -1185,6 +1199,7 @@ class World {
                 st = pull_colliders(sc);
                 if (st != NXM_OK) return st;
                 sc->cols_host = true;
+                sc->clock_host = true;   // the clock table may be reallocated too
             }
         }
         h_total_ = hsum;
@@ -1221,6 +1236,18 @@ class World {
             sc->cols_host = false;
         }
         cols_cur_ = nc;
+        // Clocks stay at their slot while the scene is resident; scenes new to the world
+        // (or after a reallocation) bring theirs from the host.
+        HIP_TRY(clock_.reserve(std::max<size_t>(slots_.size(), 1)));
+        for (uint32_t i = 0; i < slots_.size(); ++i) {
+            if (slots_[i].state != SLOT_ACTIVE) continue;
+            nxm_scene *sc = slots_[i].scene;
+            if (!sc->clock_host) continue;
+            st = put_async(clock_.p + i, &sc->clock, sizeof(SceneClock));
+            if (st != NXM_OK) return st;
+            sc->clock_host = false;
+        }
+        any_adaptive_ = any_adaptive;
         if (b_total_ > bscene_.n) {
             size_t nb = (size_t)b_total_ + b_total_ / 4;
             HIP_TRY(bscene_.reserve(nb));
@@ -1513,6 +1540,8 @@ class World {
     DevBuf<float> arenas_[2];
     DevBuf<uint32_t> pkey_, pdense_, prank_, perm_, sblock_;
     DevBuf<uint32_t> bscene_, bprim_, bmask_, bnp_, bfirst_, bup_, bdn_, hkeys_, hvals_, hmark_;
+    DevBuf<SceneClock> clock_;       // per slot
+    bool any_adaptive_ = false;
     DevBuf<unsigned long long> scan_state_;
     DevBuf<uint32_t> p2g_start_;
     uint32_t scan_stride_ = 0;
@@ -2090,6 +2119,27 @@ nxm_status nxm_scene_timestep_bound(nxm_scene *sc, float *max_dt)
     nxm_status st = sc->world->timestep_bound(sc, &units);
     if (st != NXM_OK) return st;
     *max_dt = (float)units / 1.0e12f;
+    return latched(sc, NXM_OK);
+}
+
+nxm_status nxm_scene_set_adaptive_dt(nxm_scene *sc, int enabled)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    sc->adaptive = enabled != 0;
+    if (sc->world) sc->world->mark_dirty();
+    return NXM_OK;
+}
+
+nxm_status nxm_scene_read_clock(nxm_scene *sc, double *time, float *last_dt, uint64_t *substeps)
+{
+    if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
+    if (sc->world) {
+        nxm_status st = sc->world->pull_colliders(sc);   // synchronizes; takes the clock too
+        if (st != NXM_OK) return st;
+    }
+    if (time) *time = sc->clock.t;
+    if (last_dt) *last_dt = sc->clock.dt;
+    if (substeps) *substeps = sc->clock.steps;
     return latched(sc, NXM_OK);
 }
 

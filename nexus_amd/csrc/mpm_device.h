@@ -62,6 +62,21 @@ struct SceneDev {
     uint32_t col_start, col_count;
     uint32_t class_base;          // first entry of the scene's material classes
     uint32_t cpic;                // compatible particle-in-cell transfers (MpmState::use_cpic)
+    // Adaptive substeps: each substep's length is min(dt_max, the CFL bound of the state
+    // at its start); dt and vlim above are then rewritten on the device each substep.
+    float dt_max;                 // the scene's substep length (the maximum when adaptive)
+    uint32_t adaptive;
+};
+
+// Per scene slot, on the device while the scene is resident: its simulated time, the
+// length of its last substep and the CFL bound (1e-12 s units, NONE = none yet) that the
+// next substep uses when adaptive.
+struct SceneClock {
+    uint32_t bound;
+    float dt;
+    uint32_t steps;
+    uint32_t pad;
+    double t;
 };
 
 struct SceneCtr {

@@ -60,6 +60,10 @@ struct WorldDev {
     uint32_t *maff;                // per node: the meshes' affinity bits
     uint32_t n_rp;
     uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
+    // Per scene slot: simulated time and the CFL bound for adaptive substeps (null: no
+    // clocks kept, as in the no-allocation substep); any_adaptive: some scene is adaptive.
+    SceneClock *clock;
+    uint32_t any_adaptive;
 };
 
 struct LaunchConfig {
@@ -86,7 +90,9 @@ void launch_prep_readback(const Arena &a, uint32_t start, uint32_t count, const 
                           const float *radius, const float4 *palette, uint32_t npal, uint32_t mode, float h, float dt,
                           bool cdf, float *out, int cap, hipStream_t st);
 void launch_set_particles(Globals *g, uint32_t p_in, hipStream_t st);
-void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st);
+// Minimum over each scene's enabled particles of the CFL bound (1e-12 s units), into
+// out[s * stride] by atomicMin (the caller fills it with NONE first).
+void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, uint32_t stride, int max_blocks, hipStream_t st);
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,
                       hipStream_t st);
 

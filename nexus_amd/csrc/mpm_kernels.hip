@@ -127,12 +127,13 @@ __device__ inline int4 unpack_vid(uint32_t key)
 
 __global__ void k_clear(WorldDev w)
 {
-    uint32_t n = max(max(w.h_total, w.b_total), 2u * w.scan_stride);
+    uint32_t n = max(max(max(w.h_total, w.b_total), 2u * w.scan_stride), w.s_count);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < w.h_total) { w.hkeys[i] = EMPTY_KEY; w.hmark[i] = 0u; }
         if (i < w.b_total) w.bnp[i] = 0u;
         if (i < 2u * w.scan_stride) w.scan_state[i] = 0ull;
         if (i == 0) { w.g->p_out = 0u; w.g->n_blocks = 0u; w.g->ticket[0] = 0u; w.g->ticket[1] = 0u; }
+        if (w.clock && i < w.s_count) w.clock[i].bound = NONE;   // recomputed by the key pass's bound
     }
 }
 
@@ -311,10 +312,13 @@ __device__ inline void scan_out(const WorldDev &w, uint32_t i, uint32_t r, uint3
 // running, which publish their aggregate before waiting themselves, so every wait ends.
 constexpr unsigned long long ST_AGG = 1ull << 32, ST_PREFIX = 2ull << 32;
 
+__device__ inline void advance_clocks(const WorldDev &w);
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_scan(WorldDev w)
 {
     constexpr int PER = SCAN_TILE / 256;
+    if (MODE == MODE_SLOTS && blockIdx.x == 0) advance_clocks(w);   // the substep's first kernel
     __shared__ uint32_t sh[4];
     __shared__ uint32_t s_tile, s_prefix;
     const uint32_t len = MODE == MODE_SLOTS ? w.h_total : min(w.g->n_blocks, w.b_total);
@@ -1317,6 +1321,125 @@ __device__ inline bool contact_particle(const WorldDev &w, uint32_t pos, uint32_
     return w.breach[d] && (w.out.uplane(PAFF)[pos] & 0xFFFFu) != 0u;
 }
 
+// ------------------------------------------------------------------------------------
+// CFL timestep bound (timestep_bound.rs:61-111, models/default.rs:268-330,
+// models/utils.rs:157-172) over the input arena: per enabled particle the smaller of the
+// model's sound-speed bound and the velocity bound, as 1e-12 s units (saturating),
+// min-reduced per scene.  Particles of one scene are contiguous, so a wave first
+// reduces over the lanes that share a scene and issues one atomic per scene present.
+__device__ inline uint32_t secs_to_int(float dt)
+{
+    const float x = dt * 1.0e12f;
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967295.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+__device__ inline float particle_dt_bound(const ClassDev &cls, float h, float mass, float vol0, const f3 &v,
+                                          const m3 &F, const m3 &A, float pdet)
+{
+    const uint32_t tag = cls.tag;
+    float bulk, shear, cfl;
+    if (tag <= 1u) { bulk = cls.w[0] + 2.0f * cls.w[1] / 3.0f; shear = cls.w[1]; cfl = cls.w[2]; }
+    else if (tag <= 3u) { bulk = cls.w[10] + 2.0f * cls.w[11] / 3.0f; shear = cls.w[11]; cfl = cls.w[12]; }
+    else if (tag == 4u) { bulk = cls.w[0] * cls.w[1]; shear = 0.0f; cfl = cls.w[3]; }
+    else if (tag == 5u) {
+        // Snow: the linear elastic backbone with its moduli scaled by the hardening factor.
+        const float hard = expf(cls.w[3] * (1.0f - pdet));
+        bulk = (cls.w[4] + 2.0f * cls.w[5] / 3.0f) * hard; shear = cls.w[5] * hard; cfl = cls.w[6];
+    }
+    else { bulk = shear = cfl = 0.0f; }
+    float dt = 0.0f;
+    if (tag <= 5u) {
+        const float density0 = mass / vol0;
+        const float density = density0 / fmaxf(det(F), 1.0e-6f);
+        const float c = sqrtf((bulk + shear * 4.0f / 3.0f) / density);
+        dt = cfl * h / fmaxf(length(v), c);
+    }
+    const float na2 = dot(A.c0, A.c0) + dot(A.c1, A.c1) + dot(A.c2, A.c2);
+    const float norm_b = (h * h) / 4.0f * sqrtf(na2) / mass;
+    const float apic_v = norm_b * 6.0f * sqrtf(3.0f) / h;
+    return fminf(dt, h / (length(v) + apic_v));
+}
+
+// The bound of arena slot i (flags fl, scene sc) in 1e-12 s units.
+__device__ inline uint32_t arena_dt_bound(const Arena &a, uint32_t i, uint32_t fl, const SceneDev &sc,
+                                          const ClassDev *classes)
+{
+    m3 F, A;
+    F.c0 = mk3(a.plane(PF0)[i], a.plane(PF1)[i], a.plane(PF2)[i]);
+    F.c1 = mk3(a.plane(PF3)[i], a.plane(PF4)[i], a.plane(PF5)[i]);
+    F.c2 = mk3(a.plane(PF6)[i], a.plane(PF7)[i], a.plane(PF8)[i]);
+    A.c0 = mk3(a.plane(PC0)[i], a.plane(PC1)[i], a.plane(PC2)[i]);
+    A.c1 = mk3(a.plane(PC3)[i], a.plane(PC4)[i], a.plane(PC5)[i]);
+    A.c2 = mk3(a.plane(PC6)[i], a.plane(PC7)[i], a.plane(PC8)[i]);
+    const f3 v = mk3(a.plane(PVX)[i], a.plane(PVY)[i], a.plane(PVZ)[i]);
+    const ClassDev &cls = classes[sc.class_base + flags_class(fl)];
+    const float pdet = cls.tag == 5u ? a.plane(PS0)[i] : 1.0f;
+    return secs_to_int(particle_dt_bound(cls, sc.h, a.plane(PMASS)[i], a.plane(PVOL0)[i], v, F, A, pdet));
+}
+
+// Wave reduction per scene of the lanes' running minima (cur_s = NONE: none): one
+// atomicMin into out[s * stride] per distinct scene held by the wave's lanes.  Called by
+// every lane of the wave.
+__device__ inline void wave_scene_min(uint32_t cur_s, uint32_t cur_min, uint32_t *out, uint32_t stride)
+{
+    const uint32_t l = lane_id();
+    uint64_t pending = __ballot(cur_s != NONE);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t s = __shfl(cur_s, (int)leader, WAVE);
+        uint32_t v = (cur_s == s) ? cur_min : 0xFFFFFFFFu;
+        for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, WAVE));
+        if (l == leader) atomicMin(&out[(size_t)s * stride], v);
+        pending &= ~__ballot(cur_s == s);
+    }
+}
+
+// Adaptive scenes: the CFL bound of the state a G2P kernel has just written at sorted
+// position pos (valid lanes; every lane of the wave calls), min-reduced per scene into the
+// scene's clock for the next substep.
+__device__ inline void next_dt_bound(const WorldDev &w, uint32_t pos, bool valid, uint32_t s)
+{
+    uint32_t cand = NONE, cs = NONE;
+    if (valid) {
+        const SceneDev sc = w.scenes[s];
+        const uint32_t fl = w.out.uplane(PFLAGS)[pos];
+        if (sc.adaptive && (fl & FLAG_ENABLED)) {
+            cand = arena_dt_bound(w.out, pos, fl, sc, w.classes);
+            cs = s;
+        }
+    }
+    wave_scene_min(cs, cand, &w.clock[0].bound, (uint32_t)(sizeof(SceneClock) / 4));
+}
+
+// Start of a substep (first workgroup of its first scan): each adaptive scene's substep
+// length becomes min(dt_max, the bound of its state), written into the scene table for
+// the substep's later kernels; every scene's clock advances by its substep length.
+__device__ inline void advance_clocks(const WorldDev &w)
+{
+    if (!w.clock) return;
+    for (uint32_t s = threadIdx.x; s < w.s_count; s += blockDim.x) {
+        SceneDev &sc = const_cast<SceneDev &>(w.scenes[s]);
+        if (!sc.active) continue;
+        SceneClock &c = w.clock[s];
+        float dt = sc.dt;
+        if (sc.adaptive) {
+            dt = sc.dt_max;
+            // The bound in seconds as the reference's host converts it (timestep_bound.rs:45).
+            const float b = (float)c.bound / 1.0e12f;
+            if (c.bound != NONE && b < dt) dt = b;
+            if (!(dt > 0.0f)) dt = sc.dt_max;   // a zero bound (no finite one) keeps the maximum
+            sc.dt = dt;
+            sc.vlim = sc.h / dt;
+        }
+        c.bound = NONE;
+        c.dt = dt;
+        c.steps += 1u;
+        c.t += (double)dt;
+    }
+}
+
 // After this substep's scans: their look-back state and tickets start empty for the next
 // substep, whose keys the G2P kernels insert into the hash (emptied by the grid update).
 __device__ inline void prepare_next_sort(const WorldDev &w)
@@ -1359,6 +1482,7 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
             z = w.out.plane(PZ)[pos];
         }
         touch_keys(w, pos, valid, s, x, y, z);
+        if (w.any_adaptive) next_dt_bound(w, pos, valid, s);
     }
     prepare_next_sort(w);
     if (!CPIC) integrate_colliders_part(w);   // CPIC worlds: after k_g2p_contact, which reads them
@@ -1388,6 +1512,7 @@ __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
             z = w.out.plane(PZ)[pos];
         }
         touch_keys(w, pos, valid, s, x, y, z);
+        if (w.any_adaptive) next_dt_bound(w, pos, valid, s);
     }
 }
 
@@ -1645,51 +1770,9 @@ __global__ void __launch_bounds__(256) k_particle_cdf(WorldDev w)
     }
 }
 
-// ------------------------------------------------------------------------------------
-// CFL timestep bound (timestep_bound.rs:61-111, models/default.rs:268-330,
-// models/utils.rs:157-172) over the input arena: per enabled particle the smaller of the
-// model's sound-speed bound and the velocity bound, as 1e-12 s units (saturating),
-// min-reduced per scene.  Particles of one scene are contiguous, so a wave first
-// reduces over the lanes that share a scene and issues one atomic per scene present.
-__device__ inline uint32_t secs_to_int(float dt)
-{
-    const float x = dt * 1.0e12f;
-    if (!(x > 0.0f)) return 0u;
-    if (x >= 4294967295.0f) return 0xFFFFFFFFu;
-    return (uint32_t)x;
-}
-
-__device__ inline float particle_dt_bound(const ClassDev &cls, float h, float mass, float vol0, const f3 &v,
-                                          const m3 &F, const m3 &A, float pdet)
-{
-    const uint32_t tag = cls.tag;
-    float bulk, shear, cfl;
-    if (tag <= 1u) { bulk = cls.w[0] + 2.0f * cls.w[1] / 3.0f; shear = cls.w[1]; cfl = cls.w[2]; }
-    else if (tag <= 3u) { bulk = cls.w[10] + 2.0f * cls.w[11] / 3.0f; shear = cls.w[11]; cfl = cls.w[12]; }
-    else if (tag == 4u) { bulk = cls.w[0] * cls.w[1]; shear = 0.0f; cfl = cls.w[3]; }
-    else if (tag == 5u) {
-        // Snow: the linear elastic backbone with its moduli scaled by the hardening factor.
-        const float hard = expf(cls.w[3] * (1.0f - pdet));
-        bulk = (cls.w[4] + 2.0f * cls.w[5] / 3.0f) * hard; shear = cls.w[5] * hard; cfl = cls.w[6];
-    }
-    else { bulk = shear = cfl = 0.0f; }
-    float dt = 0.0f;
-    if (tag <= 5u) {
-        const float density0 = mass / vol0;
-        const float density = density0 / fmaxf(det(F), 1.0e-6f);
-        const float c = sqrtf((bulk + shear * 4.0f / 3.0f) / density);
-        dt = cfl * h / fmaxf(length(v), c);
-    }
-    const float na2 = dot(A.c0, A.c0) + dot(A.c1, A.c1) + dot(A.c2, A.c2);
-    const float norm_b = (h * h) / 4.0f * sqrtf(na2) / mass;
-    const float apic_v = norm_b * 6.0f * sqrtf(3.0f) / h;
-    return fminf(dt, h / (length(v) + apic_v));
-}
-
-__global__ void __launch_bounds__(256) k_timestep_bound(WorldDev w, uint32_t n, uint32_t *out)
+__global__ void __launch_bounds__(256) k_timestep_bound(WorldDev w, uint32_t n, uint32_t *out, uint32_t stride)
 {
     const uint32_t nr = (n + WAVE - 1) / WAVE * WAVE;
-    const uint32_t l = lane_id();
     uint32_t cur_s = NONE, cur_min = 0xFFFFFFFFu;     // per lane: running minimum of one scene
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
         if (i >= n) continue;
@@ -1697,35 +1780,16 @@ __global__ void __launch_bounds__(256) k_timestep_bound(WorldDev w, uint32_t n, 
         const uint32_t s = flags_scene(fl);
         const SceneDev sc = w.scenes[s];
         if (!sc.active || !(fl & FLAG_ENABLED)) continue;
-        m3 F, A;
-        F.c0 = mk3(w.in.plane(PF0)[i], w.in.plane(PF1)[i], w.in.plane(PF2)[i]);
-        F.c1 = mk3(w.in.plane(PF3)[i], w.in.plane(PF4)[i], w.in.plane(PF5)[i]);
-        F.c2 = mk3(w.in.plane(PF6)[i], w.in.plane(PF7)[i], w.in.plane(PF8)[i]);
-        A.c0 = mk3(w.in.plane(PC0)[i], w.in.plane(PC1)[i], w.in.plane(PC2)[i]);
-        A.c1 = mk3(w.in.plane(PC3)[i], w.in.plane(PC4)[i], w.in.plane(PC5)[i]);
-        A.c2 = mk3(w.in.plane(PC6)[i], w.in.plane(PC7)[i], w.in.plane(PC8)[i]);
-        const f3 v = mk3(w.in.plane(PVX)[i], w.in.plane(PVY)[i], w.in.plane(PVZ)[i]);
-        const ClassDev &cls = w.classes[sc.class_base + flags_class(fl)];
-        const float pdet = cls.tag == 5u ? w.in.plane(PS0)[i] : 1.0f;
-        const uint32_t cand = secs_to_int(particle_dt_bound(cls, sc.h, w.in.plane(PMASS)[i], w.in.plane(PVOL0)[i], v, F, A, pdet));
+        const uint32_t cand = arena_dt_bound(w.in, i, fl, sc, w.classes);
         if (s != cur_s) {
-            if (cur_s != NONE) atomicMin(&out[cur_s], cur_min);
+            if (cur_s != NONE) atomicMin(&out[(size_t)cur_s * stride], cur_min);
             cur_s = s;
             cur_min = cand;
         } else {
             cur_min = min(cur_min, cand);
         }
     }
-    // Wave reduction per scene: one atomic per distinct scene held by the wave's lanes.
-    uint64_t pending = __ballot(cur_s != NONE);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
-        const uint32_t s = __shfl(cur_s, (int)leader, WAVE);
-        uint32_t v = (cur_s == s) ? cur_min : 0xFFFFFFFFu;
-        for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, WAVE));
-        if (l == leader) atomicMin(&out[s], v);
-        pending &= ~__ballot(cur_s == s);
-    }
+    wave_scene_min(cur_s, cur_min, out, stride);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1778,6 +1842,11 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     const int cap = lc.persistent_blocks;
     if (ev) (void)hipEventRecord(ev[0], st);
     if (keys) k_keys_touch<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+    // Adaptive scenes after a host change: the bound of the input state (otherwise the
+    // previous substep's G2P left it in the clocks).
+    if (keys && w.any_adaptive && w.clock)
+        k_timestep_bound<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w, (uint32_t)lc.p_hint, &w.clock[0].bound,
+                                                                      (uint32_t)(sizeof(SceneClock) / 4));
     const int hscan = std::min((int)((w.h_total + SCAN_TILE - 1) / SCAN_TILE), cap);
     k_scan<MODE_SLOTS><<<std::max(hscan, 1), 256, 0, st>>>(w);
     k_block_count<<<grid_for(std::max<uint64_t>(lc.p_hint, (uint64_t)w.b_total * 16), T, cap), T, 0, st>>>(w);
@@ -1839,9 +1908,9 @@ void launch_clear(const WorldDev &w, const LaunchConfig &lc, hipStream_t st)
     k_clear<<<grid_for(std::max(std::max(w.h_total, w.b_total), 2u * w.scan_stride), T, lc.persistent_blocks), T, 0, st>>>(w);
 }
 
-void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, int max_blocks, hipStream_t st)
+void launch_timestep_bound(const WorldDev &w, uint32_t n, uint32_t *out, uint32_t stride, int max_blocks, hipStream_t st)
 {
-    k_timestep_bound<<<grid_for(n, 256, max_blocks), 256, 0, st>>>(w, n, out);
+    k_timestep_bound<<<grid_for(n, 256, max_blocks), 256, 0, st>>>(w, n, out, stride);
 }
 
 void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uint32_t *buckets, uint32_t p_hint,

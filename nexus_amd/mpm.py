@@ -40,7 +40,8 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
-    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
+    "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound",
+    "nxm_scene_set_adaptive_dt", "nxm_scene_read_clock", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config",
     "nxm_batch_get_autotune", "nxm_scene_get_autotune", "nxm_scene_read_positions", "nxm_scene_read_particles_soa", "nxm_scene_sort_size",
     "nxm_scene_read_sort_canonical", "nxm_scene_read_grid", "nxm_scene_synchronize", "nxm_scene_set_debug_grid",
@@ -173,6 +174,10 @@ def library() -> ct.CDLL:
     L.nxm_debug_fail_uploads.argtypes = [ct.c_uint32]
     L.nxm_scene_set_cpic.argtypes = [P, I]
     L.nxm_pipeline_step_noalloc.argtypes = [P, P, P, ct.c_uint32, P, P]
+    L.nxm_scene_set_adaptive_dt.argtypes = [P, ct.c_int]
+    L.nxm_scene_set_adaptive_dt.restype = ct.c_int
+    L.nxm_scene_read_clock.argtypes = [P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_float), ct.POINTER(ct.c_uint64)]
+    L.nxm_scene_read_clock.restype = ct.c_int
     L.nxm_pipeline_step_noalloc.restype = ct.c_int
     L.nxm_scene_stage_upload.argtypes = [P]
     L.nxm_scene_read_colliders.argtypes = [P, P, ct.c_size_t]
@@ -436,6 +441,16 @@ class MpmState:
         t, k = ct.c_int(0), ct.c_uint32(0)
         _check(self.lib.nxm_scene_get_autotune(self.handle, ct.byref(t), ct.byref(k)))
         return bool(t.value), int(k.value)
+
+    def set_adaptive_dt(self, enabled: bool = True) -> None:
+        """Each substep's length becomes min(dt, the CFL bound of the state at its start)."""
+        _check(self.lib.nxm_scene_set_adaptive_dt(self.handle, int(bool(enabled))))
+
+    def read_clock(self) -> tuple[float, float, int]:
+        """(simulated time, last substep's length, substeps) of this scene."""
+        t, dt, k = ct.c_double(0), ct.c_float(0), ct.c_uint64(0)
+        _check(self.lib.nxm_scene_read_clock(self.handle, ct.byref(t), ct.byref(dt), ct.byref(k)))
+        return float(t.value), float(dt.value), int(k.value)
 
     def timestep_bound(self) -> float:
         """CFL bound on the substep length of the current state (WgTimestepBounds)."""

@@ -112,8 +112,12 @@ class Server:
     continuous batching at substep granularity."""
 
     def __init__(self, pipeline: mpm.MpmPipeline, hbm_budget_bytes: float = 200e9,
-                 max_active_particles: int | None = None, ingest_workers: int = 4):
+                 max_active_particles: int | None = None, ingest_workers: int = 4, adaptive_dt: bool = False):
+        """adaptive_dt: every admitted scene takes adaptive substeps (each substep's length
+        min(its dt, the CFL bound of its state), chosen on the device), so that a request
+        whose dt is too long for its material or speed still steps stably."""
         self.pipeline = pipeline
+        self.adaptive_dt = bool(adaptive_dt)
         self.budget = float(hbm_budget_bytes)
         self.max_particles = max_active_particles
         self.ingest_workers = max(1, int(ingest_workers))
@@ -126,6 +130,8 @@ class Server:
         sc = r.scene if isinstance(r.scene, _scenes.Scene) else r.scene.build()
         r.n = sc.n
         r.state = mpm.MpmState.from_scene(self.pipeline, sc)   # host-side until admitted
+        if self.adaptive_dt:
+            r.state.set_adaptive_dt(True)
         r.state.stage_upload()          # packed here, so that admission only queues copies
         r.bytes = r.state.device_bytes()
 

@@ -213,6 +213,18 @@ class Oracle:
             if rc != 0:
                 raise RuntimeError(f"oracle step failed: {rc}")
 
+    def step_adaptive(self, dt_max: float) -> float:
+        """One substep whose length is min(dt_max, the CFL bound of the current state),
+        the bound converted to seconds as the reference's host does (units / 1e12 in f32,
+        src_mpm/solver/timestep_bound.rs:45); returns that length."""
+        units, secs = timestep_bound(self.s)
+        dt = np.float32(dt_max)
+        if units != 0xFFFFFFFF and units > 0 and np.float32(secs) < dt:
+            dt = np.float32(secs)
+        self.s.sim[3] = dt
+        self.step(1)
+        return float(dt)
+
     # ---- inspection of the last substep ----
     def blocks(self):
         L, ws = self.L, self.ws

@@ -83,3 +83,26 @@ def test_served_on_the_wall_clock_equals_alone(pipeline):
         assert r.steps == r.horizon
         for f in ("x", "v", "C", "F", "enabled"):
             assert np.array_equal(r.result[f], ref[f]), (r.rid, f)
+
+
+def test_served_adaptive_scenes_equal_alone(pipeline):
+    """With adaptive substeps every served scene takes its own substep lengths on the
+    device; each result equals the scene stepped alone with adaptive
+    substeps, bit for bit."""
+    mix = _mix()
+    alone = []
+    for _, sc, horizon in mix:
+        st = mpm.MpmState.from_scene(pipeline, sc)
+        st.set_adaptive_dt(True)
+        for _ in range(horizon):
+            pipeline.step(st)
+        alone.append(st.read_particles())
+        st.close()
+    reqs = [serve.Request(a, sc, h, rid=i, keep="particles") for i, (a, sc, h) in enumerate(mix)]
+    srv = serve.Server(pipeline, adaptive_dt=True)
+    rep = srv.run(reqs, clock="steps")
+    srv.close()
+    assert rep.completed == len(mix)
+    for r, ref in zip(reqs, alone):
+        for f in ("x", "v", "C", "F", "enabled"):
+            assert np.array_equal(r.result[f], ref[f]), (r.rid, f)

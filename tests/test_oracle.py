@@ -524,3 +524,22 @@ def test_timestep_bound_minimum_and_disabled():
     assert O.timestep_bound(o.s)[0] == ua
     o.s.enabled[:] = 0
     assert O.timestep_bound(o.s)[0] == 0xFFFFFFFF
+
+
+def test_adaptive_schedule_is_min_of_dt_and_the_bound():
+    """The oracle's adaptive substep (the schedule the GPU's adaptive scenes are checked
+    against): a block at rest takes the P-wave CFL bound cfl h / sqrt(M / rho) when dt is
+    larger, and dt itself when dt is smaller."""
+    from nexus_amd import scenes
+    h = 1 / 64
+    idx = np.stack(np.meshgrid(*(np.arange(6),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = 0.5 + (idx + 0.5) * h / 2
+    lam, mu = scenes.lame_lambda_mu(1e5, 0.3)
+    cfl_dt = 0.5 * h / np.sqrt((float(lam) + 2 * float(mu)) / 1000.0)
+    for dt_max, expect in ((1e-2, cfl_dt), (1e-4, 1e-4)):
+        sc = scenes.make_scene(scenes.particles(pos, h / 4, 1000.0, scenes.model_neo_hookean(1e5, 0.3)), h, dt_max,
+                               gravity=(0.0, 0.0, 0.0))
+        o = O.Oracle(sc, 64)
+        got = o.step_adaptive(sc.dt)
+        assert abs(got - expect) <= 1e-6 * expect, (got, expect)
+        assert o.s.sim[3] == np.float32(got)
