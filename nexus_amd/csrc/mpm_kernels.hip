@@ -608,29 +608,59 @@ template <bool CPIC> constexpr int bad_any() { return (tile_fields<CPIC>() * TIL
 // ones, or the incompatible ones.
 constexpr int PART_ALL = 0, PART_COMPATIBLE = 1, PART_INCOMPATIBLE = 2;
 
-template <bool CHECK, bool CPIC, int PART>
-__device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int rot,
-                                            const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wxy)[3][3],
-                                            float wz0, float wz1, float wz2, float smom, float msm,
+// K lanes share one particle: K = 1 scatters all 27 stencil nodes, starting at z slab
+// sel (the lane's rotation, so that particles of one cell in one wave add to different
+// nodes at the same instruction); K = 3 the 9 nodes of z slab sel; K = 9 the 3 nodes of
+// row (sz, sy) = (sel / 3, sel % 3); K = 27 node (sel / 9, sel / 3 % 3, sel % 3).  Every
+// node's contribution is computed by the same operations whatever K is.
+template <bool CHECK, bool CPIC, int PART, int K>
+__device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *bad, int bx, int by, int bz, int sel,
+                                            const f3 base, const f3 hx, const f3 hy, const f3 hz, const float (&wx)[3],
+                                            const float (&wy)[3], const float (&wz)[3], float smom, float msm,
                                             const uint32_t *naff, uint32_t pa)
 {
+    constexpr int NZ = K == 1 ? 3 : 1, NY = K <= 3 ? 3 : 1, NX = K <= 9 ? 3 : 1;
     #pragma unroll
-    for (int kz = 0; kz < 3; ++kz) {
-        const int sz = kz + rot >= 3 ? kz + rot - 3 : kz + rot;
+    for (int kz = 0; kz < NZ; ++kz) {
+        int sz;
+        if (K == 1) sz = kz + sel >= 3 ? kz + sel - 3 : kz + sel;
+        else if (K == 3) sz = sel;
+        else if (K == 9) sz = sel / 3;
+        else sz = sel / 9;
         // base + sz hz, exactly as base + hz or base + 2 hz (sz hz is exact).
         const float fsz = (float)sz;
         const f3 az = mk3(__builtin_fmaf(hz.x, fsz, base.x), __builtin_fmaf(hz.y, fsz, base.y),
                           __builtin_fmaf(hz.z, fsz, base.z));
-        float wzs = wz0;
-        wzs = sz == 1 ? wz1 : wzs;
-        wzs = sz == 2 ? wz2 : wzs;
+        float wzs = wz[0];
+        wzs = sz == 1 ? wz[1] : wzs;
+        wzs = sz == 2 ? wz[2] : wzs;
         #pragma unroll
-        for (int sy = 0; sy < 3; ++sy) {
-            const f3 ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
+        for (int ky = 0; ky < NY; ++ky) {
+            const int sy = NY == 3 ? ky : (K == 9 ? sel % 3 : (sel / 3) % 3);
+            f3 ay;
+            float wys;
+            if (NY == 3) {
+                ay = sy == 0 ? az : (sy == 1 ? az + hy : az + hy * 2.0f);
+                wys = wy[sy];
+            } else {
+                const f3 a1 = az + hy, a2 = az + hy * 2.0f;
+                ay = sy == 0 ? az : (sy == 1 ? a1 : a2);
+                wys = sy == 0 ? wy[0] : (sy == 1 ? wy[1] : wy[2]);
+            }
             #pragma unroll
-            for (int sx = 0; sx < 3; ++sx) {
-                const f3 a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
-                const float wgt = wxy[sy][sx] * wzs;
+            for (int kx = 0; kx < NX; ++kx) {
+                const int sx = NX == 3 ? kx : sel % 3;
+                f3 a;
+                float wxs;
+                if (NX == 3) {
+                    a = sx == 0 ? ay : (sx == 1 ? ay + hx : ay + hx * 2.0f);
+                    wxs = wx[sx];
+                } else {
+                    const f3 a1 = ay + hx, a2 = ay + hx * 2.0f;
+                    a = sx == 0 ? ay : (sx == 1 ? a1 : a2);
+                    wxs = sx == 0 ? wx[0] : (sx == 1 ? wx[1] : wx[2]);
+                }
+                const float wgt = (wxs * wys) * wzs;
                 const f3 vc = a * (wgt * smom);
                 const float mw = msm * wgt;
                 const int tn = (bx + sx) + TILE * (by + sy) + TILE * TILE * (bz + sz);
@@ -727,9 +757,9 @@ __device__ inline void p2g_bound(const P2GIn &p, float h, float &bmom, float &bm
 }
 
 // Scatter of one enabled particle of block vid into the tile.
-template <bool CPIC, int PART>
+template <bool CPIC, int PART, int K = 1>
 __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
-                                        const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t l,
+                                        const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t sel,
                                         const uint32_t *naff, uint32_t pa)
 {
     const float h = sc.h;
@@ -773,19 +803,51 @@ __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad,
     // two, so folding them into the weight changes no bits.
     const f3 base = mul(C, d0) + mom;
     const f3 hx = C.c0 * h, hy = C.c1 * h, hz = C.c2 * h;
-    float wxy[3][3];
-    #pragma unroll
-    for (int sy = 0; sy < 3; ++sy) {
-        #pragma unroll
-        for (int sx = 0; sx < 3; ++sx) wxy[sy][sx] = wx[sx] * wy[sy];
-    }
     const float msm = m * sm;
-    const int rot = (int)(l % 3u);
     if (inside) {
         if (suspect)
-            p2g_scatter<true, CPIC, PART>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
+            p2g_scatter<true, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
         else
-            p2g_scatter<false, CPIC, PART>(tile, bad, bx, by, bz, rot, base, hx, hy, hz, wxy, wz[0], wz[1], wz[2], smom, msm, naff, pa);
+            p2g_scatter<false, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
+    }
+}
+
+// Scatter of the particle at sorted position pos (when enabled; the incompatible pass
+// takes only particles with affinity bits).
+template <bool CPIC, int PART, int K>
+__device__ __forceinline__ void p2g_pos(const WorldDev &w, unsigned long long *tile, uint32_t *bad, uint32_t pos,
+                                        const int4 v, const SceneDev &sc, float smom, float sm, float bmom, float bm,
+                                        uint32_t sel, const uint32_t *naff)
+{
+    const uint32_t slot = w.perm[pos];
+    if (!(w.in.uplane(PFLAGS)[slot] & FLAG_ENABLED)) return;
+    const uint32_t pa = PART == PART_ALL ? 0u : w.out.uplane(PAFF)[pos];
+    if (PART == PART_INCOMPATIBLE && (pa & 0xFFFFu) == 0u) return;
+    p2g_one<CPIC, PART, K>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, sel, naff, pa);
+}
+
+// One scatter pass over the np particles of a block starting at sorted position first:
+// whole chunks of 64 one particle per lane; the remainder r with K lanes per particle,
+// K = 27, 9, 3 or 1 for r <= 2, 7, 21 or more, so that a small remainder costs a fraction
+// of a full 27-node pass.
+template <bool CPIC, int PART>
+__device__ __forceinline__ void p2g_block_pass(const WorldDev &w, unsigned long long *tile, uint32_t *bad, uint32_t first,
+                                               uint32_t np, const int4 v, const SceneDev &sc, float smom, float sm,
+                                               float bmom, float bm, uint32_t l, const uint32_t *naff)
+{
+    const uint32_t full = np / WAVE * WAVE, rem = np - full;
+    for (uint32_t c = 0; c < full; c += WAVE)
+        p2g_pos<CPIC, PART, 1>(w, tile, bad, first + c + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
+    if (rem == 0u) return;
+    const uint32_t at = first + full;
+    if (rem <= 2u) {
+        if (l < 27u * rem) p2g_pos<CPIC, PART, 27>(w, tile, bad, at + l / 27u, v, sc, smom, sm, bmom, bm, l % 27u, naff);
+    } else if (rem <= 7u) {
+        if (l < 9u * rem) p2g_pos<CPIC, PART, 9>(w, tile, bad, at + l / 9u, v, sc, smom, sm, bmom, bm, l % 9u, naff);
+    } else if (rem <= 21u) {
+        if (l < 3u * rem) p2g_pos<CPIC, PART, 3>(w, tile, bad, at + l / 3u, v, sc, smom, sm, bmom, bm, l % 3u, naff);
+    } else if (l < rem) {
+        p2g_pos<CPIC, PART, 1>(w, tile, bad, at + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
     }
 }
 
@@ -824,7 +886,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         const float h = sc.h;
         const uint32_t first = w.bfirst[d];
         float bmom = 0.0f, bm = 0.0f;
-        if (np <= (uint32_t)WAVE) {
+        if (np <= (uint32_t)WAVE && np > 21u) {
             // One particle per lane: load it once, bound, then scatter from registers.
             P2GIn p;
             bool on = false;
@@ -847,8 +909,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             if (inc) load_tile_affinity(w, d, true, naff, l);
             __syncthreads();
             if (on) {
-                if (inc) p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
-                else p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+                if (inc) p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
+                else p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
             }
             p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
             if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
@@ -857,12 +919,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 p2g_clear(tile, bad, l);
                 __syncthreads();
                 if (on && (pa & 0xFFFFu) != 0u)
-                    p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l, naff, pa);
+                    p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
                 p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
             }
             continue;
         }
-        // Pass 1: bounds over the block; pass 2: scatter, 64 particles at a time.
+        // Pass 1: bounds over the block; pass 2: the scatter (p2g_block_pass).
         bool anyc = false;
         for (uint32_t c = 0; c < np; c += WAVE) {
             const uint32_t pos = first + c + l;
@@ -882,35 +944,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         p2g_clear(tile, bad, l);
         if (inc) load_tile_affinity(w, d, true, naff, l);
         __syncthreads();
-        for (uint32_t c = 0; c < np; c += WAVE) {
-            const uint32_t pos = first + c + l;
-            if (pos < first + np) {
-                const uint32_t slot = w.perm[pos];
-                if (FL[slot] & FLAG_ENABLED) {
-                    if (inc) {
-                        const uint32_t pa = w.out.uplane(PAFF)[pos];
-                        p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
-                    } else {
-                        p2g_one<CPIC, PART_ALL>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, 0u);
-                    }
-                }
-            }
-        }
+        if (inc) p2g_block_pass<CPIC, PART_COMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
+        else p2g_block_pass<CPIC, PART_ALL>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
         p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
         if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
         if (CPIC && inc) {
             // Second pass: the incompatible contributions of the particles with affinity.
             p2g_clear(tile, bad, l);
             __syncthreads();
-            for (uint32_t c = 0; c < np; c += WAVE) {
-                const uint32_t pos = first + c + l;
-                if (pos < first + np) {
-                    const uint32_t slot = w.perm[pos];
-                    const uint32_t pa = w.out.uplane(PAFF)[pos];
-                    if ((FL[slot] & FLAG_ENABLED) && (pa & 0xFFFFu) != 0u)
-                        p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, l, naff, pa);
-                }
-            }
+            p2g_block_pass<CPIC, PART_INCOMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
             p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
         }
     }
