@@ -237,6 +237,15 @@ enum ScanMode { MODE_SLOTS = 0, MODE_COUNTS = 1 };
 template <int MODE>
 __device__ inline uint32_t scan_len(const WorldDev &w) { return MODE == MODE_SLOTS ? w.h_total : w.b_total; }
 
+// Runs of SCAN_TILE elements per scan tile for a scan of len elements (at most
+// SCAN_MAX_TILES tiles).
+constexpr uint32_t SCAN_MAX_TILES = 1024;
+__host__ __device__ inline uint32_t scan_chunks(uint32_t len)
+{
+    const uint32_t per = (uint32_t)SCAN_TILE * SCAN_MAX_TILES;
+    return len <= per ? 1u : (uint32_t)(((uint64_t)len + per - 1) / per);
+}
+
 template <int MODE>
 __device__ inline uint32_t scan_in(const WorldDev &w, uint32_t i)
 {
@@ -322,9 +331,14 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
     __shared__ uint32_t sh[4];
     __shared__ uint32_t s_tile, s_prefix;
     const uint32_t len = MODE == MODE_SLOTS ? w.h_total : min(w.g->n_blocks, w.b_total);
-    // The state array has scan_stride words per scan (the host sizes it for len); the
-    // bound keeps every tile's word inside it whatever the host passed.
-    const uint32_t ntiles = min((len + SCAN_TILE - 1) / SCAN_TILE, w.scan_stride - 1u);
+    // A tile is `chunks` runs of SCAN_TILE elements, so that there are at most
+    // SCAN_MAX_TILES tiles: every tile takes a ticket, one same-address atomic, and on a
+    // hash of millions of slots those tickets, not the data, would set the time.  The state
+    // array has scan_stride words per scan (the host sizes it for SCAN_TILE tiles); the bound
+    // keeps every tile's word inside it whatever the host passed.
+    const uint32_t chunks = scan_chunks(len);
+    const uint32_t tile_len = SCAN_TILE * chunks;
+    const uint32_t ntiles = min((len + tile_len - 1) / tile_len, w.scan_stride - 1u);
     if (MODE == MODE_SLOTS) {
         // The block counts of this substep start at zero (k_block_count adds to them; the
         // previous substep's stay readable until here).
@@ -336,15 +350,32 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= ntiles) break;
+        // One run (a small scan): thread elements i1 .. i1 + PER - 1, one block scan.  Several
+        // runs (a large scan): element i0 + 256 k of each run (k < PER) belongs to this
+        // thread, row k of the run being 256 consecutive elements, so that loads and stores
+        // are coalesced; a row's exclusive scan plus the rows before it give the rank.
         uint32_t v[PER], sum = 0;
-        const uint32_t i0 = t * SCAN_TILE + PER * threadIdx.x;
-        #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            v[k] = i0 + k < len ? scan_in<MODE>(w, i0 + k) : 0u;
-            sum += v[k];
+        const uint32_t i0 = t * tile_len + threadIdx.x, i1 = t * tile_len + PER * threadIdx.x;
+        uint32_t total = 0, ex = 0;
+        if (chunks == 1u) {
+            #pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                v[k] = i1 + k < len ? scan_in<MODE>(w, i1 + k) : 0u;
+                sum += v[k];
+            }
+            ex = block_excl_scan_256(sum, sh, total);
+        } else {
+            // First pass: the tile's total; the second (below) reads the runs again.
+            for (uint32_t c = 0; c < chunks; ++c) {
+                const uint32_t ic = i0 + c * SCAN_TILE;
+                #pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const uint32_t i = ic + 256u * k;
+                    sum += i < len ? scan_in<MODE>(w, i) : 0u;
+                }
+            }
+            total = block_reduce_256(sum, sh);
         }
-        uint32_t total;
-        const uint32_t ex = block_excl_scan_256(sum, sh, total);
         if (threadIdx.x < WAVE) {
             // Look-back by the first wave, 64 predecessors at a time: lane l reads tile
             // hi - l; the nearest tile with an inclusive prefix ends the walk once every
@@ -385,10 +416,30 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
             }
         }
         __syncthreads();
-        uint32_t r = s_prefix + ex;
-        for (int k = 0; k < PER; ++k) {
-            if (i0 + k < len) scan_out<MODE>(w, i0 + k, r, v[k]);
-            r += v[k];
+        if (chunks == 1u) {
+            uint32_t r = s_prefix + ex;
+            for (int k = 0; k < PER; ++k) {
+                if (i1 + k < len) scan_out<MODE>(w, i1 + k, r, v[k]);
+                r += v[k];
+            }
+        } else {
+            uint32_t run = s_prefix;
+            for (uint32_t c = 0; c < chunks; ++c) {
+                const uint32_t ic = i0 + c * SCAN_TILE;
+                #pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const uint32_t i = ic + 256u * k;
+                    v[k] = i < len ? scan_in<MODE>(w, i) : 0u;
+                }
+                #pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    uint32_t rt;
+                    const uint32_t e = block_excl_scan_256(v[k], sh, rt);
+                    const uint32_t i = ic + 256u * k;
+                    if (i < len) scan_out<MODE>(w, i, run + e, v[k]);
+                    run += rt;
+                }
+            }
         }
     }
 }
@@ -692,7 +743,7 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
 // block's node order in the staging array st, so the consumer reads each region as one run
 // of float4.  Ends with the tile free for the next pass.
 __device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const uint32_t *bad, float4 *stage,
-                                          uint32_t d, float smom, float sm, uint32_t l)
+                                          uint32_t d, float smom, float sm, uint32_t l, uint32_t touched)
 {
     __syncthreads();
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
@@ -702,6 +753,7 @@ __device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const 
         const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
         const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
         const int e = 4 * ex + 2 * ey + ez;
+        if (!((touched >> e) & 1u)) continue;   // no particle's stencil reaches region e
         const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
         const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
         float v4[4];
@@ -756,9 +808,27 @@ __device__ inline void p2g_bound(const P2GIn &p, float h, float &bmom, float &bm
     bm = fmaxf(bm, fabsf(p.m));
 }
 
+// Staging regions of a block's tile that a particle with base cell (bx, by, bz) in the
+// block reaches: region e = 4 ex + 2 ey + ez (the part of the +e neighbour in the tile)
+// needs base cell coordinates >= 2 on the axes where e is 1 (the reference's spill rule,
+// grid/sort.rs:322-328).
+__device__ inline uint32_t spill_mask(int bx, int by, int bz)
+{
+    const bool sx = bx >= 2, sy = by >= 2, sz = bz >= 2;
+    uint32_t m = 1u;
+    if (sz) m |= 1u << 1;
+    if (sy) m |= 1u << 2;
+    if (sy && sz) m |= 1u << 3;
+    if (sx) m |= 1u << 4;
+    if (sx && sz) m |= 1u << 5;
+    if (sx && sy) m |= 1u << 6;
+    if (sx && sy && sz) m |= 1u << 7;
+    return m;
+}
+
 // Scatter of one enabled particle of block vid into the tile.
 template <bool CPIC, int PART, int K = 1>
-__device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
+__device__ __forceinline__ uint32_t p2g_one(unsigned long long *tile, uint32_t *bad, const P2GIn &p, const int4 v,
                                         const SceneDev &sc, float smom, float sm, float bmom, float bm, uint32_t sel,
                                         const uint32_t *naff, uint32_t pa)
 {
@@ -804,26 +874,26 @@ __device__ __forceinline__ void p2g_one(unsigned long long *tile, uint32_t *bad,
     const f3 base = mul(C, d0) + mom;
     const f3 hx = C.c0 * h, hy = C.c1 * h, hz = C.c2 * h;
     const float msm = m * sm;
-    if (inside) {
-        if (suspect)
-            p2g_scatter<true, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
-        else
-            p2g_scatter<false, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
-    }
+    if (!inside) return 0u;
+    if (suspect)
+        p2g_scatter<true, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
+    else
+        p2g_scatter<false, CPIC, PART, K>(tile, bad, bx, by, bz, (int)sel, base, hx, hy, hz, wx, wy, wz, smom, msm, naff, pa);
+    return spill_mask(bx, by, bz);
 }
 
 // Scatter of the particle at sorted position pos (when enabled; the incompatible pass
 // takes only particles with affinity bits).
 template <bool CPIC, int PART, int K>
-__device__ __forceinline__ void p2g_pos(const WorldDev &w, unsigned long long *tile, uint32_t *bad, uint32_t pos,
-                                        const int4 v, const SceneDev &sc, float smom, float sm, float bmom, float bm,
-                                        uint32_t sel, const uint32_t *naff)
+__device__ __forceinline__ uint32_t p2g_pos(const WorldDev &w, unsigned long long *tile, uint32_t *bad, uint32_t pos,
+                                            const int4 v, const SceneDev &sc, float smom, float sm, float bmom, float bm,
+                                            uint32_t sel, const uint32_t *naff)
 {
     const uint32_t slot = w.perm[pos];
-    if (!(w.in.uplane(PFLAGS)[slot] & FLAG_ENABLED)) return;
+    if (!(w.in.uplane(PFLAGS)[slot] & FLAG_ENABLED)) return 0u;
     const uint32_t pa = PART == PART_ALL ? 0u : w.out.uplane(PAFF)[pos];
-    if (PART == PART_INCOMPATIBLE && (pa & 0xFFFFu) == 0u) return;
-    p2g_one<CPIC, PART, K>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, sel, naff, pa);
+    if (PART == PART_INCOMPATIBLE && (pa & 0xFFFFu) == 0u) return 0u;
+    return p2g_one<CPIC, PART, K>(tile, bad, p2g_load(w, slot), v, sc, smom, sm, bmom, bm, sel, naff, pa);
 }
 
 // One scatter pass over the np particles of a block starting at sorted position first:
@@ -831,24 +901,30 @@ __device__ __forceinline__ void p2g_pos(const WorldDev &w, unsigned long long *t
 // K = 27, 9, 3 or 1 for r <= 2, 7, 21 or more, so that a small remainder costs a fraction
 // of a full 27-node pass.
 template <bool CPIC, int PART>
-__device__ __forceinline__ void p2g_block_pass(const WorldDev &w, unsigned long long *tile, uint32_t *bad, uint32_t first,
-                                               uint32_t np, const int4 v, const SceneDev &sc, float smom, float sm,
-                                               float bmom, float bm, uint32_t l, const uint32_t *naff)
+__device__ __forceinline__ uint32_t p2g_block_pass(const WorldDev &w, unsigned long long *tile, uint32_t *bad,
+                                                   uint32_t first, uint32_t np, const int4 v, const SceneDev &sc,
+                                                   float smom, float sm, float bmom, float bm, uint32_t l,
+                                                   const uint32_t *naff)
 {
     const uint32_t full = np / WAVE * WAVE, rem = np - full;
+    uint32_t spill = 0u;
     for (uint32_t c = 0; c < full; c += WAVE)
-        p2g_pos<CPIC, PART, 1>(w, tile, bad, first + c + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
-    if (rem == 0u) return;
+        spill |= p2g_pos<CPIC, PART, 1>(w, tile, bad, first + c + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
+    if (rem == 0u) return spill;
     const uint32_t at = first + full;
     if (rem <= 2u) {
-        if (l < 27u * rem) p2g_pos<CPIC, PART, 27>(w, tile, bad, at + l / 27u, v, sc, smom, sm, bmom, bm, l % 27u, naff);
+        if (l < 27u * rem)
+            spill |= p2g_pos<CPIC, PART, 27>(w, tile, bad, at + l / 27u, v, sc, smom, sm, bmom, bm, l % 27u, naff);
     } else if (rem <= 7u) {
-        if (l < 9u * rem) p2g_pos<CPIC, PART, 9>(w, tile, bad, at + l / 9u, v, sc, smom, sm, bmom, bm, l % 9u, naff);
+        if (l < 9u * rem)
+            spill |= p2g_pos<CPIC, PART, 9>(w, tile, bad, at + l / 9u, v, sc, smom, sm, bmom, bm, l % 9u, naff);
     } else if (rem <= 21u) {
-        if (l < 3u * rem) p2g_pos<CPIC, PART, 3>(w, tile, bad, at + l / 3u, v, sc, smom, sm, bmom, bm, l % 3u, naff);
+        if (l < 3u * rem)
+            spill |= p2g_pos<CPIC, PART, 3>(w, tile, bad, at + l / 3u, v, sc, smom, sm, bmom, bm, l % 3u, naff);
     } else if (l < rem) {
-        p2g_pos<CPIC, PART, 1>(w, tile, bad, at + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
+        spill |= p2g_pos<CPIC, PART, 1>(w, tile, bad, at + l, v, sc, smom, sm, bmom, bm, l % 3u, naff);
     }
+    return spill;
 }
 
 // Node affinity words of block d's 6^3 tile (CPIC scenes; zero, compatible with every
@@ -860,6 +936,26 @@ __device__ inline void load_tile_affinity(const WorldDev &w, uint32_t d, bool cp
         const int e = 4 * (tx >> 2) + 2 * (ty >> 2) + (tz >> 2);
         const uint32_t blk = cpic ? w.bup[8 * (size_t)d + e] : NONE;
         naff[t] = blk == NONE ? 0u : w.ncdf[(size_t)blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3)].y;
+    }
+}
+
+// The union over the wave of the lanes' region masks.
+__device__ inline uint32_t wave_or_mask8(uint32_t m)
+{
+    uint32_t out = 0u;
+    #pragma unroll
+    for (int e = 0; e < 8; ++e) out |= __ballot((m >> e) & 1u) ? (1u << e) : 0u;
+    return out;
+}
+
+// Regions of block d that no particle reached are neither flushed nor read: the block
+// receiving region e (bup[d][e]) stops listing d as its source e (bdn), so the grid update
+// skips it.  Only block d writes that entry, and the grid update reads it after P2G.
+__device__ inline void unlink_untouched(const WorldDev &w, uint32_t d, uint32_t touched, uint32_t l)
+{
+    if (l < 8u && !((touched >> l) & 1u)) {
+        const uint32_t dst = w.bup[8 * (size_t)d + l];
+        if (dst != NONE) w.bdn[8 * (size_t)dst + l] = NONE;
     }
 }
 
@@ -890,7 +986,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             // One particle per lane: load it once, bound, then scatter from registers.
             P2GIn p;
             bool on = false;
-            uint32_t pa = 0u;
+            uint32_t pa = 0u, spill = 0u;
             if (l < np) {
                 const uint32_t slot = w.perm[first + l];
                 on = (FL[slot] & FLAG_ENABLED) != 0u;
@@ -909,10 +1005,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             if (inc) load_tile_affinity(w, d, true, naff, l);
             __syncthreads();
             if (on) {
-                if (inc) p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
-                else p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
+                if (inc) spill = p2g_one<CPIC, PART_COMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
+                else spill = p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
             }
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
+            const uint32_t touched = wave_or_mask8(spill);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched);
             if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
             if (CPIC && inc) {
                 // Second pass: the incompatible contributions of the particles with affinity.
@@ -920,8 +1017,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 __syncthreads();
                 if (on && (pa & 0xFFFFu) != 0u)
                     p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
-                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
+                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched);
             }
+            unlink_untouched(w, d, touched, l);
             continue;
         }
         // Pass 1: bounds over the block; pass 2: the scatter (p2g_block_pass).
@@ -944,17 +1042,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         p2g_clear(tile, bad, l);
         if (inc) load_tile_affinity(w, d, true, naff, l);
         __syncthreads();
-        if (inc) p2g_block_pass<CPIC, PART_COMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
-        else p2g_block_pass<CPIC, PART_ALL>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
-        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l);
+        const uint32_t spill =
+            inc ? p2g_block_pass<CPIC, PART_COMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff)
+                : p2g_block_pass<CPIC, PART_ALL>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
+        const uint32_t touched = wave_or_mask8(spill);
+        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched);
         if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
         if (CPIC && inc) {
             // Second pass: the incompatible contributions of the particles with affinity.
             p2g_clear(tile, bad, l);
             __syncthreads();
             p2g_block_pass<CPIC, PART_INCOMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched);
         }
+        unlink_untouched(w, d, touched, l);
     }
 }
 
@@ -1499,7 +1600,7 @@ __device__ inline void prepare_next_sort(const WorldDev &w)
 // input arena, in the same order, so the next substep starts at its scan.  That second
 // loop runs whole waves (lanes past the range are invalid) over the positions this same
 // thread has just written.
-template <bool CPIC>
+template <bool CPIC, bool ADAPT>
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
     uint32_t pbegin, pend;
@@ -1524,13 +1625,14 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
             z = w.out.plane(PZ)[pos];
         }
         touch_keys(w, pos, valid, s, x, y, z);
-        if (w.any_adaptive) next_dt_bound(w, pos, valid, s);
+        if (ADAPT) next_dt_bound(w, pos, valid, s);
     }
     prepare_next_sort(w);
     if (!CPIC) integrate_colliders_part(w);   // CPIC worlds: after k_g2p_contact, which reads them
 }
 
 // The contact particles listed by k_particle_cdf, then their keys (as in k_g2p_update).
+template <bool ADAPT>
 __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
 {
     const uint32_t n = w.g->n_contact;
@@ -1554,7 +1656,7 @@ __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
             z = w.out.plane(PZ)[pos];
         }
         touch_keys(w, pos, valid, s, x, y, z);
-        if (w.any_adaptive) next_dt_bound(w, pos, valid, s);
+        if (ADAPT) next_dt_bound(w, pos, valid, s);
     }
 }
 
@@ -1889,10 +1991,12 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
     if (keys && w.any_adaptive && w.clock)
         k_timestep_bound<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w, (uint32_t)lc.p_hint, &w.clock[0].bound,
                                                                       (uint32_t)(sizeof(SceneClock) / 4));
-    const int hscan = std::min((int)((w.h_total + SCAN_TILE - 1) / SCAN_TILE), cap);
+    const uint32_t htile = SCAN_TILE * scan_chunks(w.h_total);
+    const int hscan = std::min((int)((w.h_total + htile - 1) / htile), cap);
     k_scan<MODE_SLOTS><<<std::max(hscan, 1), 256, 0, st>>>(w);
     k_block_count<<<grid_for(std::max<uint64_t>(lc.p_hint, (uint64_t)w.b_total * 16), T, cap), T, 0, st>>>(w);
-    const int bscan = std::min((int)((w.b_total + SCAN_TILE - 1) / SCAN_TILE), cap);
+    const uint32_t btile = SCAN_TILE * scan_chunks(w.b_total);
+    const int bscan = std::min((int)((w.b_total + btile - 1) / btile), cap);
     k_scan<MODE_COUNTS><<<std::max(bscan, 1), 256, 0, st>>>(w);
     k_scatter<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     if (ev) (void)hipEventRecord(ev[1], st);
@@ -1915,10 +2019,12 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
             (void)hipStreamWaitEvent(lc.side, lc.fork, 0);
             cs = lc.side;
         }
-        k_g2p_contact<<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
-        if (ev) hipExtLaunchKernelGGL(k_g2p_update<true>, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st,
-                                      ev[7], ev[8], 0, w);
-        else k_g2p_update<true><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        // Adaptive worlds take the variants that also min-reduce the next substep's bound.
+        if (w.any_adaptive) k_g2p_contact<true><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
+        else k_g2p_contact<false><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
+        auto g2p = w.any_adaptive ? k_g2p_update<true, true> : k_g2p_update<true, false>;
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
         if (cs != st) {
             (void)hipEventRecord(lc.join, cs);
             (void)hipStreamWaitEvent(st, lc.join, 0);
@@ -1930,9 +2036,9 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (ev) (void)hipEventRecord(ev[2], st);
         k_grid_update<false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
-        if (ev) hipExtLaunchKernelGGL(k_g2p_update<false>, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st,
-                                      ev[7], ev[8], 0, w);
-        else k_g2p_update<false><<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        auto g2p = w.any_adaptive ? k_g2p_update<false, true> : k_g2p_update<false, false>;
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
 }
