@@ -22,9 +22,12 @@ def pipeline():
 
 
 def pytest_terminal_summary(terminalreporter):
-    """Per-field parity of this session: the worst relative error of each field and how
-    many comparisons passed by the 1e-5 relative bound or only by the FP32-mirror bound.
-    With PARITY_REPORT set, every comparison is also written there as JSON."""
+    """Per-field parity of this session: how many comparisons passed by each clause of the
+    one-substep bar and, per clause, the worst margin: the largest |gpu - fp64| / max|fp64|
+    among those within 1e-5, the largest |gpu - fp64| / |fp32 mirror - fp64| among those
+    passing by the mirror bound (2 allowed), the largest |gpu - fp64| / resolution among
+    those passing by C's FP32 resolution (4 allowed).  With PARITY_REPORT set, every
+    comparison is also written there as JSON."""
     try:
         import parity
     except ImportError:
@@ -33,23 +36,34 @@ def pytest_terminal_summary(terminalreporter):
     if not recs:
         return
     tr = terminalreporter
-    tr.write_sep("-", "parity: |gpu - fp64| / max|fp64| per field (one-substep bar)")
+    tr.write_sep("-", "parity per field: clause counts and worst margin per clause (one-substep bar)")
     by = {}
-    for test, field, e_gpu, e_32, scale, clause in recs:
-        r = by.setdefault(field, dict(n=0, rel=0, fp32=0, ulp=0, worst=0.0, worst32=0.0, where=""))
+    for test, field, e_gpu, e_32, scale, clause, res in recs:
+        r = by.setdefault(field, {"n": 0, "rel": [0, 0.0, ""], "fp32": [0, 0.0, ""], "ulp": [0, 0.0, ""]})
         r["n"] += 1
-        if clause in ("rel", "fp32", "ulp"):
-            r[clause] += 1
-        rel = e_gpu / scale if scale > 0 else 0.0
-        if rel >= r["worst"]:
-            r["worst"], r["worst32"], r["where"] = rel, (e_32 / scale if scale > 0 else 0.0), test
+        if clause == "rel":
+            m = e_gpu / scale if scale > 0 else 0.0
+        elif clause == "fp32":
+            m = e_gpu / e_32 if e_32 > 0 else 0.0
+        elif clause == "ulp":
+            m = e_gpu / res if res > 0 else 0.0
+        else:
+            continue
+        c = r[clause]
+        c[0] += 1
+        if m >= c[1]:
+            c[1], c[2] = m, test.split("::")[-1]
     for field, r in sorted(by.items()):
-        tr.write_line(f"  {field:>6}: {r['n']:4d} comparisons, {r['rel']:4d} within 1e-5, {r['fp32']:4d} by the "
-                      f"FP32 mirror bound, {r['ulp']:4d} by C's FP32 resolution; worst {r['worst']:.2e} "
-                      f"(mirror {r['worst32']:.2e}) in {r['where']}")
+        parts = [f"{field:>22}: {r['n']:3d}"]
+        for clause, label in (("rel", "within 1e-5, worst rel"), ("fp32", "by mirror, worst gpu/mirror"),
+                              ("ulp", "by C resolution, worst gpu/res")):
+            n, m, where = r[clause]
+            if n:
+                parts.append(f"{n} {label} {m:.2e} ({where})")
+        tr.write_line("; ".join(parts))
     path = os.environ.get("PARITY_REPORT")
     if path:
         import json
         with open(path, "w") as f:
-            json.dump([dict(test=t, field=fl, err_gpu=a, err_fp32=b, scale=c, clause=d)
-                       for t, fl, a, b, c, d in recs], f, indent=0)
+            json.dump([dict(test=t, field=fl, err_gpu=a, err_fp32=b, scale=c, clause=d, c_resolution=e)
+                       for t, fl, a, b, c, d, e in recs], f, indent=0)

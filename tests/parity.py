@@ -29,7 +29,8 @@ ULP_FACTOR = 4.0
 EPS32 = float(np.finfo(np.float32).eps)
 
 # Every comparison made in this process: (test, field, |gpu - fp64|, |fp32 mirror - fp64|,
-# scale, clause) -- printed at the end of the session (tests/conftest.py).
+# scale, clause, C's FP32 resolution or 0) -- summarized at the end of the session
+# (tests/conftest.py).
 RECORDS: list = []
 
 
@@ -97,7 +98,7 @@ def assert_field_close(name: str, gpu, o32, o64, state=None):
     else:
         clause = "FAIL"
     test = os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0]
-    RECORDS.append((test, name, e_gpu, e_32, scale, clause))
+    RECORDS.append((test, name, e_gpu, e_32, scale, clause, ulp_bound / ULP_FACTOR if ulp_bound else 0.0))
     bound = max(rel_bound, FP32_FACTOR * e_32, ulp_bound, 1e-30)
     assert e_gpu <= bound, (f"{name}: |gpu - fp64| = {e_gpu:.3e} > {bound:.3e} (fp32 mirror {e_32:.3e}, "
                             f"scale {scale:.3e}, relative {e_gpu / max(scale, 1e-30):.2e}, "
