@@ -288,6 +288,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--graph", action="store_true", help="replay each timed substep as one captured HIP graph")
     ap.add_argument("--workload", default="sand", choices=["sand", "cube", "fluid"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -327,6 +328,10 @@ def main(argv=None):
     # Whatever --warmup is, the clock starts only once the tuner has kept a shape.
     tuner = finish_tuning(st, lambda: pipe.step(st)) if not args.no_autotune else {"finished": False,
                                                                                    "substeps": 0, "off": True}
+    if args.graph:
+        st.set_graph(True)          # substeps with stage events still run uncaptured
+        for _ in range(2):
+            pipe.step(st)
     st.synchronize()
 
     # Stage events on one substep in ten: an event record between kernels costs GPU time of
@@ -385,6 +390,7 @@ def main(argv=None):
                    "seed": SEEDS[args.workload], "generator": "numpy PCG64 (nexus_amd/scenes.py)"},
         "launch_config": dict(zip(("p2g_groups", "p2g_block_cost", "grid_groups"), launch_cfg)),
         "tuner": tuner,
+        "graph_replay": bool(args.graph),
         "stages_ms": {"sort": stages.sort_ms, "p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
                       "g2p_update": stages.g2p_update_ms},
         "roofline": figures[dominant],
