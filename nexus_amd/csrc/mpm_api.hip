@@ -598,6 +598,14 @@ class World {
         if (!sc->cols_host && sc->dev_col_start != NONE && !sc->cols.empty())
             HIP_TRY(hipMemcpy(sc->cols.data(), cols_[cols_cur_].p + sc->dev_col_start, sc->cols.size() * sizeof(ColliderDev),
                               hipMemcpyDeviceToHost));
+        return NXM_OK;
+    }
+
+    // Synchronizes and copies the scene's clock to its host copy.
+    nxm_status pull_clock(nxm_scene *sc)
+    {
+        nxm_status st = sync();
+        if (st != NXM_OK) return st;
         if (!sc->clock_host && sc->world == this && sc->slot < clock_.n)
             HIP_TRY(hipMemcpy(&sc->clock, clock_.p + sc->slot, sizeof(SceneClock), hipMemcpyDeviceToHost));
         return NXM_OK;
@@ -607,6 +615,7 @@ class World {
     nxm_status pull(nxm_scene *sc)
     {
         nxm_status st = pull_colliders(sc);
+        if (st == NXM_OK) st = pull_clock(sc);
         if (st != NXM_OK) return st;
         HostParticles &hp = sc->hp;
         Arena a = arena(cur_);
@@ -1181,7 +1190,6 @@ class World {
         const uint32_t scan_stride = (hsum + SCAN_TILE - 1u) / SCAN_TILE + 2u;
         const uint32_t nreg = (uint32_t)region_start.size();
         const bool grow = std::max<size_t>(slots_.size(), 1) > scenes_.n || std::max<size_t>(slots_.size(), 1) > ctr_.n ||
-                          std::max<size_t>(slots_.size(), 1) > clock_.n ||
                           std::max<size_t>(cls.size(), 1) > classes_.n || std::max<size_t>(ncols, 1) > cols_[0].n || std::max<size_t>(ncols, 1) > cols_[1].n ||
                           hsum > bscene_.n || 2 * (size_t)scan_stride > scan_state_.n ||
                           std::max<uint32_t>(hsum, 1) > hkeys_.n || std::max<uint32_t>(nreg, 1) > region_start_.n ||
@@ -1199,7 +1207,6 @@ class World {
                 st = pull_colliders(sc);
                 if (st != NXM_OK) return st;
                 sc->cols_host = true;
-                sc->clock_host = true;   // the clock table may be reallocated too
             }
         }
         h_total_ = hsum;
@@ -1236,9 +1243,17 @@ class World {
             sc->cols_host = false;
         }
         cols_cur_ = nc;
-        // Clocks stay at their slot while the scene is resident; scenes new to the world
-        // (or after a reallocation) bring theirs from the host.
-        HIP_TRY(clock_.reserve(std::max<size_t>(slots_.size(), 1)));
+        // Clocks stay at their slot while the scene is resident (a larger table takes the
+        // old one's contents on the stream); scenes new to the world bring theirs.
+        if (slots_.size() > clock_.n) {
+            DevBuf<SceneClock> grown;
+            HIP_TRY(grown.reserve(std::max<size_t>(2 * slots_.size(), 64)));
+            if (clock_.n)
+                HIP_TRY(hipMemcpyAsync(grown.p, clock_.p, clock_.n * sizeof(SceneClock), hipMemcpyDeviceToDevice, stream_));
+            HIP_TRY(hipStreamSynchronize(stream_));   // the old table may still be in use
+            std::swap(clock_.p, grown.p);
+            std::swap(clock_.n, grown.n);
+        }
         for (uint32_t i = 0; i < slots_.size(); ++i) {
             if (slots_[i].state != SLOT_ACTIVE) continue;
             nxm_scene *sc = slots_[i].scene;
@@ -2134,7 +2149,7 @@ nxm_status nxm_scene_read_clock(nxm_scene *sc, double *time, float *last_dt, uin
 {
     if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
     if (sc->world) {
-        nxm_status st = sc->world->pull_colliders(sc);   // synchronizes; takes the clock too
+        nxm_status st = sc->world->pull_clock(sc);   // synchronizes
         if (st != NXM_OK) return st;
     }
     if (time) *time = sc->clock.t;
