@@ -145,6 +145,32 @@ typedef struct {
     float local_com[3];
 } nxm_cuboid;
 
+/* Analytic collider shapes: the reference's Shape tags (src_rbd_shaders/shapes/shape.rs:
+ * 29-33), converted from rapier's TypedShape by src_rbd/shapes/shape.rs:35-41, 252-254. */
+#define NXM_SHAPE_BALL 0u
+#define NXM_SHAPE_CUBOID 1u
+#define NXM_SHAPE_CAPSULE 2u
+#define NXM_SHAPE_CONE 3u      /* apex at +half_height on the body's y axis, base at -half_height */
+#define NXM_SHAPE_CYLINDER 4u  /* along the body's y axis */
+
+/* An analytic collider on its body.  The body record is nxm_cuboid's (pose, velocities,
+ * boundary condition, mass properties); its half_extents are the shape's for
+ * NXM_SHAPE_CUBOID and ignored otherwise.  dims, in the body frame:
+ *   NXM_SHAPE_BALL      radius
+ *   NXM_SHAPE_CUBOID    unused
+ *   NXM_SHAPE_CAPSULE   segment end a (3), end b (3), radius
+ *   NXM_SHAPE_CONE      half height, base radius
+ *   NXM_SHAPE_CYLINDER  half height, radius
+ * Each grid node projects onto the shape's boundary as the reference's
+ * Shape::project_point_on_boundary (shape.rs:378-399): parry3d 0.30's Ball and Cuboid,
+ * Capsule (capsule.rs:62-91, segment.rs:43-60), Cone (cone.rs:98-155), Cylinder
+ * (cylinder.rs:90-155). */
+typedef struct {
+    nxm_cuboid body;
+    uint32_t shape;   /* NXM_SHAPE_* */
+    float dims[7];
+} nxm_collider;
+
 /* A triangle-mesh collider on its body (a rapier TriMesh or HeightField collider coupled
  * to MPM; GpuRigidParticles::from_rapier, src_mpm/solver/particle.rs:295-345): vertices
  * and triangles in the body frame, and the body as in nxm_cuboid (pose, velocities,
@@ -217,6 +243,13 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *scene, const nxm_particles_soa
  * colliders at most 16, the reference's GpuMaterials limit).  Sets the poses; from then on the device
  * integrates them every substep (nxm_scene_read_colliders). */
 nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders, size_t n);
+/* MpmState::set_coupling (src_mpm/pipeline.rs:220) for analytic colliders of any
+ * NXM_SHAPE_* kind, in the caller's order (ids 0..n-1: the collider pass keeps the first
+ * of equally near colliders, grid_update_collide.rs:89-93).  Replaces the colliders set by
+ * nxm_scene_set_colliders; mesh colliders keep their ids after these.  Rejects unknown
+ * kinds and negative or non-finite dimensions (NXM_ERR_INVALID_ARGUMENT).
+ * nxm_scene_read_colliders lists them with half extents zero except for cuboids. */
+nxm_status nxm_scene_set_shape_colliders(nxm_scene *scene, const nxm_collider *colliders, size_t n);
 /* The mesh colliders of a CPIC scene (MpmState::with_select_coupling with TriMesh or
  * HeightField colliders, src_mpm/pipeline.rs:297-316).  Their ids follow the cuboids'
  * (cuboids + meshes <= 16); setting either kind keeps the other with its current poses.

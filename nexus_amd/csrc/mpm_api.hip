@@ -1916,14 +1916,59 @@ nxm_status nxm_scene_add_particles_soa(nxm_scene *sc, const nxm_particles_soa *p
     return append_host(sc, p, nullptr, n);
 }
 
-static ColliderDev collider_record(const nxm_cuboid &c, uint32_t shape)
+// The device record of a collider: the body, the shape and, for the analytic shapes,
+// the radius about the body origin of a sphere holding the shape (the block masks'
+// bound, widened by a relative 1e-5 against rounding).
+static ColliderDev collider_record(const nxm_cuboid &c, uint32_t shape, const float *dims = nullptr)
 {
     static_assert(offsetof(ColliderDev, shape) == sizeof(nxm_cuboid), "nxm_cuboid is a prefix of ColliderDev");
+    static_assert(offsetof(ColliderDev, shape) == offsetof(nxm_collider, shape) &&
+                  offsetof(ColliderDev, dims) == offsetof(nxm_collider, dims), "nxm_collider is a prefix of ColliderDev");
     ColliderDev d{};
     std::memcpy(&d, &c, sizeof(nxm_cuboid));
     d.shape = shape;
+    if (dims) std::memcpy(d.dims, dims, sizeof(d.dims));
     if (shape != SHAPE_CUBOID) d.he[0] = d.he[1] = d.he[2] = 0.0f;
+    auto norm = [](double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); };
+    double reach = 0.0;
+    if (shape == SHAPE_CUBOID) reach = norm(c.half_extents[0], c.half_extents[1], c.half_extents[2]);
+    else if (shape == SHAPE_BALL) reach = d.dims[0];
+    else if (shape == SHAPE_CAPSULE)
+        reach = std::max(norm(d.dims[0], d.dims[1], d.dims[2]), norm(d.dims[3], d.dims[4], d.dims[5])) + d.dims[6];
+    else if (shape == SHAPE_CONE || shape == SHAPE_CYLINDER) reach = norm(d.dims[0], d.dims[1], 0.0);
+    d.reach = (float)(reach * (1.0 + 1e-5));
     return d;
+}
+
+// Checks an analytic collider: a known kind with finite, non-negative dimensions (a
+// capsule's segment ends may be anywhere finite).
+static bool valid_shape(const nxm_collider &c, std::string &why)
+{
+    auto finite_nonneg = [](float x) { return std::isfinite(x) && x >= 0.0f; };
+    switch (c.shape) {
+    case NXM_SHAPE_BALL:
+        if (!finite_nonneg(c.dims[0])) { why = "ball radius must be finite and non-negative"; return false; }
+        return true;
+    case NXM_SHAPE_CUBOID:
+        for (int i = 0; i < 3; ++i)
+            if (!finite_nonneg(c.body.half_extents[i])) { why = "cuboid half extents must be finite and non-negative"; return false; }
+        return true;
+    case NXM_SHAPE_CAPSULE:
+        for (int i = 0; i < 6; ++i)
+            if (!std::isfinite(c.dims[i])) { why = "capsule segment ends must be finite"; return false; }
+        if (!finite_nonneg(c.dims[6])) { why = "capsule radius must be finite and non-negative"; return false; }
+        return true;
+    case NXM_SHAPE_CONE:
+    case NXM_SHAPE_CYLINDER:
+        if (!finite_nonneg(c.dims[0]) || !finite_nonneg(c.dims[1])) {
+            why = "half height and radius must be finite and non-negative";
+            return false;
+        }
+        return true;
+    default:
+        why = "unknown shape kind " + std::to_string(c.shape);
+        return false;
+    }
 }
 
 // Replaces the cuboids (ids 0..n-1) or the meshes (the ids after the cuboids) and keeps
@@ -1951,6 +1996,19 @@ nxm_status nxm_scene_set_colliders(nxm_scene *sc, const nxm_cuboid *c, size_t n)
     std::vector<ColliderDev> cub(n);
     for (size_t i = 0; i < n; ++i) cub[i] = collider_record(c[i], SHAPE_CUBOID);
     return replace_colliders(sc, cub, nullptr);
+}
+
+nxm_status nxm_scene_set_shape_colliders(nxm_scene *sc, const nxm_collider *c, size_t n)
+{
+    if (!sc || (!c && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
+    if (n + sc->meshes.size() > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
+    std::vector<ColliderDev> cols(n);
+    for (size_t i = 0; i < n; ++i) {
+        std::string why;
+        if (!valid_shape(c[i], why)) return fail(NXM_ERR_INVALID_ARGUMENT, "collider " + std::to_string(i) + ": " + why);
+        cols[i] = collider_record(c[i].body, c[i].shape, c[i].dims);
+    }
+    return replace_colliders(sc, cols, nullptr);
 }
 
 nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *sc, const nxm_trimesh *m, size_t n)
@@ -2424,7 +2482,11 @@ nxm_status nxm_pipeline_step_noalloc(nxm_ctx *ctx, const nxm_noalloc_scene *scen
         k.bfric = m.boundary_friction;
         std::memcpy(k.w, m.model_data, sizeof(k.w));
     }
-    for (uint32_t i = 0; i < scene->num_colliders; ++i) t.cols[i] = collider_record(scene->colliders[i], SHAPE_CUBOID);
+    for (uint32_t i = 0; i < scene->num_colliders; ++i) {
+        const ColliderDev d = collider_record(scene->colliders[i], SHAPE_CUBOID);
+        std::memcpy(t.cols[i].body, &d, sizeof(t.cols[i].body));
+        t.cols[i].reach = d.reach;
+    }
     t.n_classes = scene->num_materials;
     t.n_cols = scene->num_colliders;
     t.n = n;

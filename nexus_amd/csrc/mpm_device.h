@@ -109,10 +109,16 @@ struct ClassDev {
     uint32_t pad;
 };
 
-// A collider on its body: the nxm_cuboid fields, then the shape (a triangle mesh's
-// geometry lives in the world's mesh tables; its half extents are zero).
-constexpr uint32_t SHAPE_CUBOID = 0u;
-constexpr uint32_t SHAPE_TRIMESH = 1u;
+// A collider on its body: the nxm_cuboid fields, then the shape kind and dimensions as
+// nxm_collider holds them (a triangle mesh's geometry lives in the world's mesh tables;
+// its half extents are zero).  Kinds are the reference's Shape tags
+// (src_rbd_shaders/shapes/shape.rs:29-35).
+constexpr uint32_t SHAPE_BALL = 0u;
+constexpr uint32_t SHAPE_CUBOID = 1u;
+constexpr uint32_t SHAPE_CAPSULE = 2u;
+constexpr uint32_t SHAPE_CONE = 3u;
+constexpr uint32_t SHAPE_CYLINDER = 4u;
+constexpr uint32_t SHAPE_TRIMESH = 6u;
 struct ColliderDev {
     float he[3];
     float q[4];
@@ -124,6 +130,8 @@ struct ColliderDev {
     float inv_mass[3];
     float lcom[3];      // centre of mass in the body frame
     uint32_t shape;
+    float dims[7];      // ball: r; capsule: a, b, r; cone, cylinder: half height, r
+    float reach;        // radius about the body origin of a sphere holding the shape
     uint32_t pad;
 };
 
@@ -469,20 +477,17 @@ __device__ inline f3 collider_com(const ColliderDev &c)
     return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], mk3(c.lcom[0], c.lcom[1], c.lcom[2])) + mk3(c.t[0], c.t[1], c.t[2]);
 }
 
-// Closest point on the boundary of a posed cuboid: outside, the clamp to the box;
+// Closest point on the boundary of a cuboid in its frame: outside, the clamp to the box;
 // inside, the nearest face (first axis wins ties; the min face wins a same-axis tie),
-// as parry3d's Cuboid::project_point with solid = false
-// (src_rbd_shaders/shapes/shape.rs:378-384).
-__device__ inline f3 project_cuboid(const ColliderDev &c, f3 pt, bool &inside)
+// as parry3d 0.30's Cuboid::project_local_point with solid = false.
+__device__ inline f3 project_cuboid_local(const float he[3], f3 local, bool &inside)
 {
-    f3 t = mk3(c.t[0], c.t[1], c.t[2]);
-    f3 local = quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], pt - t);
     float lp[3] = {local.x, local.y, local.z};
     float sh[3];
     bool zero = true;
     for (int i = 0; i < 3; ++i) {
-        float mins_pt = -c.he[i] - lp[i];
-        float pt_maxs = lp[i] - c.he[i];
+        float mins_pt = -he[i] - lp[i];
+        float pt_maxs = lp[i] - he[i];
         sh[i] = fmaxf(mins_pt, 0.0f) - fmaxf(pt_maxs, 0.0f);
         zero = zero && sh[i] == 0.0f;
     }
@@ -495,7 +500,7 @@ __device__ inline f3 project_cuboid(const ColliderDev &c, f3 pt, bool &inside)
         bool is_mins = false;
         int best_id = 0;
         for (int i = 0; i < 3; ++i) {
-            float a = -c.he[i] - lp[i], b = lp[i] - c.he[i];
+            float a = -he[i] - lp[i], b = lp[i] - he[i];
             if (a < b) {
                 if (b > best) { best_id = i; is_mins = false; best = b; }
             } else if (a > best) {
@@ -505,7 +510,116 @@ __device__ inline f3 project_cuboid(const ColliderDev &c, f3 pt, bool &inside)
         for (int i = 0; i < 3; ++i)
             if (i == best_id) lp[i] += is_mins ? best : -best;
     }
-    return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], mk3(lp[0], lp[1], lp[2])) + t;
+    return mk3(lp[0], lp[1], lp[2]);
+}
+
+// parry3d 0.30's Ball::project_local_point with solid = false: the point scaled onto the
+// sphere.  At the centre the scale is 0/0, a NaN projection that no collider pass
+// picks, as in the reference.
+__device__ inline f3 project_ball_local(float r, f3 p, bool &inside)
+{
+    const float d2 = dot(p, p);
+    inside = d2 <= r * r;
+    return p * (r / sqrtf(d2));
+}
+
+// Segment::project_local_point (src_rbd_shaders/shapes/segment.rs:43-60): the end a or b
+// beyond the segment's slab, else a + ab (ab.ap / ab.ab).
+__device__ inline f3 project_segment(f3 a, f3 b, f3 p)
+{
+    const f3 ab = b - a, ap = p - a;
+    const float t = dot(ab, ap), n2 = dot(ab, ab);
+    if (t <= 0.0f) return a;
+    if (t >= n2) return b;
+    return a + ab * (t / n2);
+}
+
+// Capsule::project_local_point_on_boundary (capsule.rs:62-83), d = {a, b, radius}:
+// radially out from the nearest point of the axis segment; a point on the axis leaves
+// along the first vector of orthonormal_basis3(axis direction) (capsule.rs:150-161).
+__device__ inline f3 project_capsule_local(const float d[7], f3 p, bool &inside)
+{
+    const f3 a = mk3(d[0], d[1], d[2]), b = mk3(d[3], d[4], d[5]);
+    const float r = d[6];
+    const f3 on_axis = project_segment(a, b, p);
+    const f3 dp = p - on_axis;
+    const float dist = length(dp);
+    if (dist > 0.0f) {
+        inside = dist <= r;
+        return on_axis + dp * (r / dist);
+    }
+    const f3 ax = b - a;
+    const f3 u = ax / length(ax);
+    const float sg = u.z >= 0.0f ? 1.0f : -1.0f;
+    const float k = -1.0f / (sg + u.z);
+    const float m = u.x * u.y * k;
+    inside = true;
+    return on_axis + mk3(1.0f + sg * u.x * u.x * k, sg * m, -sg * u.x) * r;
+}
+
+// Cylinder::project_local_point_on_boundary (cylinder.rs:90-148), axis y: inside, the
+// nearest of the top, the bottom and the side (the side on ties); outside, the cap plane
+// or its rim above or below, the side between.
+__device__ inline f3 project_cylinder_local(float hh, float r, f3 p, bool &inside)
+{
+    const float planar = sqrtf(p.x * p.x + p.z * p.z);
+    float ux = 1.0f, uz = 0.0f;
+    if (planar > 0.0f) { ux = p.x / planar; uz = p.z / planar; }
+    const float rx = ux * r, rz = uz * r;
+    if (p.y >= -hh && p.y <= hh && planar <= r) {
+        inside = true;
+        const float top = hh - p.y, bottom = p.y - (-hh), side = r - planar;
+        if (top < bottom && top < side) return mk3(p.x, hh, p.z);
+        if (bottom < top && bottom < side) return mk3(p.x, -hh, p.z);
+        return mk3(rx, p.y, rz);
+    }
+    inside = false;
+    if (p.y > hh) return planar <= r ? mk3(p.x, hh, p.z) : mk3(rx, hh, rz);
+    if (p.y < -hh) return planar <= r ? mk3(p.x, -hh, p.z) : mk3(rx, -hh, rz);
+    return mk3(rx, p.y, rz);
+}
+
+// Cone::project_local_point_on_boundary (cone.rs:98-147), apex (0, hh, 0), base disc at
+// y = -hh: below the disc, its plane; otherwise the segment from the apex to the base rim
+// in the point's azimuth, or, inside, the nearer of that and the base plane.
+__device__ inline f3 project_cone_local(float hh, float r, f3 p, bool &inside)
+{
+    const float planar = sqrtf(p.x * p.x + p.z * p.z);
+    float ux = 1.0f, uz = 0.0f;
+    if (planar > 0.0f) { ux = p.x / planar; uz = p.z / planar; }
+    const f3 on_base = mk3(p.x, -hh, p.z);
+    if (p.y < -hh && planar <= r) {
+        inside = false;
+        return on_base;
+    }
+    const f3 rim = mk3(ux * r, -hh, uz * r);
+    const f3 apex = mk3(0.0f, hh, 0.0f);
+    const f3 side = rim - apex;
+    const f3 proj = project_segment(apex, rim, p);
+    const f3 down = mk3(0.0f, -2.0f * hh, 0.0f);
+    if (p.y >= -hh && p.y <= hh && dot(cross(side, p - apex), cross(side, down)) >= 0.0f) {
+        inside = true;
+        const f3 to_side = proj - p, to_base = on_base - p;
+        return dot(to_side, to_side) > dot(to_base, to_base) ? on_base : proj;
+    }
+    inside = false;
+    return proj;
+}
+
+// Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-399) for a posed
+// analytic collider: the point in the body frame (rotation^-1 (pt - translation)), its
+// projection, back to the world.
+__device__ inline f3 project_shape(const ColliderDev &c, f3 pt, bool &inside)
+{
+    const f3 t = mk3(c.t[0], c.t[1], c.t[2]);
+    const f3 local = quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], pt - t);
+    f3 lp;
+    if (c.shape == SHAPE_BALL) lp = project_ball_local(c.dims[0], local, inside);
+    else if (c.shape == SHAPE_CAPSULE) lp = project_capsule_local(c.dims, local, inside);
+    else if (c.shape == SHAPE_CONE) lp = project_cone_local(c.dims[0], c.dims[1], local, inside);
+    else if (c.shape == SHAPE_CYLINDER) lp = project_cylinder_local(c.dims[0], c.dims[1], local, inside);
+    else lp = project_cuboid_local(c.he, local, inside);
+    return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], lp) + t;
 }
 
 // BoundaryCondition::project_velocity (solver/boundary_condition.rs:77-114)
@@ -539,7 +653,7 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
     for (uint32_t i = 0; i < ncol; ++i) {
         if (!(mask & (1u << i))) continue;
         bool inside;
-        f3 proj = project_cuboid(cols[i], pt, inside);
+        f3 proj = project_shape(cols[i], pt, inside);
         f3 dpt = proj - pt;
         bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;
         if (inside || within) {
@@ -588,7 +702,7 @@ __device__ inline void node_cdf(const ColliderDev *cols, uint32_t ncol, uint32_t
     for (uint32_t i = 0; i < ncol; ++i) {
         if (!(mask & (1u << i))) continue;
         bool inside;
-        const f3 proj = project_cuboid(cols[i], pt, inside);
+        const f3 proj = project_shape(cols[i], pt, inside);
         const f3 dpt = proj - pt;
         const bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;
         if (inside || within) {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "mpm_device.h"
+#include <cstddef>
 
 namespace nxm {
 
@@ -101,10 +102,18 @@ void launch_canonical(const WorldDev &w, const Arena &prev, uint32_t *pcell, uin
 // launch carries them by value and no host buffer has to outlive the call.
 constexpr int NOALLOC_MAX_CLASSES = 16;
 constexpr int NOALLOC_MAX_COLLIDERS = 16;
+// A cuboid of that path: ColliderDev's leading body fields (the nxm_cuboid words) and its
+// bounding radius; k_noalloc_tables completes the record.
+constexpr int COLLIDER_BODY_WORDS = 24;
+static_assert(offsetof(ColliderDev, shape) == COLLIDER_BODY_WORDS * sizeof(float), "body words lead ColliderDev");
+struct NoallocCuboid {
+    float body[COLLIDER_BODY_WORDS];
+    float reach;
+};
 struct NoallocTables {
     SceneDev scene;
     ClassDev classes[NOALLOC_MAX_CLASSES];
-    ColliderDev cols[NOALLOC_MAX_COLLIDERS];
+    NoallocCuboid cols[NOALLOC_MAX_COLLIDERS];
     uint32_t n_classes, n_cols, n;
 };
 static_assert(sizeof(NoallocTables) <= 4000, "kernel arguments are limited to 4 KB");

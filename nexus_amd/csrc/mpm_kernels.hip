@@ -474,15 +474,16 @@ __device__ inline void block_info_part(const WorldDev &w)
             const uint32_t i = j - 8;
             w.bup[8 * d + i] = i == 0 ? d : find_block(w, sc, v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]);
         }
-        // Collider mask: collider j can touch a node of this block only if its bounding
-        // sphere comes within 1.5 h (the projection reach) of the block's node sphere.
+        // Collider mask: analytic collider j can touch a node of this block only if its
+        // bounding sphere (radius reach about the body origin) comes within 1.5 h (the
+        // projection reach) of the block's node sphere.  Meshes act through CPIC only.
         bool hit = false;
-        if (j < sc.col_count && w.cols[sc.col_start + j].shape == SHAPE_CUBOID) {
+        if (j < sc.col_count && w.cols[sc.col_start + j].shape != SHAPE_TRIMESH) {
             const ColliderDev &c = w.cols[sc.col_start + j];
             const float h = sc.h;
             const f3 center = mk3(((float)(4 * v.x) + 1.5f) * h, ((float)(4 * v.y) + 1.5f) * h, ((float)(4 * v.z) + 1.5f) * h);
             const float rb = 1.5f * h * 1.7320509f;                  // node sphere of the block
-            const float rc = sqrtf(c.he[0] * c.he[0] + c.he[1] * c.he[1] + c.he[2] * c.he[2]);
+            const float rc = c.reach;
             const f3 dc = center - mk3(c.t[0], c.t[1], c.t[2]);
             const float reach = rb + rc + 1.5f * h * 1.7320509f + 2.0f * h;
             hit = dot(dc, dc) <= reach * reach;
@@ -2168,7 +2169,14 @@ __global__ void k_noalloc_tables(WorldDev w, NoallocTables t)
         w.g->n_contact = 0u;
     }
     if (tid < t.n_classes) const_cast<ClassDev *>(w.classes)[tid] = t.classes[tid];
-    if (tid < t.n_cols) w.cols[tid] = t.cols[tid];
+    if (tid < t.n_cols) {
+        ColliderDev c{};
+        float *words = reinterpret_cast<float *>(&c);
+        for (int k = 0; k < COLLIDER_BODY_WORDS; ++k) words[k] = t.cols[tid].body[k];
+        c.shape = SHAPE_CUBOID;
+        c.reach = t.cols[tid].reach;
+        w.cols[tid] = c;
+    }
 }
 
 // The input arena from the caller's arrays (after k_noalloc_tables: it may report a bad

@@ -9,7 +9,7 @@ objects and calls:
   state.write_substep_params(n)            MpmState::write_substep_params (:183)
   state.set_cell_width(h, capacity)        MpmState::set_cell_width (:200)
   state.add_particles(...)                 GpuParticles::append (src_mpm/solver/particle.rs:533)
-  state.set_colliders(cuboids)             MpmState::set_coupling (:220), one-way cuboids
+  state.set_colliders(records)             MpmState::set_coupling (:220), one-way analytic colliders
   pipeline.step(state)                     MpmPipeline::step (:385)
   state.read_positions()                   GpuParticles::read_positions (particle.rs:634)
 
@@ -40,6 +40,7 @@ EXPORTS = [
     "nxm_ctx_create", "nxm_ctx_destroy", "nxm_last_error", "nxm_version",
     "nxm_scene_create", "nxm_scene_destroy", "nxm_scene_set_simulation_params", "nxm_scene_write_substep_params",
     "nxm_scene_set_cell_width", "nxm_scene_add_particles", "nxm_scene_add_particles_soa", "nxm_scene_set_colliders",
+    "nxm_scene_set_shape_colliders",
     "nxm_scene_len", "nxm_scene_device_bytes", "nxm_scene_set_graph", "nxm_scene_timestep_bound",
     "nxm_scene_set_adaptive_dt", "nxm_scene_read_clock", "nxm_scene_swap_remove", "nxm_batch_set_autotune", "nxm_scene_set_autotune",
     "nxm_batch_get_launch_config", "nxm_batch_set_launch_config", "nxm_scene_get_launch_config",
@@ -140,6 +141,7 @@ def library() -> ct.CDLL:
     L.nxm_scene_add_particles.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_add_particles_soa.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_set_colliders.argtypes = [P, P, ct.c_size_t]
+    L.nxm_scene_set_shape_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_set_trimesh_colliders.argtypes = [P, P, ct.c_size_t]
     L.nxm_scene_prep_readback.argtypes = [P, ct.c_uint32, P, ct.c_uint32, P, ct.c_size_t]
     L.nxm_scene_prep_readback_rigid.argtypes = [P, P, ct.c_uint32, P, ct.c_size_t, ct.POINTER(ct.c_size_t)]
@@ -356,9 +358,15 @@ class MpmState:
     def set_cell_width(self, cell_width: float, grid_capacity: int) -> None:
         _check(self.lib.nxm_scene_set_cell_width(self.handle, cell_width, grid_capacity))
 
-    def set_colliders(self, cuboids: np.ndarray) -> None:
-        c = np.ascontiguousarray(cuboids, dtype=_scenes.CUBOID_DTYPE)
-        _check(self.lib.nxm_scene_set_colliders(self.handle, _ptr(c) if c.size else None, c.size))
+    def set_colliders(self, colliders: np.ndarray) -> None:
+        """Analytic colliders: cuboid records (nxm_scene_set_colliders) or shape records of
+        any kind (scenes.COLLIDER_DTYPE, nxm_scene_set_shape_colliders)."""
+        if np.asarray(colliders).dtype == _scenes.COLLIDER_DTYPE:
+            c = np.ascontiguousarray(colliders, dtype=_scenes.COLLIDER_DTYPE).reshape(-1)
+            _check(self.lib.nxm_scene_set_shape_colliders(self.handle, _ptr(c) if c.size else None, c.size))
+        else:
+            c = np.ascontiguousarray(colliders, dtype=_scenes.CUBOID_DTYPE).reshape(-1)
+            _check(self.lib.nxm_scene_set_colliders(self.handle, _ptr(c) if c.size else None, c.size))
         self.num_cuboids = int(c.size)
         self.num_colliders = self.num_cuboids + int(getattr(self, "num_meshes", 0))
 

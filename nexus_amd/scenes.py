@@ -180,6 +180,65 @@ def particles(positions, radius, density, model, *, velocity=None) -> dict:
 def concat(parts: list[dict]) -> dict:
     return {k: np.concatenate([p[k] for p in parts], axis=0) for k in parts[0]}
 
+# Analytic collider records (nxm_collider): the cuboid record's body fields, then the
+# shape kind (the reference's Shape tags, src_rbd_shaders/shapes/shape.rs:29-33) and its
+# dimensions in the body frame.
+SHAPE_BALL, SHAPE_CUBOID, SHAPE_CAPSULE, SHAPE_CONE, SHAPE_CYLINDER = 0, 1, 2, 3, 4
+COLLIDER_DTYPE = np.dtype(CUBOID_DTYPE.descr + [("shape", "<u4"), ("dims", "<f4", 7)])
+
+
+def as_colliders(*records) -> np.ndarray:
+    """Collider records of any kind (cuboid() or shape records) as one COLLIDER_DTYPE
+    array, in the given order (the collider ids)."""
+    out = []
+    for r in records:
+        r = np.asarray(r).reshape(-1)
+        if r.dtype == COLLIDER_DTYPE:
+            out.append(r)
+            continue
+        c = np.zeros(r.shape[0], COLLIDER_DTYPE)
+        for f in CUBOID_DTYPE.names:
+            c[f] = r[f]
+        c["shape"] = SHAPE_CUBOID
+        out.append(c)
+    return np.concatenate(out) if out else np.zeros(0, COLLIDER_DTYPE)
+
+
+def _shape(kind, dims, center, rotation, bc, friction, linvel, angvel, inv_mass, local_com):
+    c = as_colliders(cuboid(center, (0.0, 0.0, 0.0), rotation, bc, friction, linvel, angvel, inv_mass, local_com))
+    c["shape"] = kind
+    d = np.zeros(7, np.float32)
+    d[:len(dims)] = dims
+    c["dims"] = d
+    return c
+
+
+def ball(center, radius, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0, linvel=(0, 0, 0),
+         angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
+    """A ball collider (rapier Ball)."""
+    return _shape(SHAPE_BALL, (radius,), center, rotation, bc, friction, linvel, angvel, inv_mass, local_com)
+
+
+def capsule(center, a, b, radius, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0, linvel=(0, 0, 0),
+            angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
+    """A capsule collider: segment a-b (body frame) swept by radius (parry Capsule)."""
+    return _shape(SHAPE_CAPSULE, (*a, *b, radius), center, rotation, bc, friction, linvel, angvel, inv_mass,
+                  local_com)
+
+
+def cone(center, half_height, radius, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0, linvel=(0, 0, 0),
+         angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
+    """A cone collider along the body's y axis, apex up (parry Cone)."""
+    return _shape(SHAPE_CONE, (half_height, radius), center, rotation, bc, friction, linvel, angvel, inv_mass,
+                  local_com)
+
+
+def cylinder(center, half_height, radius, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0,
+             linvel=(0, 0, 0), angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
+    """A cylinder collider along the body's y axis (parry Cylinder)."""
+    return _shape(SHAPE_CYLINDER, (half_height, radius), center, rotation, bc, friction, linvel, angvel, inv_mass,
+                  local_com)
+
 
 def cuboid(center, half_extents, rotation=(0.0, 0.0, 0.0, 1.0), bc=BC_STICK, friction=0.0,
            linvel=(0, 0, 0), angvel=(0, 0, 0), inv_mass=(0, 0, 0), local_com=(0, 0, 0)):
@@ -333,6 +392,8 @@ def scaled(scene: Scene, k: float, *, cpic: bool | None = None, name: str | None
     if cols.size:
         cols["translation"] = cols["translation"] * f
         cols["half_extents"] = cols["half_extents"] * f
+        if cols.dtype == COLLIDER_DTYPE:
+            cols["dims"] = cols["dims"] * f
     out.colliders = cols
     out.cell_width = float(np.float32(scene.cell_width * k))
     out.dt = float(np.float32(scene.dt * k))

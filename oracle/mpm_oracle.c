@@ -537,37 +537,138 @@ static v3 collider_com(const orc_cuboid *c)
                 V3(c->translation[0], c->translation[1], c->translation[2]));
 }
 
-/* parry3d Cuboid::project_point(pose, pt, solid = false), reached through
- * Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-384). */
-static v3 project_cuboid(const orc_cuboid *c, v3 pt, int *inside)
+/* parry3d 0.30 Cuboid::project_local_point(pt, solid = false), reached through
+ * Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-384): outside,
+ * the clamp to the box; inside, the nearest face (first axis wins ties, the min face a
+ * same-axis tie). */
+static v3 project_cuboid_local(const orc_cuboid *c, v3 local, int *inside)
 {
-    REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
-    v3 t = V3(c->translation[0], c->translation[1], c->translation[2]);
-    v3 local = quat_rotate(-qx, -qy, -qz, qw, vsub(pt, t));
     v3 he = V3(c->half_extents[0], c->half_extents[1], c->half_extents[2]);
     v3 mins_pt = vsub(vmul(he, K(-1.0)), local);
     v3 pt_maxs = vsub(local, he);
     v3 shift = vsub(V3(RFMAX(mins_pt.x, K(0.0)), RFMAX(mins_pt.y, K(0.0)), RFMAX(mins_pt.z, K(0.0))),
                     V3(RFMAX(pt_maxs.x, K(0.0)), RFMAX(pt_maxs.y, K(0.0)), RFMAX(pt_maxs.z, K(0.0))));
-    v3 lp;
     if (!(shift.x == K(0.0) && shift.y == K(0.0) && shift.z == K(0.0))) {
         *inside = 0;
-        lp = vadd(local, shift);
-    } else {
-        *inside = 1;
-        REAL best = -MAX_FLT_REF * K(1.0e20);
-        int is_mins = 0, best_id = 0;
-        for (int i = 0; i < 3; ++i) {
-            REAL a = vget(mins_pt, i), b = vget(pt_maxs, i);
-            if (a < b) {
-                if (b > best) { best_id = i; is_mins = 0; best = b; }
-            } else if (a > best) {
-                best_id = i; is_mins = 1; best = a;
-            }
+        return vadd(local, shift);
+    }
+    *inside = 1;
+    REAL best = -MAX_FLT_REF * K(1.0e20);
+    int is_mins = 0, best_id = 0;
+    for (int i = 0; i < 3; ++i) {
+        REAL a = vget(mins_pt, i), b = vget(pt_maxs, i);
+        if (a < b) {
+            if (b > best) { best_id = i; is_mins = 0; best = b; }
+        } else if (a > best) {
+            best_id = i; is_mins = 1; best = a;
         }
-        v3 sh = V3(0, 0, 0);
-        vset(&sh, best_id, is_mins ? best : -best);
-        lp = vadd(local, sh);
+    }
+    v3 sh = V3(0, 0, 0);
+    vset(&sh, best_id, is_mins ? best : -best);
+    return vadd(local, sh);
+}
+
+/* parry3d 0.30 Ball::project_local_point(pt, solid = false): pt * (r / |pt|), inside
+ * when |pt|^2 <= r^2 (NaN at the centre). */
+static v3 project_ball_local(REAL r, v3 p, int *inside)
+{
+    REAL d2 = vdot(p, p);
+    *inside = d2 <= r * r;
+    return vmul(p, r / RSQRT(d2));
+}
+
+/* Segment::project_local_point (src_rbd_shaders/shapes/segment.rs:43-60). */
+static v3 project_segment(v3 a, v3 b, v3 p)
+{
+    v3 ab = vsub(b, a), ap = vsub(p, a);
+    REAL t = vdot(ab, ap), n2 = vdot(ab, ab);
+    if (t <= K(0.0)) return a;
+    if (t >= n2) return b;
+    return vadd(a, vmul(ab, t / n2));
+}
+
+/* Capsule::project_local_point_on_boundary (capsule.rs:62-83) with orthonormal_basis3
+ * (capsule.rs:150-161) for a point on the axis. */
+static v3 project_capsule_local(const float *d, v3 p, int *inside)
+{
+    v3 a = V3(d[0], d[1], d[2]), b = V3(d[3], d[4], d[5]);
+    REAL r = d[6];
+    v3 on_axis = project_segment(a, b, p);
+    v3 dp = vsub(p, on_axis);
+    REAL dist = vlen(dp);
+    if (dist > K(0.0)) {
+        *inside = dist <= r;
+        return vadd(on_axis, vmul(dp, r / dist));
+    }
+    v3 ax = vsub(b, a);
+    v3 u = vdiv(ax, vlen(ax));
+    REAL sg = u.z >= K(0.0) ? K(1.0) : K(-1.0);
+    REAL k = K(-1.0) / (sg + u.z);
+    REAL m = u.x * u.y * k;
+    *inside = 1;
+    return vadd(on_axis, vmul(V3(K(1.0) + sg * u.x * u.x * k, sg * m, -sg * u.x), r));
+}
+
+/* Cylinder::project_local_point_on_boundary (cylinder.rs:90-148), axis y. */
+static v3 project_cylinder_local(REAL hh, REAL r, v3 p, int *inside)
+{
+    REAL planar = RSQRT(p.x * p.x + p.z * p.z);
+    REAL ux = K(1.0), uz = K(0.0);
+    if (planar > K(0.0)) { ux = p.x / planar; uz = p.z / planar; }
+    REAL rx = ux * r, rz = uz * r;
+    if (p.y >= -hh && p.y <= hh && planar <= r) {
+        *inside = 1;
+        REAL top = hh - p.y, bottom = p.y - (-hh), side = r - planar;
+        if (top < bottom && top < side) return V3(p.x, hh, p.z);
+        if (bottom < top && bottom < side) return V3(p.x, -hh, p.z);
+        return V3(rx, p.y, rz);
+    }
+    *inside = 0;
+    if (p.y > hh) return planar <= r ? V3(p.x, hh, p.z) : V3(rx, hh, rz);
+    if (p.y < -hh) return planar <= r ? V3(p.x, -hh, p.z) : V3(rx, -hh, rz);
+    return V3(rx, p.y, rz);
+}
+
+/* Cone::project_local_point_on_boundary (cone.rs:98-147): apex (0, hh, 0), base disc at
+ * y = -hh. */
+static v3 project_cone_local(REAL hh, REAL r, v3 p, int *inside)
+{
+    REAL planar = RSQRT(p.x * p.x + p.z * p.z);
+    REAL ux = K(1.0), uz = K(0.0);
+    if (planar > K(0.0)) { ux = p.x / planar; uz = p.z / planar; }
+    v3 on_base = V3(p.x, -hh, p.z);
+    if (p.y < -hh && planar <= r) {
+        *inside = 0;
+        return on_base;
+    }
+    v3 rim = V3(ux * r, -hh, uz * r);
+    v3 apex = V3(K(0.0), hh, K(0.0));
+    v3 side = vsub(rim, apex);
+    v3 proj = project_segment(apex, rim, p);
+    v3 down = V3(K(0.0), K(-2.0) * hh, K(0.0));
+    if (p.y >= -hh && p.y <= hh && vdot(vcross(side, vsub(p, apex)), vcross(side, down)) >= K(0.0)) {
+        *inside = 1;
+        v3 to_side = vsub(proj, p), to_base = vsub(on_base, p);
+        return vdot(to_side, to_side) > vdot(to_base, to_base) ? on_base : proj;
+    }
+    *inside = 0;
+    return proj;
+}
+
+/* Shape::project_point_on_boundary (src_rbd_shaders/shapes/shape.rs:378-399): the point
+ * in the body frame, rotation^-1 (pt - translation), projected, and back. */
+static v3 project_shape(const orc_cuboid *c, v3 pt, int *inside)
+{
+    REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
+    v3 t = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 local = quat_rotate(-qx, -qy, -qz, qw, vsub(pt, t));
+    v3 lp;
+    switch (c->shape) {
+    case ORC_SHAPE_BALL: lp = project_ball_local(c->dims[0], local, inside); break;
+    case ORC_SHAPE_CAPSULE: lp = project_capsule_local(c->dims, local, inside); break;
+    case ORC_SHAPE_CONE: lp = project_cone_local(c->dims[0], c->dims[1], local, inside); break;
+    case ORC_SHAPE_CYLINDER: lp = project_cylinder_local(c->dims[0], c->dims[1], local, inside); break;
+    default: lp = project_cuboid_local(c, local, inside); break;
     }
     return vadd(quat_rotate(qx, qy, qz, qw, lp), t);
 }
@@ -606,7 +707,7 @@ static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt,
     int best_id = 0;
     for (int i = 0; i < ncol; ++i) {
         int inside;
-        v3 proj = project_cuboid(&cols[i], pt, &inside);
+        v3 proj = project_shape(&cols[i], pt, &inside);
         v3 dpt = vsub(proj, pt);
         int within = RFABS(dpt.x) <= cap && RFABS(dpt.y) <= cap && RFABS(dpt.z) <= cap;
         if (inside || within) {
@@ -661,7 +762,7 @@ static void node_cdf(const orc_cuboid *cols, int ncol, REAL h, v3 pt, REAL *dist
     const REAL cap = h * K(1.5);
     for (int i = 0; i < ncol; ++i) {
         int inside;
-        v3 proj = project_cuboid(&cols[i], pt, &inside);
+        v3 proj = project_shape(&cols[i], pt, &inside);
         v3 dpt = vsub(proj, pt);
         int within = RFABS(dpt.x) <= cap && RFABS(dpt.y) <= cap && RFABS(dpt.z) <= cap;
         if (inside || within) {
@@ -1656,7 +1757,7 @@ REAL orc_m4_solve(const REAL *m, const REAL *b, REAL *out)
 
 int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *inside)
 {
-    v3 p = project_cuboid(c, vload(pt), inside);
+    v3 p = project_shape(c, vload(pt), inside);
     vstore(proj, p);
     return 0;
 }

@@ -30,11 +30,21 @@
 extern "C" {
 #endif
 
-/* A cuboid collider on its body, with a one-way boundary condition.  Same layout as
- * nxm_cuboid in include/nexus_mpm.h.  inv_mass / local_com: the body's mass
- * properties as MPM holds them (zero unless the body is dynamic and coupled two ways):
- * the world centre of mass is pose * local_com, and gravity acts on the body along the
- * axes where inv_mass is nonzero. */
+/* Shape kinds: the reference's Shape tags (src_rbd_shaders/shapes/shape.rs:29-35). */
+#define ORC_SHAPE_BALL 0u
+#define ORC_SHAPE_CUBOID 1u
+#define ORC_SHAPE_CAPSULE 2u
+#define ORC_SHAPE_CONE 3u
+#define ORC_SHAPE_CYLINDER 4u
+#define ORC_SHAPE_TRIMESH 6u
+
+/* A collider on its body, with a one-way boundary condition.  Same layout as
+ * nxm_collider in include/nexus_mpm.h (the nxm_cuboid body, then the shape).
+ * inv_mass / local_com: the body's mass properties as MPM holds them (zero unless the
+ * body is dynamic and coupled two ways): the world centre of mass is pose * local_com,
+ * and gravity acts on the body along the axes where inv_mass is nonzero.  dims: ball
+ * radius; capsule segment ends a, b and radius; cone and cylinder half height and
+ * radius (y axis); a cuboid's half extents are half_extents. */
 typedef struct {
     float half_extents[3];
     float rotation[4];      /* unit quaternion x, y, z, w */
@@ -45,6 +55,8 @@ typedef struct {
     float bc_friction;
     float inv_mass[3];
     float local_com[3];
+    uint32_t shape;         /* ORC_SHAPE_* */
+    float dims[7];
 } orc_cuboid;
 
 typedef struct orc_ws orc_ws;
@@ -136,6 +148,8 @@ int orc_ws_grid_cpic(const orc_ws *ws, REAL *dist, uint32_t *aff, uint32_t *clos
 int orc_svd3(const REAL *F, REAL *U, REAL *s, REAL *V);
 int orc_model_update(uint32_t tag, const float *params, REAL *pstate, REAL *F,
                      const REAL *gradv, REAL *tau);
+/* Shape::project_point_on_boundary of any analytic collider (the name predates the
+ * other shapes). */
 int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *inside);
 int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim,
                      const REAL *node_pos, REAL *vel);

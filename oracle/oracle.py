@@ -21,6 +21,28 @@ BUILD = os.path.abspath(os.environ.get("ORACLE_BUILD", os.path.join(HERE, "build
 _LIBS: dict[int, ct.CDLL] = {}
 
 
+# Collider records as the C oracle takes them (orc_cuboid = include/nexus_mpm.h's
+# nxm_collider): the cuboid record's body fields, then the shape kind and 7 dimension
+# words.  Cuboid-only records (the body fields alone) get the cuboid kind.
+COLLIDER_DTYPE = np.dtype([
+    ("half_extents", "<f4", 3), ("rotation", "<f4", 4), ("translation", "<f4", 3), ("linvel", "<f4", 3),
+    ("angvel", "<f4", 3), ("bc_type", "<u4"), ("bc_friction", "<f4"), ("inv_mass", "<f4", 3),
+    ("local_com", "<f4", 3), ("shape", "<u4"), ("dims", "<f4", 7)])
+BODY_BYTES = 96
+SHAPE_CUBOID, SHAPE_TRIMESH = 1, 6
+
+
+def collider_records(cols, shape: int = SHAPE_CUBOID) -> np.ndarray:
+    a = np.ascontiguousarray(cols).reshape(-1)
+    if a.dtype.itemsize == COLLIDER_DTYPE.itemsize:
+        return a.view(COLLIDER_DTYPE).copy()
+    assert a.dtype.itemsize == BODY_BYTES, a.dtype
+    out = np.zeros(a.shape[0], COLLIDER_DTYPE)
+    out.view(np.uint8).reshape(-1, COLLIDER_DTYPE.itemsize)[:, :BODY_BYTES] = a.view(np.uint8).reshape(-1, BODY_BYTES)
+    out["shape"] = shape
+    return out
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -145,12 +167,12 @@ class OracleState:
         # array; their rigid particles are sampled at the cell width
         # (src_mpm/pipeline.rs:310-316).
         meshes = list(getattr(scene, "meshes", None) or [])
-        cols = np.array(scene.colliders)
+        cols = collider_records(scene.colliders)
         mesh_arrays = []
         if meshes:
             if not getattr(scene, "cpic", False):
                 raise ValueError("mesh colliders act through CPIC only")
-            cols = np.concatenate([cols, np.concatenate([m.body for m in meshes])])
+            cols = np.concatenate([cols, collider_records(np.concatenate([m.body for m in meshes]), SHAPE_TRIMESH)])
             for m in meshes:
                 v = np.ascontiguousarray(m.vertices, np.float32).reshape(-1, 3)
                 t = np.ascontiguousarray(m.triangles, np.uint32).reshape(-1, 3)
@@ -268,6 +290,7 @@ def node_cdf(cols: np.ndarray, cell_width: float, pt, bits: int = 64):
     d = np.zeros(1, R)
     a = np.zeros(1, np.uint32)
     c = np.zeros(1, np.uint32)
+    cols = collider_records(cols)
     rc = lib(bits).orc_node_cdf(_p(cols) if cols.size else None, int(cols.size), ct.c_float(cell_width), _p(p),
                                 _p(d), _p(a), _p(c))
     assert rc == 0
@@ -347,7 +370,7 @@ def project_cuboid(col: np.ndarray, pt, bits: int = 64):
     p = np.ascontiguousarray(pt, R)
     out = np.zeros(3, R)
     inside = ct.c_int(0)
-    lib(bits).orc_project_cuboid(_p(col), _p(p), _p(out), ct.byref(inside))
+    lib(bits).orc_project_cuboid(_p(collider_records(col)), _p(p), _p(out), ct.byref(inside))
     return out, bool(inside.value)
 
 
@@ -355,6 +378,7 @@ def grid_node_bc(cols: np.ndarray, sim: np.ndarray, node_pos, vel, bits: int = 6
     R = real(bits)
     p = np.ascontiguousarray(node_pos, R)
     v = np.ascontiguousarray(vel, R).copy()
+    cols = collider_records(cols)
     lib(bits).orc_grid_node_bc(_p(cols) if cols.size else None, int(cols.size),
                                _p(np.ascontiguousarray(sim, np.float32)), _p(p), _p(v))
     return v
