@@ -742,32 +742,57 @@ __device__ __forceinline__ void p2g_scatter(unsigned long long *tile, uint32_t *
 // Flush: after the block's scatter pass, node t of the 6^3 tile goes to the region of the
 // block it belongs to (offset e = (tx >= 4, ty >= 4, tz >= 4)), stored contiguously in that
 // block's node order in the staging array st, so the consumer reads each region as one run
-// of float4.  Ends with the tile free for the next pass.
+// of float4.  fdst, the kernel's LDS copy of FLUSH_CODE, gives each tile node's region e
+// (bits 8..10) and its offset in the block's 216-node staging slot (bits 0..7).  Ends with
+// the tile free for the next pass.
+struct alignas(8) FlushCodes {
+    uint16_t c[TILE_NODES];
+};
+constexpr FlushCodes flush_codes()
+{
+    FlushCodes f{};
+    const int off[8] = {0, 64, 96, 128, 144, 176, 192, 208};   // REGION_OFFSET
+    for (int t = 0; t < TILE_NODES; ++t) {
+        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
+        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+        const int e = 4 * ex + 2 * ey + ez;
+        const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
+        const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+        f.c[t] = (uint16_t)((e << 8) | (off[e] + idx));
+    }
+    return f;
+}
+__device__ __constant__ const FlushCodes FLUSH_CODE = flush_codes();
+
+// Copies FLUSH_CODE into the kernel's LDS table (54 lanes, 8 bytes each).
+__device__ inline void load_flush_table(uint16_t *fdst, uint32_t l)
+{
+    static_assert(TILE_NODES * sizeof(uint16_t) == 54 * sizeof(uint64_t), "216 codes in 54 words");
+    if (l < 54u) reinterpret_cast<uint64_t *>(fdst)[l] = reinterpret_cast<const uint64_t *>(FLUSH_CODE.c)[l];
+}
+
 __device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const uint32_t *bad, float4 *stage,
-                                          uint32_t d, float smom, float sm, uint32_t l, uint32_t touched)
+                                          uint32_t d, float smom, float sm, uint32_t l, uint32_t touched,
+                                          const uint16_t *fdst)
 {
     __syncthreads();
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
     const bool any_bad = bad[bad_any<false>()] != 0u;
     float4 *st = stage + (size_t)d * TILE_NODES;
-    for (int t = l; t < TILE_NODES; t += WAVE) {
-        const int tx = t % TILE, ty = (t / TILE) % TILE, tz = t / (TILE * TILE);
-        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-        const int e = 4 * ex + 2 * ey + ez;
-        if (!((touched >> e) & 1u)) continue;   // no particle's stencil reaches region e
-        const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
-        const int idx = (tx - 4 * ex) + nx * ((ty - 4 * ey) + ny * (tz - 4 * ez));
+    for (int t = (int)l; t < TILE_NODES; t += WAVE) {
+        const uint32_t code = fdst[t];
+        if (!((touched >> (code >> 8)) & 1u)) continue;   // no particle's stencil reaches region e
         float v4[4];
         #pragma unroll
         for (int c = 0; c < 4; ++c) v4[c] = Fixed::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
         if (any_bad) {
             #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const int k = c * TILE_NODES + t;
-                if (bad[k >> 5] & (1u << (k & 31))) v4[c] = __builtin_nanf("");
+                const int q = c * TILE_NODES + t;
+                if (bad[q >> 5] & (1u << (q & 31))) v4[c] = __builtin_nanf("");
             }
         }
-        st[REGION_OFFSET[e] + idx] = make_float4(v4[0], v4[1], v4[2], v4[3]);
+        st[code & 0xFFu] = make_float4(v4[0], v4[1], v4[2], v4[3]);
     }
     __syncthreads();
 }
@@ -970,6 +995,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     __shared__ uint32_t naff[CPIC ? TILE_NODES : 1];
     const uint32_t l = threadIdx.x;
     const uint32_t *FL = w.in.uplane(PFLAGS);
+    __shared__ __align__(8) uint16_t fdst[TILE_NODES];
+    load_flush_table(fdst, l);   // read after the first __syncthreads
     // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const uint32_t g = (G % NUM_XCD == 0) ? (b % NUM_XCD) * (G / NUM_XCD) + b / NUM_XCD : b;
@@ -985,7 +1012,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         float bmom = 0.0f, bm = 0.0f;
         if (np <= (uint32_t)WAVE && np > 21u) {
             // One particle per lane: load it once, bound, then scatter from registers.
-            P2GIn p;
+            P2GIn p{};
             bool on = false;
             uint32_t pa = 0u, spill = 0u;
             if (l < np) {
@@ -1010,7 +1037,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 else spill = p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
             }
             const uint32_t touched = wave_or_mask8(spill);
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst);
             if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
             if (CPIC && inc) {
                 // Second pass: the incompatible contributions of the particles with affinity.
@@ -1018,7 +1045,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 __syncthreads();
                 if (on && (pa & 0xFFFFu) != 0u)
                     p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
-                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched);
+                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst);
             }
             unlink_untouched(w, d, touched, l);
             continue;
@@ -1047,14 +1074,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             inc ? p2g_block_pass<CPIC, PART_COMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff)
                 : p2g_block_pass<CPIC, PART_ALL>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
         const uint32_t touched = wave_or_mask8(spill);
-        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched);
+        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst);
         if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
         if (CPIC && inc) {
             // Second pass: the incompatible contributions of the particles with affinity.
             p2g_clear(tile, bad, l);
             __syncthreads();
             p2g_block_pass<CPIC, PART_INCOMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst);
         }
         unlink_untouched(w, d, touched, l);
     }
