@@ -696,10 +696,208 @@ static v3 bc_project(uint32_t ty, REAL friction, v3 vel, v3 n)
     return vel;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Mesh colliders in the grid collider pass (CPIC off). */
+
+/* Triangle::project_local_point_and_get_location as the mesh query uses it
+ * (src_rbd_shaders/shapes/trimesh.rs:237-391: Ericson's Voronoi regions, a then b then c,
+ * then the edges ab, ac, bc by the signed volumes of stable_check_edges_voronoi, then the
+ * face; a degenerate face falls through to the solid case, the point itself).  Feature
+ * types: 0 vertex (id 0 a, 1 b, 2 c), 1 edge (id 0 ab, 1 bc, 2 ac), 2 face, 3 solid. */
+static v3 tri_project(v3 a, v3 b, v3 c, v3 pt, int *ftype, int *fid)
+{
+    v3 ab = vsub(b, a), ac = vsub(c, a), ap = vsub(pt, a);
+    REAL ab_ap = vdot(ab, ap), ac_ap = vdot(ac, ap);
+    if (ab_ap <= K(0.0) && ac_ap <= K(0.0)) { *ftype = 0; *fid = 0; return a; }
+    v3 bp = vsub(pt, b);
+    REAL ab_bp = vdot(ab, bp), ac_bp = vdot(ac, bp);
+    if (ab_bp >= K(0.0) && ac_bp <= ab_bp) { *ftype = 0; *fid = 1; return b; }
+    v3 cp = vsub(pt, c);
+    REAL ab_cp = vdot(ab, cp), ac_cp = vdot(ac, cp);
+    if (ac_cp >= K(0.0) && ab_cp <= ac_cp) { *ftype = 0; *fid = 2; return c; }
+    v3 bc = vsub(c, b);
+    v3 n = vcross(ab, ac);
+    REAL vc = vdot(n, vcross(ab, ap));
+    if (vc < K(0.0) && ab_ap >= K(0.0) && ab_bp <= K(0.0)) {
+        REAL v = ab_ap / vdot(ab, ab);
+        *ftype = 1; *fid = 0;
+        return vadd(a, vmul(ab, v));
+    }
+    REAL vb = -vdot(n, vcross(ac, cp));
+    if (vb < K(0.0) && ac_ap >= K(0.0) && ac_cp <= K(0.0)) {
+        REAL w = ac_ap / vdot(ac, ac);
+        *ftype = 1; *fid = 2;
+        return vadd(a, vmul(ac, w));
+    }
+    REAL va = vdot(n, vcross(bc, bp));
+    if (va < K(0.0) && ac_bp - ab_bp >= K(0.0) && ab_cp - ac_cp >= K(0.0)) {
+        REAL w = vdot(bc, bp) / vdot(bc, bc);
+        *ftype = 1; *fid = 1;
+        return vadd(b, vmul(bc, w));
+    }
+    if (va + vb + vc != K(0.0)) {
+        REAL denom = K(1.0) / (va + vb + vc);
+        REAL v = vb * denom, w = vc * denom;
+        *ftype = 2; *fid = 0;
+        return vadd(vadd(a, vmul(ab, v)), vmul(ac, w));
+    }
+    *ftype = 3; *fid = 0;
+    return pt;
+}
+
+/* A mesh with the pseudo-normals of its inside test (parry3d 0.30's
+ * TriMesh::pseudo_normals, read by trimesh.rs:116-152): per vertex the sum of its faces'
+ * unit normals weighted by each face's angle at the vertex; per triangle edge (ab, bc,
+ * ca) the sum of the unit normals of the faces sharing it (the angle at an edge is pi for
+ * every face); faces without a normal (zero area) add nothing.  Both normalized. */
+typedef struct {
+    const orc_trimesh *m;
+    double *vn;   /* nv*3 */
+    double *en;   /* nt*9 */
+} mesh_pn;
+
+typedef struct {
+    uint64_t key;   /* (min vertex << 32) | max vertex */
+    int64_t slot;   /* 3 triangle + edge */
+} edge_ref;
+
+static int edge_ref_cmp(const void *x, const void *y)
+{
+    const edge_ref *a = (const edge_ref *)x, *b = (const edge_ref *)y;
+    if (a->key != b->key) return a->key < b->key ? -1 : 1;
+    return a->slot < b->slot ? -1 : (a->slot > b->slot);
+}
+
+static void dnormalize(double *v)
+{
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l > 0.0) { v[0] /= l; v[1] /= l; v[2] /= l; }
+}
+
+static void mesh_pseudo_normals(mesh_pn *mp)
+{
+    const orc_trimesh *m = mp->m;
+    mp->vn = (double *)calloc((size_t)(3 * (m->nv > 0 ? m->nv : 1)), sizeof(double));
+    mp->en = (double *)calloc((size_t)(9 * (m->nt > 0 ? m->nt : 1)), sizeof(double));
+    double *fn = (double *)calloc((size_t)(3 * (m->nt > 0 ? m->nt : 1)), sizeof(double));
+    edge_ref *er = (edge_ref *)malloc(sizeof(edge_ref) * (size_t)(3 * (m->nt > 0 ? m->nt : 1)));
+    for (int64_t t = 0; t < m->nt; ++t) {
+        const uint32_t *id = m->triangles + 3 * t;
+        double p[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) p[k][c] = m->vertices[3 * id[k] + c];
+        double e1[3], e2[3];
+        for (int c = 0; c < 3; ++c) { e1[c] = p[1][c] - p[0][c]; e2[c] = p[2][c] - p[0][c]; }
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (l > 0.0) {
+            for (int c = 0; c < 3; ++c) fn[3 * t + c] = n[c] / l;
+            for (int k = 0; k < 3; ++k) {
+                const double *o = p[k], *q1 = p[(k + 1) % 3], *q2 = p[(k + 2) % 3];
+                double u[3], v[3];
+                for (int c = 0; c < 3; ++c) { u[c] = q1[c] - o[c]; v[c] = q2[c] - o[c]; }
+                double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+                double ang = atan2(sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]), u[0] * v[0] + u[1] * v[1] + u[2] * v[2]);
+                for (int c = 0; c < 3; ++c) mp->vn[3 * id[k] + c] += fn[3 * t + c] * ang;
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            uint32_t i0 = id[k], i1 = id[(k + 1) % 3];
+            uint32_t lo = i0 < i1 ? i0 : i1, hi = i0 < i1 ? i1 : i0;
+            er[3 * t + k].key = ((uint64_t)lo << 32) | hi;
+            er[3 * t + k].slot = 3 * t + k;
+        }
+    }
+    for (int64_t q = 0; q < m->nv; ++q) dnormalize(mp->vn + 3 * q);
+    qsort(er, (size_t)(3 * m->nt), sizeof(edge_ref), edge_ref_cmp);
+    for (int64_t a = 0; a < 3 * m->nt;) {
+        int64_t b = a;
+        double sum[3] = {0.0, 0.0, 0.0};
+        while (b < 3 * m->nt && er[b].key == er[a].key) {
+            for (int c = 0; c < 3; ++c) sum[c] += fn[3 * (er[b].slot / 3) + c];
+            ++b;
+        }
+        dnormalize(sum);
+        for (int64_t q = a; q < b; ++q)
+            for (int c = 0; c < 3; ++c) mp->en[3 * er[q].slot + c] = sum[c];
+        a = b;
+    }
+    free(fn);
+    free(er);
+}
+
+static void mesh_pn_free(mesh_pn *mp)
+{
+    free(mp->vn);
+    free(mp->en);
+    mp->vn = mp->en = NULL;
+}
+
+/* TriMesh::project_local_point (trimesh.rs:155-207) without its BVH: the nearest triangle
+ * within max_dist (a later triangle wins a tie, dist <= best), inside when the feature's
+ * pseudo-normal points away from the point (pn . (pt - proj) <= 0; a face's is ab x ac).
+ * Returns 0 when no triangle is that near. */
+static int project_trimesh_local(const mesh_pn *mp, v3 lp, REAL max_dist, v3 *proj, int *inside)
+{
+    const orc_trimesh *m = mp->m;
+    REAL best = max_dist;
+    int64_t best_t = -1;
+    int bt = 0, bid = 0;
+    v3 bp = lp, ba = lp, bb = lp, bc = lp;
+    for (int64_t t = 0; t < m->nt; ++t) {
+        const uint32_t *id = m->triangles + 3 * t;
+        v3 a = V3(m->vertices[3 * id[0]], m->vertices[3 * id[0] + 1], m->vertices[3 * id[0] + 2]);
+        v3 b = V3(m->vertices[3 * id[1]], m->vertices[3 * id[1] + 1], m->vertices[3 * id[1] + 2]);
+        v3 c = V3(m->vertices[3 * id[2]], m->vertices[3 * id[2] + 1], m->vertices[3 * id[2] + 2]);
+        int ft, fi;
+        v3 p = tri_project(a, b, c, lp, &ft, &fi);
+        REAL dist = vlen(vsub(p, lp));
+        if (dist <= best) {
+            best = dist; best_t = t; bp = p; bt = ft; bid = fi;
+            ba = a; bb = b; bc = c;
+        }
+    }
+    if (best_t < 0) return 0;
+    v3 pn;
+    if (bt == 0) {
+        const double *q = mp->vn + 3 * m->triangles[3 * best_t + bid];
+        pn = V3((REAL)q[0], (REAL)q[1], (REAL)q[2]);
+    } else if (bt == 1) {
+        /* edge ids: 0 ab, 1 bc, 2 ac = the pseudo-normal slots ab, bc, ca */
+        const double *q = mp->en + 9 * best_t + 3 * bid;
+        pn = V3((REAL)q[0], (REAL)q[1], (REAL)q[2]);
+    } else {
+        pn = vcross(vsub(bb, ba), vsub(bc, ba));
+    }
+    *inside = vdot(pn, vsub(lp, bp)) <= K(0.0);
+    *proj = bp;
+    return 1;
+}
+
+/* The mesh collider's projection in the world (grid_update_collide.rs:53-67): the point
+ * in the body frame by the inverse pose (R^-1 p - R^-1 t), the projection back by the
+ * pose. */
+static int project_trimesh(const mesh_pn *mp, const orc_cuboid *c, REAL max_dist, v3 pt, v3 *proj, int *inside)
+{
+    REAL qx = c->rotation[0], qy = c->rotation[1], qz = c->rotation[2], qw = c->rotation[3];
+    v3 t = V3(c->translation[0], c->translation[1], c->translation[2]);
+    v3 lp = vsub(quat_rotate(-qx, -qy, -qz, qw, pt), quat_rotate(-qx, -qy, -qz, qw, t));
+    v3 lproj;
+    if (!project_trimesh_local(mp, lp, max_dist, &lproj, inside)) return 0;
+    *proj = vadd(quat_rotate(qx, qy, qz, qw, lproj), t);
+    return 1;
+}
+
+/* The meshes of the colliders (the last nmesh of them) for the node pass. */
+typedef struct {
+    mesh_pn *mp;
+    int nmesh;
+} mesh_set;
+
 /* gpu_grid_update_collide body (solver/grid_update_collide.rs:30-99, 105-183).
  * Velocity::velocity_at_point (src_rbd_shaders/dynamics/body.rs:239-250) with the
  * collider's world centre of mass at its pose translation. */
-static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt, v3 vel)
+static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt, v3 vel, const mesh_set *ms)
 {
     REAL cap = h * K(1.5);
     REAL best = MAX_FLT_REF;
@@ -707,7 +905,14 @@ static v3 collide_node(const orc_cuboid *cols, int ncol, REAL h, REAL dt, v3 pt,
     int best_id = 0;
     for (int i = 0; i < ncol; ++i) {
         int inside;
-        v3 proj = project_shape(&cols[i], pt, &inside);
+        v3 proj;
+        if (cols[i].shape == ORC_SHAPE_TRIMESH) {
+            /* a mesh only reports triangles within 1.5 h (its max_dist) */
+            const int k = i - (ncol - (ms ? ms->nmesh : 0));
+            if (!ms || k < 0 || !project_trimesh(&ms->mp[k], &cols[i], cap, pt, &proj, &inside)) continue;
+        } else {
+            proj = project_shape(&cols[i], pt, &inside);
+        }
         v3 dpt = vsub(proj, pt);
         int within = RFABS(dpt.x) <= cap && RFABS(dpt.y) <= cap && RFABS(dpt.z) <= cap;
         if (inside || within) {
@@ -1196,6 +1401,7 @@ struct orc_ws {
     REAL *ndist, *mv_i, *m_i, *vel_i;   /* CPIC: node CDF distance, incompatible fields */
     uint32_t *naff, *nclosest;
     int64_t cap_b, cap_s;
+    mesh_set meshes;   /* mesh colliders of the node pass (orc_ws_set_meshes) */
 };
 
 orc_ws *orc_ws_create(void)
@@ -1210,7 +1416,37 @@ void orc_ws_destroy(orc_ws *ws)
     free(ws->vid); free(ws->np); free(ws->npe); free(ws->first); free(ws->buckets);
     free(ws->nbh); free(ws->sorted); free(ws->mv); free(ws->m); free(ws->vel);
     free(ws->ndist); free(ws->mv_i); free(ws->m_i); free(ws->vel_i); free(ws->naff); free(ws->nclosest);
+    orc_ws_set_meshes(ws, NULL, 0);
     free(ws);
+}
+
+int orc_ws_set_meshes(orc_ws *ws, const orc_trimesh *meshes, int nmesh)
+{
+    for (int k = 0; k < ws->meshes.nmesh; ++k) mesh_pn_free(&ws->meshes.mp[k]);
+    free(ws->meshes.mp);
+    ws->meshes.mp = NULL;
+    ws->meshes.nmesh = 0;
+    if (nmesh <= 0) return 0;
+    ws->meshes.mp = (mesh_pn *)calloc((size_t)nmesh, sizeof(mesh_pn));
+    for (int k = 0; k < nmesh; ++k) {
+        ws->meshes.mp[k].m = &meshes[k];
+        mesh_pseudo_normals(&ws->meshes.mp[k]);
+    }
+    ws->meshes.nmesh = nmesh;
+    return 0;
+}
+
+int orc_project_trimesh(const orc_trimesh *m, const orc_cuboid *body, REAL max_dist, const REAL *pt, REAL *proj,
+                        int *inside)
+{
+    mesh_pn mp;
+    mp.m = m;
+    mesh_pseudo_normals(&mp);
+    v3 p;
+    int ok = project_trimesh(&mp, body, max_dist, vload(pt), &p, inside);
+    if (ok) vstore(proj, p);
+    mesh_pn_free(&mp);
+    return ok;
 }
 
 static void ws_reserve(orc_ws *ws, int64_t nb, int64_t ns)
@@ -1549,7 +1785,7 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
             vstore(gvel_i + 3 * i, vi);
         } else {
             v3 pt = vmul(V3((REAL)(vid[0] * 4 + tx), (REAL)(vid[1] * 4 + ty), (REAL)(vid[2] * 4 + tz)), h);
-            vel = collide_node(cols, ncol, h, dt, pt, vel);
+            vel = collide_node(cols, ncol, h, dt, pt, vel, ws->meshes.nmesh ? &ws->meshes : NULL);
         }
         vstore(gvel + 3 * i, vel);
     }
@@ -1764,7 +2000,7 @@ int orc_project_cuboid(const orc_cuboid *c, const REAL *pt, REAL *proj, int *ins
 
 int orc_grid_node_bc(const orc_cuboid *cols, int ncol, const float *sim, const REAL *node_pos, REAL *vel)
 {
-    v3 r = collide_node(cols, ncol, (REAL)sim[4], (REAL)sim[3], vload(node_pos), vload(vel));
+    v3 r = collide_node(cols, ncol, (REAL)sim[4], (REAL)sim[3], vload(node_pos), vload(vel), NULL);
     vstore(vel, r);
     return 0;
 }

@@ -126,6 +126,17 @@ int orc_step_ex(orc_ws *ws, int64_t n, REAL *x, REAL *v, REAL *C, REAL *F, REAL 
                 const uint32_t *fixed, const uint32_t *tag, const float *params,
                 const float *sim, orc_cuboid *cols, int ncol, int keep_debug, const orc_cpic *cp);
 
+/* Mesh colliders in the grid collider pass of substeps without CPIC
+ * (grid_update_collide.rs:53-67, TriMesh::project_local_point, trimesh.rs:155-207): the
+ * meshes of the last nmesh colliders, kept by pointer (with their pseudo-normals
+ * computed here) until the next call; nmesh = 0 forgets them. */
+int orc_ws_set_meshes(orc_ws *ws, const orc_trimesh *meshes, int nmesh);
+/* One point's projection onto a posed mesh collider within max_dist: 1 with the world
+ * point and the pseudo-normal side, 0 when no triangle is that near.  Only the
+ * vertices and triangles of m are read. */
+int orc_project_trimesh(const orc_trimesh *m, const orc_cuboid *body, REAL max_dist, const REAL *pt, REAL *proj,
+                        int *inside);
+
 /* Inspection of the last substep run with keep_debug != 0. */
 int64_t orc_ws_nblocks(const orc_ws *ws);
 int64_t orc_ws_nprimary_blocks(const orc_ws *ws);

@@ -69,6 +69,10 @@ def lib(bits: int = 32) -> ct.CDLL:
     L.orc_particle_cdf.argtypes = [P, ct.c_float, P, P, ct.c_uint32, P, P, P]
     L.orc_m4_solve.restype = ct.c_double if bits == 64 else ct.c_float
     L.orc_m4_solve.argtypes = [P, P, P]
+    L.orc_ws_set_meshes.argtypes = [P, P, ct.c_int]
+    L.orc_ws_set_meshes.restype = ct.c_int
+    L.orc_project_trimesh.argtypes = [P, P, ct.c_double if bits == 64 else ct.c_float, P, P, P]
+    L.orc_project_trimesh.restype = ct.c_int
     L.orc_sample_trimesh.restype = ct.c_int64
     L.orc_sample_trimesh.argtypes = [P, ct.c_int64, P, ct.c_int64, ct.c_float, P, P, ct.c_int64]
     for f in ("orc_ws_nblocks", "orc_ws_nprimary_blocks", "orc_ws_nsorted"):
@@ -166,17 +170,21 @@ class OracleState:
         # Mesh colliders (CPIC only): their bodies follow the cuboids in the collider
         # array; their rigid particles are sampled at the cell width
         # (src_mpm/pipeline.rs:310-316).
+        # Without CPIC the node pass projects the grid nodes onto the meshes instead
+        # (grid_update_collide.rs:53-67), and no rigid particles are sampled.
         meshes = list(getattr(scene, "meshes", None) or [])
+        cpic = bool(getattr(scene, "cpic", False))
         cols = collider_records(scene.colliders)
         mesh_arrays = []
         if meshes:
-            if not getattr(scene, "cpic", False):
-                raise ValueError("mesh colliders act through CPIC only")
             cols = np.concatenate([cols, collider_records(np.concatenate([m.body for m in meshes]), SHAPE_TRIMESH)])
             for m in meshes:
                 v = np.ascontiguousarray(m.vertices, np.float32).reshape(-1, 3)
                 t = np.ascontiguousarray(m.triangles, np.uint32).reshape(-1, 3)
-                pts, tri = sample_trimesh(v, t, float(np.float32(scene.cell_width)))
+                if cpic:
+                    pts, tri = sample_trimesh(v, t, float(np.float32(scene.cell_width)))
+                else:
+                    pts, tri = np.zeros((0, 3), np.float32), np.zeros(0, np.uint32)
                 mesh_arrays.append(dict(vertices=v, triangles=t, samples=np.ascontiguousarray(pts),
                                         sample_triangle=np.ascontiguousarray(tri)))
         return OracleState(
@@ -220,10 +228,12 @@ class Oracle:
     def step(self, count: int = 1, keep_debug: bool = True) -> None:
         s = self.s
         cp = None
+        self._meshes = (_TriMesh * max(len(s.meshes), 1))(*[
+            _TriMesh(_p(m["vertices"]), _p(m["triangles"]), _p(m["samples"]), _p(m["sample_triangle"]),
+                     m["vertices"].shape[0], m["triangles"].shape[0], m["samples"].shape[0]) for m in s.meshes])
+        if not s.cpic:
+            self.L.orc_ws_set_meshes(self.ws, ct.cast(self._meshes, ct.c_void_p), len(s.meshes))
         if s.cpic:
-            self._meshes = (_TriMesh * max(len(s.meshes), 1))(*[
-                _TriMesh(_p(m["vertices"]), _p(m["triangles"]), _p(m["samples"]), _p(m["sample_triangle"]),
-                         m["vertices"].shape[0], m["triangles"].shape[0], m["samples"].shape[0]) for m in s.meshes])
             cp = _Cpic(_p(s.affinity), _p(s.normal), _p(s.signed_distance), _p(s.rigid_vel),
                        _p(s.boundary_friction), ct.cast(self._meshes, ct.c_void_p), len(s.meshes))
         for _ in range(count):
@@ -372,6 +382,21 @@ def project_cuboid(col: np.ndarray, pt, bits: int = 64):
     inside = ct.c_int(0)
     lib(bits).orc_project_cuboid(_p(collider_records(col)), _p(p), _p(out), ct.byref(inside))
     return out, bool(inside.value)
+
+
+def project_trimesh(vertices, triangles, body, pt, max_dist: float, bits: int = 64):
+    """(world projection or None, inside) of a point onto a posed mesh collider within
+    max_dist (body: the mesh's collider record)."""
+    R = real(bits)
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    m = _TriMesh(_p(v), _p(t), None, None, v.shape[0], t.shape[0], 0)
+    b = collider_records(body, SHAPE_TRIMESH)
+    p = np.ascontiguousarray(pt, R)
+    out = np.zeros(3, R)
+    inside = ct.c_int(0)
+    ok = lib(bits).orc_project_trimesh(ct.byref(m), _p(b), max_dist, _p(p), _p(out), ct.byref(inside))
+    return (out if ok else None), bool(inside.value)
 
 
 def grid_node_bc(cols: np.ndarray, sim: np.ndarray, node_pos, vel, bits: int = 64):
