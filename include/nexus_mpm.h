@@ -250,17 +250,19 @@ nxm_status nxm_scene_set_colliders(nxm_scene *scene, const nxm_cuboid *colliders
  * kinds and negative or non-finite dimensions (NXM_ERR_INVALID_ARGUMENT).
  * nxm_scene_read_colliders lists them with half extents zero except for cuboids. */
 nxm_status nxm_scene_set_shape_colliders(nxm_scene *scene, const nxm_collider *colliders, size_t n);
-/* The mesh colliders of a CPIC scene (MpmState::with_select_coupling with TriMesh or
- * HeightField colliders, src_mpm/pipeline.rs:297-316).  Their ids follow the cuboids'
- * (cuboids + meshes <= 16); setting either kind keeps the other with its current poses.
- * The meshes act on the particles through the contact distance field only: each
- * substep their rigid particles (nxm_sample_trimesh at the cell width) reach the grid
- * nodes around them (gpu_p2g_cdf, src_mpm_shaders/solver/p2g_cdf.rs:60-259).  The
- * reference's grid collider pass, with CPIC off, also projects onto meshes; here a scene
- * with meshes must use CPIC: NXM_ERR_INVALID_ARGUMENT when CPIC is off, and
- * nxm_scene_set_cpic(scene, 0) is refused while meshes are set.  The bodies are
- * integrated like the cuboids (nxm_scene_read_colliders lists them after the cuboids,
- * half extents zero). */
+/* The mesh colliders of a scene (MpmState::with_select_coupling with TriMesh or
+ * HeightField colliders, src_mpm/pipeline.rs:297-316).  Their ids follow the analytic
+ * colliders' (all together <= 16); setting either kind keeps the other with its current
+ * poses.  With CPIC the meshes act through the contact distance field: each substep
+ * their rigid particles (nxm_sample_trimesh at the cell width) reach the grid nodes around
+ * them (gpu_p2g_cdf, src_mpm_shaders/solver/p2g_cdf.rs:60-259).  Without CPIC the grid
+ * collider pass projects each node onto the nearest triangle within 1.5 cell widths
+ * (grid_update_collide.rs:53-67, TriMesh::project_local_point, src_rbd_shaders/shapes/
+ * trimesh.rs:155-207), inside by the pseudo-normal of the nearest feature; nodes deeper
+ * than that inside a closed mesh see no triangle and no collider, as in the reference.
+ * CPIC may be switched either way while meshes are set.  Rejects out-of-range vertex
+ * indices and non-finite vertices.  The bodies are integrated like the other colliders
+ * (nxm_scene_read_colliders lists them after them, half extents zero). */
 nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *scene, const nxm_trimesh *meshes, size_t n);
 
 /* ReadbackData (src_mpm_shaders/solver/prep_readback.rs:84-92): a particle's render

@@ -296,12 +296,145 @@ static MeshSamples sample_mesh_surface(const float *v, const uint32_t *tri, uint
 
 // A triangle-mesh collider's geometry (body frame) and its rigid particles at the cell
 // width they were sampled for.
+// A mesh collider: geometry in the body frame, the pseudo-normals of its inside test, the
+// rigid particles of CPIC scenes and the cell grid of the node pass (both for the cell
+// width they were made at).
 struct HostMesh {
     std::vector<float> v;
     std::vector<uint32_t> tri;
+    std::vector<float> vn, en;    // per vertex 3, per triangle 9 (edges ab, bc, ca)
     MeshSamples samples;
     float sampled_h = 0.0f;
+    float grid_h = 0.0f;
+    float origin[3] = {0.0f, 0.0f, 0.0f};
+    float cell = 1.0f;
+    uint32_t dims[3] = {1, 1, 1};
+    std::vector<uint32_t> start, ctri;   // per cell its list in ctri (start[c]..start[c + 1])
 };
+
+// Pseudo-normals of the mesh's inside test as parry3d 0.30 computes them
+// (TriMesh::pseudo_normals, read by src_rbd_shaders/shapes/trimesh.rs:116-152): per vertex
+// the sum of its faces' unit normals weighted by each face's angle at the vertex; per
+// triangle edge (ab, bc, ca) the sum of the unit normals of the faces sharing it; faces of
+// zero area add nothing; both normalized.
+static void mesh_pseudo_normals(HostMesh &m)
+{
+    const size_t nv = m.v.size() / 3, nt = m.tri.size() / 3;
+    std::vector<double> vn(3 * nv, 0.0), fn(3 * nt, 0.0);
+    std::vector<std::pair<uint64_t, size_t>> edges(3 * nt);
+    for (size_t t = 0; t < nt; ++t) {
+        double p[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) p[k][c] = m.v[3 * (size_t)m.tri[3 * t + k] + c];
+        const double e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+        const double e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+        const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (l > 0.0) {
+            for (int c = 0; c < 3; ++c) fn[3 * t + c] = n[c] / l;
+            for (int k = 0; k < 3; ++k) {
+                const double *o = p[k], *q1 = p[(k + 1) % 3], *q2 = p[(k + 2) % 3];
+                const double u[3] = {q1[0] - o[0], q1[1] - o[1], q1[2] - o[2]};
+                const double w[3] = {q2[0] - o[0], q2[1] - o[1], q2[2] - o[2]};
+                const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+                const double ang = std::atan2(std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]),
+                                              u[0] * w[0] + u[1] * w[1] + u[2] * w[2]);
+                for (int c = 0; c < 3; ++c) vn[3 * (size_t)m.tri[3 * t + k] + c] += fn[3 * t + c] * ang;
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t i0 = m.tri[3 * t + k], i1 = m.tri[3 * t + (k + 1) % 3];
+            edges[3 * t + k] = {((uint64_t)std::min(i0, i1) << 32) | std::max(i0, i1), 3 * t + k};
+        }
+    }
+    auto unit = [](double *x) {
+        const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        if (l > 0.0) { x[0] /= l; x[1] /= l; x[2] /= l; }
+    };
+    m.vn.assign(3 * nv, 0.0f);
+    for (size_t q = 0; q < nv; ++q) {
+        unit(&vn[3 * q]);
+        for (int c = 0; c < 3; ++c) m.vn[3 * q + c] = (float)vn[3 * q + c];
+    }
+    std::sort(edges.begin(), edges.end());
+    m.en.assign(9 * nt, 0.0f);
+    for (size_t a = 0; a < edges.size();) {
+        size_t b = a;
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (; b < edges.size() && edges[b].first == edges[a].first; ++b)
+            for (int c = 0; c < 3; ++c) sum[c] += fn[3 * (edges[b].second / 3) + c];
+        unit(sum);
+        for (size_t q = a; q < b; ++q)
+            for (int c = 0; c < 3; ++c) m.en[3 * edges[q].second + c] = (float)sum[c];
+        a = b;
+    }
+}
+
+// The mesh's node-pass cell grid for cell width h: over the triangles' bounds widened by
+// the 1.5 h query distance (and a margin against rounding), cells of at least 2 h and at
+// most 64 per axis; each cell lists, in ascending order, the triangles whose widened
+// bounds meet it.
+static void mesh_grid(HostMesh &m, float h)
+{
+    if (m.grid_h == h && !m.start.empty()) return;
+    const size_t nt = m.tri.size() / 3;
+    const double r = 1.5 * (double)h * 1.001 + 1e-6;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t q = 0; q < m.v.size() / 3; ++q)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], (double)m.v[3 * q + c]);
+            hi[c] = std::max(hi[c], (double)m.v[3 * q + c]);
+        }
+    if (nt == 0) { for (int c = 0; c < 3; ++c) lo[c] = hi[c] = 0.0; }
+    double ext = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        lo[c] -= r;
+        hi[c] += r;
+        ext = std::max(ext, hi[c] - lo[c]);
+    }
+    const double cell = std::max(2.0 * (double)h, ext / 64.0);
+    for (int c = 0; c < 3; ++c) {
+        m.origin[c] = (float)lo[c];
+        m.dims[c] = (uint32_t)std::max(1.0, std::ceil((hi[c] - lo[c]) / cell));
+    }
+    m.cell = (float)cell;
+    const size_t ncell = (size_t)m.dims[0] * m.dims[1] * m.dims[2];
+    std::vector<uint32_t> count(ncell + 1, 0u);
+    auto cells_of = [&](size_t t, uint32_t *c0, uint32_t *c1) {
+        for (int c = 0; c < 3; ++c) {
+            double a = 1e300, b = -1e300;
+            for (int k = 0; k < 3; ++k) {
+                const double x = m.v[3 * (size_t)m.tri[3 * t + k] + c];
+                a = std::min(a, x);
+                b = std::max(b, x);
+            }
+            const double fa = std::floor((a - r - (double)m.origin[c]) / (double)m.cell);
+            const double fb = std::floor((b + r - (double)m.origin[c]) / (double)m.cell);
+            c0[c] = (uint32_t)std::min(std::max(fa, 0.0), (double)(m.dims[c] - 1));
+            c1[c] = (uint32_t)std::min(std::max(fb, 0.0), (double)(m.dims[c] - 1));
+        }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            for (size_t c = 0; c < ncell; ++c) count[c + 1] += count[c];
+            m.start.assign(count.begin(), count.end());
+            m.ctri.assign(count[ncell], 0u);
+            std::fill(count.begin(), count.end(), 0u);
+        }
+        for (size_t t = 0; t < nt; ++t) {
+            uint32_t c0[3], c1[3];
+            cells_of(t, c0, c1);
+            for (uint32_t z = c0[2]; z <= c1[2]; ++z)
+                for (uint32_t y = c0[1]; y <= c1[1]; ++y)
+                    for (uint32_t x = c0[0]; x <= c1[0]; ++x) {
+                        const size_t c = x + (size_t)m.dims[0] * (y + (size_t)m.dims[1] * z);
+                        if (pass == 0) ++count[c + 1];
+                        else m.ctri[m.start[c] + count[c]++] = (uint32_t)t;
+                    }
+        }
+    }
+    m.grid_h = h;
+}
 
 struct HostParticles {
     size_t n = 0;
@@ -1082,6 +1215,8 @@ class World {
         w.dbg_mom = debug_ ? dbg_mom_.p : nullptr;
         w.ncdf = ncdf_.p; w.nbody = nbody_.p; w.binc = binc_.p; w.bany = bany_.p; w.breach = any_cpic_ ? breach_.p : nullptr; w.clist = clist_.p; w.stage_i = stage_i_.p; w.gridv_i = gridv_i_.p; w.any_cpic = any_cpic_ ? 1u : 0u;
         w.mesh_v = mesh_v_.p; w.mesh_t = mesh_t_.p; w.mesh_rp = mesh_rp_.p; w.mkey = mkey_.p; w.maff = maff_.p;
+        w.mesh_vn = mesh_vn_.p; w.mesh_en = mesh_en_.p; w.mesh_grid = mesh_grid_.p; w.mesh_cell = mesh_cell_.p;
+        w.mesh_ctri = mesh_ctri_.p; w.col_mesh = col_mesh_.p; w.mesh_pass = mesh_pass_ ? 1u : 0u;
         w.n_rp = n_rp_;
         w.p_cap = p_cap_; w.b_total = b_total_; w.h_total = h_total_; w.s_count = (uint32_t)slots_.size();
         w.clock = clock_.p; w.any_adaptive = any_adaptive_ ? 1u : 0u;
@@ -1127,8 +1262,10 @@ class World {
         std::vector<uint32_t> region_start, region_scene;
         uint32_t b = 0, hsum = 0;
         bool any_cpic = false, any_adaptive = false;
-        std::vector<float4> mv, mrp;
+        std::vector<float4> mv, mrp, mvn, men;
         std::vector<uint4> mt;
+        std::vector<MeshGridDev> mgrid;
+        std::vector<uint32_t> mcell, mctri, col_mesh;
         for (uint32_t i = 0; i < slots_.size(); ++i) {
             SceneDev &L = layout_[i];
             if (slots_[i].state != SLOT_ACTIVE) continue;
@@ -1157,29 +1294,50 @@ class World {
             L.dt_max = sc->dt;
             L.adaptive = sc->adaptive ? 1u : 0u;
             any_adaptive |= sc->adaptive;
-            // Mesh colliders (CPIC scenes only): rigid particles sampled at the cell width
-            // (src_mpm/pipeline.rs:310-316), geometry appended to the world's tables.
+            // Mesh colliders: geometry appended to the world's tables; CPIC scenes add
+            // their rigid particles sampled at the cell width (src_mpm/pipeline.rs:
+            // 310-316), the others the pseudo-normals and cell grid of the node pass.
             const uint32_t nmesh = (uint32_t)sc->meshes.size();
-            for (uint32_t k = 0; k < nmesh && sc->cpic; ++k) {
+            col_mesh.resize(ncols, NONE);
+            for (uint32_t k = 0; k < nmesh; ++k) {
                 HostMesh &m = sc->meshes[k];
-                if (m.sampled_h != sc->h) {
-                    m.samples = sample_mesh_surface(m.v.data(), m.tri.data(), (uint32_t)(m.tri.size() / 3), sc->h);
-                    m.sampled_h = sc->h;
-                }
                 const uint32_t v0 = (uint32_t)mv.size(), t0 = (uint32_t)mt.size();
                 const uint32_t cid = L.col_count - nmesh + k;
                 for (size_t q = 0; q < m.v.size() / 3; ++q) mv.push_back(make_float4(m.v[3 * q], m.v[3 * q + 1], m.v[3 * q + 2], 0.0f));
                 for (size_t q = 0; q < m.tri.size() / 3; ++q)
                     mt.push_back(make_uint4(v0 + m.tri[3 * q], v0 + m.tri[3 * q + 1], v0 + m.tri[3 * q + 2], (i << 4) | cid));
-                for (size_t q = 0; q < m.samples.tri.size(); ++q) {
-                    const uint32_t tid = t0 + m.samples.tri[q];
-                    float tf;
-                    std::memcpy(&tf, &tid, 4);
-                    mrp.push_back(make_float4(m.samples.pts[3 * q], m.samples.pts[3 * q + 1], m.samples.pts[3 * q + 2], tf));
+                if (sc->cpic) {
+                    if (m.sampled_h != sc->h) {
+                        m.samples = sample_mesh_surface(m.v.data(), m.tri.data(), (uint32_t)(m.tri.size() / 3), sc->h);
+                        m.sampled_h = sc->h;
+                    }
+                    for (size_t q = 0; q < m.samples.tri.size(); ++q) {
+                        const uint32_t tid = t0 + m.samples.tri[q];
+                        float tf;
+                        std::memcpy(&tf, &tid, 4);
+                        mrp.push_back(make_float4(m.samples.pts[3 * q], m.samples.pts[3 * q + 1], m.samples.pts[3 * q + 2], tf));
+                    }
+                    continue;
                 }
+                mvn.resize(v0, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+                men.resize(3 * (size_t)t0, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+                for (size_t q = 0; q < m.v.size() / 3; ++q) mvn.push_back(make_float4(m.vn[3 * q], m.vn[3 * q + 1], m.vn[3 * q + 2], 0.0f));
+                for (size_t q = 0; q < 3 * (m.tri.size() / 3); ++q) men.push_back(make_float4(m.en[3 * q], m.en[3 * q + 1], m.en[3 * q + 2], 0.0f));
+                mesh_grid(m, sc->h);
+                MeshGridDev gd;
+                gd.ox = m.origin[0]; gd.oy = m.origin[1]; gd.oz = m.origin[2]; gd.inv_cell = 1.0f / m.cell;
+                gd.nx = m.dims[0]; gd.ny = m.dims[1]; gd.nz = m.dims[2];
+                gd.cell_base = (uint32_t)mcell.size();
+                const uint32_t c0 = (uint32_t)mctri.size();
+                for (uint32_t st0 : m.start) mcell.push_back(c0 + st0);
+                for (uint32_t ct : m.ctri) mctri.push_back(t0 + ct);
+                col_mesh[L.col_start + cid] = (uint32_t)mgrid.size();
+                mgrid.push_back(gd);
             }
         }
-        const bool any_mesh = !mrp.empty();
+        col_mesh.resize(ncols, NONE);
+        const bool any_mesh = !mrp.empty();   // rigid particles of CPIC scenes' meshes
+        const bool any_geom = !mt.empty();    // mesh geometry of any scene
         // Dense block ids are ranks of occupied hash slots: sizing the block tables by
         // the slot count keeps every rank (and each scene's overflow block) in range
         // even when a scene exceeds its capacity (reported as an error).
@@ -1196,7 +1354,11 @@ class World {
                           (debug_ && (size_t)NODES_PER_BLOCK * 3 * std::max<size_t>(hsum, 1) > dbg_mom_.n) ||
                           (any_cpic && (size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > ncdf_.n) ||
                           (any_mesh && ((size_t)NODES_PER_BLOCK * std::max<size_t>(hsum, 1) > mkey_.n ||
-                                        mv.size() > mesh_v_.n || mt.size() > mesh_t_.n || mrp.size() > mesh_rp_.n));
+                                        mrp.size() > mesh_rp_.n)) ||
+                          (any_geom && (mv.size() > mesh_v_.n || mt.size() > mesh_t_.n || mvn.size() > mesh_vn_.n ||
+                                        men.size() > mesh_en_.n || mgrid.size() > mesh_grid_.n ||
+                                        mcell.size() > mesh_cell_.n || mctri.size() > mesh_ctri_.n ||
+                                        col_mesh.size() > col_mesh_.n));
         if (grow) {
             nxm_status st = sync();
             if (st != NXM_OK) return st;
@@ -1311,15 +1473,31 @@ class World {
                 HIP_TRY(hipMemsetAsync(maff_.p, 0, maff_.n * sizeof(uint32_t), stream_));
                 mesh_nodes_clear_ = mkey_.n;
             }
+            HIP_TRY(mesh_rp_.reserve(mrp.size()));
+            nxm_status ms = put_async(mesh_rp_.p, mrp.data(), mrp.size() * sizeof(float4));
+            if (ms != NXM_OK) return ms;
+        }
+        if (any_geom) {
             HIP_TRY(mesh_v_.reserve(mv.size()));
             HIP_TRY(mesh_t_.reserve(mt.size()));
-            HIP_TRY(mesh_rp_.reserve(mrp.size()));
+            HIP_TRY(mesh_vn_.reserve(std::max<size_t>(mvn.size(), 1)));
+            HIP_TRY(mesh_en_.reserve(std::max<size_t>(men.size(), 1)));
+            HIP_TRY(mesh_grid_.reserve(std::max<size_t>(mgrid.size(), 1)));
+            HIP_TRY(mesh_cell_.reserve(std::max<size_t>(mcell.size(), 1)));
+            HIP_TRY(mesh_ctri_.reserve(std::max<size_t>(mctri.size(), 1)));
+            HIP_TRY(col_mesh_.reserve(std::max<size_t>(col_mesh.size(), 1)));
             nxm_status ms = put_async(mesh_v_.p, mv.data(), mv.size() * sizeof(float4));
             if (ms == NXM_OK) ms = put_async(mesh_t_.p, mt.data(), mt.size() * sizeof(uint4));
-            if (ms == NXM_OK) ms = put_async(mesh_rp_.p, mrp.data(), mrp.size() * sizeof(float4));
+            if (ms == NXM_OK && !mvn.empty()) ms = put_async(mesh_vn_.p, mvn.data(), mvn.size() * sizeof(float4));
+            if (ms == NXM_OK && !men.empty()) ms = put_async(mesh_en_.p, men.data(), men.size() * sizeof(float4));
+            if (ms == NXM_OK && !mgrid.empty()) ms = put_async(mesh_grid_.p, mgrid.data(), mgrid.size() * sizeof(MeshGridDev));
+            if (ms == NXM_OK && !mcell.empty()) ms = put_async(mesh_cell_.p, mcell.data(), mcell.size() * sizeof(uint32_t));
+            if (ms == NXM_OK && !mctri.empty()) ms = put_async(mesh_ctri_.p, mctri.data(), mctri.size() * sizeof(uint32_t));
+            if (ms == NXM_OK && !col_mesh.empty()) ms = put_async(col_mesh_.p, col_mesh.data(), col_mesh.size() * sizeof(uint32_t));
             if (ms != NXM_OK) return ms;
         }
         n_rp_ = (uint32_t)mrp.size();
+        mesh_pass_ = !mgrid.empty();
         HIP_TRY(hkeys_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hvals_.reserve(std::max<uint32_t>(h_total_, 1)));
         HIP_TRY(hmark_.reserve(std::max<uint32_t>(h_total_, 1)));
@@ -1571,8 +1749,12 @@ class World {
     DevBuf<float> stage_i_;
     DevBuf<float4> gridv_i_;
     bool any_cpic_ = false;
-    DevBuf<float4> mesh_v_, mesh_rp_;  // mesh colliders of CPIC scenes (k_mesh_cdf)
+    DevBuf<float4> mesh_v_, mesh_rp_;  // mesh colliders: geometry; rigid particles of CPIC scenes (k_mesh_cdf)
     DevBuf<uint4> mesh_t_;
+    DevBuf<float4> mesh_vn_, mesh_en_; // node pass (scenes without CPIC): pseudo-normals,
+    DevBuf<MeshGridDev> mesh_grid_;    // cell grids and lists, each world collider's mesh
+    DevBuf<uint32_t> mesh_cell_, mesh_ctri_, col_mesh_;
+    bool mesh_pass_ = false;           // mgrid non-empty: k_grid_update runs its mesh variant
     DevBuf<unsigned long long> mkey_;  // per node, kept empty between substeps
     DevBuf<uint32_t> maff_;
     size_t mesh_nodes_clear_ = 0;      // nodes of mkey_ / maff_ set to empty since allocation
@@ -2016,8 +2198,6 @@ nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *sc, const nxm_trimesh *m, 
     if (!sc || (!m && n)) return fail(NXM_ERR_INVALID_ARGUMENT, "null argument");
     const size_t ncub = sc->cols.size() - sc->meshes.size();
     if (ncub + n > NXM_MAX_COLLIDERS) return fail(NXM_ERR_TOO_MANY_COLLIDERS, "at most 16 colliders");
-    if (n && !sc->cpic)
-        return fail(NXM_ERR_INVALID_ARGUMENT, "mesh colliders act through CPIC only: enable it first (nxm_scene_set_cpic)");
     std::vector<HostMesh> meshes(n);
     std::vector<ColliderDev> bodies(n);
     for (size_t k = 0; k < n; ++k) {
@@ -2028,7 +2208,16 @@ nxm_status nxm_scene_set_trimesh_colliders(nxm_scene *sc, const nxm_trimesh *m, 
                 return fail(NXM_ERR_INVALID_ARGUMENT, "triangle vertex index out of range (mesh " + std::to_string(k) + ")");
         meshes[k].v.assign(m[k].vertices, m[k].vertices + (size_t)m[k].num_vertices * 3);
         meshes[k].tri.assign(m[k].triangles, m[k].triangles + (size_t)m[k].num_triangles * 3);
+        for (float x : meshes[k].v)
+            if (!std::isfinite(x)) return fail(NXM_ERR_INVALID_ARGUMENT, "non-finite mesh vertex (mesh " + std::to_string(k) + ")");
+        mesh_pseudo_normals(meshes[k]);
         bodies[k] = collider_record(m[k].body, SHAPE_TRIMESH);
+        double reach = 0.0;   // bounding sphere of the vertices about the body origin (block masks)
+        for (size_t q = 0; q < meshes[k].v.size() / 3; ++q) {
+            const double x = meshes[k].v[3 * q], y = meshes[k].v[3 * q + 1], z = meshes[k].v[3 * q + 2];
+            reach = std::max(reach, std::sqrt(x * x + y * y + z * z));
+        }
+        bodies[k].reach = (float)(reach * (1.0 + 1e-5));
     }
     std::vector<ColliderDev> cub(sc->cols.begin(), sc->cols.begin() + ncub);
     // The cuboids' current poses first (replace_colliders pulls them), then the meshes.
@@ -2322,8 +2511,6 @@ nxm_status nxm_scene_set_cpic(nxm_scene *sc, int enabled)
     if (!sc) return fail(NXM_ERR_INVALID_ARGUMENT, "null scene");
     const bool on = enabled != 0;
     if (on == sc->cpic) return NXM_OK;
-    if (!on && !sc->meshes.empty())
-        return fail(NXM_ERR_INVALID_ARGUMENT, "mesh colliders act through CPIC only: remove them first");
     World *w = sc->world;
     if (w) {
         // The world's tables and buffers change: bring the state to the host and re-enter.

@@ -622,6 +622,121 @@ __device__ inline f3 project_shape(const ColliderDev &c, f3 pt, bool &inside)
     return quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], lp) + t;
 }
 
+// ------------------------------------------------------------------------------------
+// Mesh colliders in the grid collider pass (CPIC off): grid_update_collide.rs:53-67 with
+// TriMesh::project_local_point (src_rbd_shaders/shapes/trimesh.rs:155-207).
+
+// A mesh's body-frame cell grid over the bounds of its triangles widened by the 1.5 h
+// query distance: cell (ix, iy, iz) lists, in ascending order, every triangle whose
+// widened bounds meet the cell, so a point's cell holds every triangle within 1.5 h of it.
+struct MeshGridDev {
+    float ox, oy, oz, inv_cell;
+    uint32_t nx, ny, nz, cell_base;   // cell_base: the mesh's first entry of MeshSetDev::cell
+};
+
+// The world's mesh tables: vertices and triangles (global vertex ids), per vertex its
+// pseudo-normal, per triangle those of its edges ab, bc, ca; per mesh its grid; per grid
+// cell the start of its triangle list in ctri (cell[c + 1] its end); per world collider
+// its mesh (NONE for the analytic shapes).
+struct MeshSetDev {
+    const float4 *v;
+    const uint4 *t;
+    const float4 *vn;
+    const float4 *en;
+    const MeshGridDev *grid;
+    const uint32_t *cell;
+    const uint32_t *ctri;
+    const uint32_t *col_mesh;
+};
+
+// Triangle projection with feature location as the mesh query takes it (trimesh.rs:
+// 237-391): the Voronoi regions of a, b, c, then of the edges ab, ac, bc (signed volumes),
+// then the face; a degenerate face leaves the point itself (the solid case).  ft: 0
+// vertex (fi 0 a, 1 b, 2 c), 1 edge (fi 0 ab, 1 bc, 2 ac), 2 face, 3 solid.
+__device__ inline f3 tri_project(f3 a, f3 b, f3 c, f3 pt, int &ft, int &fi)
+{
+    const f3 ab = b - a, ac = c - a, ap = pt - a;
+    const float ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+    if (ab_ap <= 0.0f && ac_ap <= 0.0f) { ft = 0; fi = 0; return a; }
+    const f3 bp = pt - b;
+    const float ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+    if (ab_bp >= 0.0f && ac_bp <= ab_bp) { ft = 0; fi = 1; return b; }
+    const f3 cp = pt - c;
+    const float ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+    if (ac_cp >= 0.0f && ab_cp <= ac_cp) { ft = 0; fi = 2; return c; }
+    const f3 bc = c - b;
+    const f3 n = cross(ab, ac);
+    const float vc = dot(n, cross(ab, ap));
+    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
+        ft = 1; fi = 0;
+        return a + ab * (ab_ap / dot(ab, ab));
+    }
+    const float vb = -dot(n, cross(ac, cp));
+    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
+        ft = 1; fi = 2;
+        return a + ac * (ac_ap / dot(ac, ac));
+    }
+    const float va = dot(n, cross(bc, bp));
+    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
+        ft = 1; fi = 1;
+        return b + bc * (dot(bc, bp) / dot(bc, bc));
+    }
+    if (va + vb + vc != 0.0f) {
+        const float denom = 1.0f / (va + vb + vc);
+        ft = 2; fi = 0;
+        return a + ab * (vb * denom) + ac * (vc * denom);
+    }
+    ft = 3; fi = 0;
+    return pt;
+}
+
+// The nearest triangle of mesh m within max_dist of the world point pt (a later triangle
+// wins a tie, dist <= best), taken in the body frame by the inverse pose (R^-1 pt -
+// R^-1 t, the reference's pose.inverse() * point); inside when the nearest feature's
+// pseudo-normal points away from the point (a face's is ab x ac).  False when no
+// triangle is that near.
+__device__ inline bool project_trimesh(const MeshSetDev &ms, uint32_t m, const ColliderDev &c, float max_dist, f3 pt,
+                                       f3 &proj, bool &inside)
+{
+    const MeshGridDev g = ms.grid[m];
+    const f3 t = mk3(c.t[0], c.t[1], c.t[2]);
+    const f3 lp = quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], pt) - quat_rotate(-c.q[0], -c.q[1], -c.q[2], c.q[3], t);
+    const float fx = (lp.x - g.ox) * g.inv_cell, fy = (lp.y - g.oy) * g.inv_cell, fz = (lp.z - g.oz) * g.inv_cell;
+    // outside the grid (or NaN): no triangle within reach
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)g.nx && fy < (float)g.ny && fz < (float)g.nz)) return false;
+    const uint32_t cell = g.cell_base + (uint32_t)fx + g.nx * ((uint32_t)fy + g.ny * (uint32_t)fz);
+    const uint32_t kb = ms.cell[cell], ke = ms.cell[cell + 1];
+    float best = max_dist;
+    uint32_t bt = NONE;
+    int bft = 0, bfi = 0;
+    f3 bp = lp;
+    for (uint32_t k = kb; k < ke; ++k) {
+        const uint32_t tri = ms.ctri[k];
+        const uint4 tv = ms.t[tri];
+        const float4 A = ms.v[tv.x], B = ms.v[tv.y], C = ms.v[tv.z];
+        int ft, fi;
+        const f3 p = tri_project(mk3(A.x, A.y, A.z), mk3(B.x, B.y, B.z), mk3(C.x, C.y, C.z), lp, ft, fi);
+        const float dist = length(p - lp);
+        if (dist <= best) { best = dist; bt = tri; bft = ft; bfi = fi; bp = p; }
+    }
+    if (bt == NONE) return false;
+    const uint4 tv = ms.t[bt];
+    f3 pn;
+    if (bft == 0) {
+        const float4 q = ms.vn[bfi == 0 ? tv.x : (bfi == 1 ? tv.y : tv.z)];
+        pn = mk3(q.x, q.y, q.z);
+    } else if (bft == 1) {
+        const float4 q = ms.en[3 * (size_t)bt + bfi];
+        pn = mk3(q.x, q.y, q.z);
+    } else {
+        const float4 A = ms.v[tv.x], B = ms.v[tv.y], C = ms.v[tv.z];
+        pn = cross(mk3(B.x - A.x, B.y - A.y, B.z - A.z), mk3(C.x - A.x, C.y - A.y, C.z - A.z));
+    }
+    inside = dot(pn, lp - bp) <= 0.0f;
+    proj = quat_rotate(c.q[0], c.q[1], c.q[2], c.q[3], bp) + t;
+    return true;
+}
+
 // BoundaryCondition::project_velocity (solver/boundary_condition.rs:77-114)
 __device__ inline f3 bc_project(uint32_t ty, float friction, f3 vel, f3 n)
 {
@@ -643,8 +758,11 @@ __device__ inline f3 bc_project(uint32_t ty, float friction, f3 vel, f3 n)
 
 // mask: colliders that can reach the node's block (bit i for collider i); the others
 // can be neither within 1.5 h of the node nor contain it, so skipping them leaves the
-// reference's result unchanged.
-__device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32_t mask, float h, float dt, f3 pt, f3 vel)
+// reference's result unchanged.  MESH: mesh colliders (their mesh ms.col_mesh[col_base +
+// i]) report only triangles within 1.5 h; without it the caller's scenes have none.
+template <bool MESH>
+__device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32_t mask, float h, float dt, f3 pt, f3 vel,
+                                  const MeshSetDev &ms, uint32_t col_base)
 {
     float cap = h * 1.5f;
     float best = MAX_FLT_REF;
@@ -653,7 +771,12 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
     for (uint32_t i = 0; i < ncol; ++i) {
         if (!(mask & (1u << i))) continue;
         bool inside;
-        f3 proj = project_shape(cols[i], pt, inside);
+        f3 proj;
+        if (cols[i].shape == SHAPE_TRIMESH) {
+            if (!MESH || !project_trimesh(ms, ms.col_mesh[col_base + i], cols[i], cap, pt, proj, inside)) continue;
+        } else {
+            proj = project_shape(cols[i], pt, inside);
+        }
         f3 dpt = proj - pt;
         bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;
         if (inside || within) {
@@ -700,7 +823,7 @@ __device__ inline void node_cdf(const ColliderDev *cols, uint32_t ncol, uint32_t
     aff = 0u;
     closest = NONE;
     for (uint32_t i = 0; i < ncol; ++i) {
-        if (!(mask & (1u << i))) continue;
+        if (!(mask & (1u << i)) || cols[i].shape == SHAPE_TRIMESH) continue;   // meshes: k_mesh_cdf
         bool inside;
         const f3 proj = project_shape(cols[i], pt, inside);
         const f3 dpt = proj - pt;

@@ -60,6 +60,15 @@ struct WorldDev {
     unsigned long long *mkey;      // per node: least (distance bits << 32) | collider, ~0 when empty
     uint32_t *maff;                // per node: the meshes' affinity bits
     uint32_t n_rp;
+    // Mesh colliders of scenes without CPIC (the grid collider pass projects onto them):
+    // vertex and edge pseudo-normals, cell grids and lists, each world collider's mesh.
+    const float4 *mesh_vn;
+    const float4 *mesh_en;
+    const MeshGridDev *mesh_grid;
+    const uint32_t *mesh_cell;
+    const uint32_t *mesh_ctri;
+    const uint32_t *col_mesh;
+    uint32_t mesh_pass;            // some scene without CPIC has mesh colliders
     uint32_t p_cap, b_total, h_total, s_count;   // b_total, h_total: allocated block ids, hash slots
     // Per scene slot: simulated time and the CFL bound for adaptive substeps (null: no
     // clocks kept, as in the no-allocation substep); any_adaptive: some scene is adaptive.

@@ -474,11 +474,11 @@ __device__ inline void block_info_part(const WorldDev &w)
             const uint32_t i = j - 8;
             w.bup[8 * d + i] = i == 0 ? d : find_block(w, sc, v.x + ASSOC[i][0], v.y + ASSOC[i][1], v.z + ASSOC[i][2]);
         }
-        // Collider mask: analytic collider j can touch a node of this block only if its
-        // bounding sphere (radius reach about the body origin) comes within 1.5 h (the
-        // projection reach) of the block's node sphere.  Meshes act through CPIC only.
+        // Collider mask: collider j can touch a node of this block only if its bounding
+        // sphere (radius reach about the body origin) comes within 1.5 h (the projection
+        // reach) of the block's node sphere.
         bool hit = false;
-        if (j < sc.col_count && w.cols[sc.col_start + j].shape != SHAPE_TRIMESH) {
+        if (j < sc.col_count) {
             const ColliderDev &c = w.cols[sc.col_start + j];
             const float h = sc.h;
             const f3 center = mk3(((float)(4 * v.x) + 1.5f) * h, ((float)(4 * v.y) + 1.5f) * h, ((float)(4 * v.z) + 1.5f) * h);
@@ -1100,8 +1100,10 @@ __device__ inline void clear_hash(const WorldDev &w)
 // ------------------------------------------------------------------------------------
 // Grid update: gpu_grid_update + gpu_grid_update_collide (grid_update.rs:21-88,
 // grid_update_collide.rs:105-183) on the summed tiles.  CPIC scenes update both velocity
-// fields and skip the collider pass (solver/grid_update.rs:26-50).
-template <bool CPIC>
+// fields and skip the collider pass (solver/grid_update.rs:26-50).  MESH: the world has
+// mesh colliders in scenes without CPIC, whose projection the collider pass then runs
+// (its registers kept out of the common variant).
+template <bool CPIC, bool MESH>
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
     clear_hash(w);   // not read after the block tables (k_block_count, k_mesh_cdf) are built
@@ -1163,7 +1165,8 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
         const uint32_t mask = cp ? 0u : w.bmask[d];
         if (mask) {
             const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
-            vel = collide_node(w.cols + sc.col_start, sc.col_count, mask, sc.h, sc.dt, pt, vel);
+            const MeshSetDev ms{w.mesh_v, w.mesh_t, w.mesh_vn, w.mesh_en, w.mesh_grid, w.mesh_cell, w.mesh_ctri, w.col_mesh};
+            vel = collide_node<MESH>(w.cols + sc.col_start, sc.col_count, mask, sc.h, sc.dt, pt, vel, ms, sc.col_start);
         }
         w.gridv[n] = make_float4(vel.x, vel.y, vel.z, m);
     }
@@ -2037,7 +2040,8 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (ev) hipExtLaunchKernelGGL(k_p2g<true>, dim3(lc.p2g_blocks), dim3(64), 0, st, ev[5], ev[6], 0, w);
         else k_p2g<true><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);
-        k_grid_update<true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        if (w.mesh_pass) k_grid_update<true, true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        else k_grid_update<true, false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
         // The contact particles' G2P touches other particles than the plain one and is
         // latency-bound on few waves: it runs beside it on the side stream when there is one.
@@ -2062,7 +2066,8 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (ev) hipExtLaunchKernelGGL(k_p2g<false>, dim3(lc.p2g_blocks), dim3(64), 0, st, ev[5], ev[6], 0, w);
         else k_p2g<false><<<lc.p2g_blocks, 64, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[2], st);
-        k_grid_update<false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        if (w.mesh_pass) k_grid_update<false, true><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
+        else k_grid_update<false, false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
         auto g2p = w.any_adaptive ? k_g2p_update<false, true> : k_g2p_update<false, false>;
         if (ev) hipExtLaunchKernelGGL(g2p, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st, ev[7], ev[8], 0, w);

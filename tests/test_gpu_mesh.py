@@ -142,23 +142,29 @@ def test_mesh_scene_batch_is_bitwise_alone(pipeline):
 
 
 def test_mesh_collider_boundary_errors(pipeline):
-    """Meshes need CPIC; CPIC cannot be switched off under them; indices are checked;
-    cuboids and meshes together are limited to 16; setting cuboids keeps the meshes."""
+    """Meshes act with CPIC (the contact field) or without it (the node pass), and CPIC
+    switches under them; indices and vertices are checked; cuboids and meshes together are
+    limited to 16; setting cuboids keeps the meshes."""
     sc = scenes.sand_on_mesh_ramp(500, seed=25)
     mesh = sc.meshes[0]
     plain = sc.copy()
     plain.cpic = False
     plain.meshes = []
     st = mpm.MpmState.from_scene(pipeline, plain)
-    with pytest.raises(mpm.MpmError):
-        st.set_trimesh_colliders([mesh])
-    st.set_cpic(True)
     st.set_trimesh_colliders([mesh])
-    with pytest.raises(mpm.MpmError):
-        st.set_cpic(False)
+    pipeline.step(st)
+    st.set_cpic(True)
+    pipeline.step(st)
+    st.set_cpic(False)
+    pipeline.step(st)
+    st.set_cpic(True)
     bad = scenes.TriMesh(mesh.vertices, np.array([[0, 1, 999]], np.uint32), mesh.body)
     with pytest.raises(mpm.MpmError):
         st.set_trimesh_colliders([bad])
+    nan = mesh.vertices.copy()
+    nan[0, 0] = np.nan
+    with pytest.raises(mpm.MpmError, match="non-finite"):
+        st.set_trimesh_colliders([scenes.TriMesh(nan, mesh.triangles, mesh.body)])
     with pytest.raises(mpm.MpmError):
         st.set_colliders(np.concatenate([sc.colliders] * 16))
     pipeline.step(st)
