@@ -1206,12 +1206,18 @@ struct GatherGrid {
         if (blk == NONE) return NONE;
         return blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3);
     }
+    // Without branches: a missing block's node reads node 0 (always allocated) and is
+    // replaced by zero, so the 27 reads of a particle go out back to back.
     __device__ f3 at(int tx, int ty, int tz) const
     {
-        const uint32_t i = index(tx, ty, tz);
-        if (i == NONE) return mk3(0, 0, 0);
+        const bool ox = tx >= 4, oy = ty >= 4, oz = tz >= 4;
+        const uint32_t z0 = oz ? n1 : n0, z1 = oz ? n3 : n2, z2 = oz ? n5 : n4, z3 = oz ? n7 : n6;
+        const uint32_t y0 = oy ? z1 : z0, y1 = oy ? z3 : z2;
+        const uint32_t blk = ox ? y1 : y0;
+        const bool none = blk == NONE;
+        const uint32_t i = none ? 0u : blk * NODES_PER_BLOCK + (tx & 3) + 4 * (ty & 3) + 16 * (tz & 3);
         const float4 g = gridv[i];
-        return mk3(g.x, g.y, g.z);
+        return none ? mk3(0, 0, 0) : mk3(g.x, g.y, g.z);
     }
 };
 
