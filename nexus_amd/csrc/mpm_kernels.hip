@@ -1630,9 +1630,12 @@ __device__ inline void prepare_next_sort(const WorldDev &w)
     mark_overflow_slots(w);
 }
 
-// One thread per sorted particle, nodes read from the grid.  In CPIC worlds this kernel
-// takes the plain particles and k_g2p_contact the contact ones, so that the contact code
-// does not weigh on the register budget of the common path.  Then the new positions give
+// One thread per sorted particle, nodes read from the grid: the launch (g2p_groups) gives
+// every workgroup at most 256 positions, and a thread takes at most one, so no loop keeps
+// the arenas' 60-odd plane addresses live in scalar registers (they spilled to vector
+// lanes and were read back for every particle).  In CPIC worlds this kernel takes the
+// plain particles and k_g2p_contact the contact ones, so that the contact code does not
+// weigh on the register budget of the common path.  Then the new positions give
 // the next substep's block keys and insertions (touch_keys): the output arena is the next
 // input arena, in the same order, so the next substep starts at its scan.  That second
 // loop runs whole waves (lanes past the range are invalid) over the positions this same
@@ -1643,10 +1646,10 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
     GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
-    for (uint32_t pos = pbegin + threadIdx.x; pos < pend; pos += blockDim.x) {
+    const uint32_t pos = pbegin + threadIdx.x;
+    if (pos < pend) {
         const uint32_t d = w.sblock[pos];
-        if (CPIC && contact_particle(w, pos, d)) continue;
-        g2p_particle<GatherGrid, false>(w, pos, d, grid);
+        if (!(CPIC && contact_particle(w, pos, d))) g2p_particle<GatherGrid, false>(w, pos, d, grid);
     }
     const uint32_t l = lane_id();
     for (uint32_t base = pbegin + (threadIdx.x & ~(uint32_t)(WAVE - 1)); base < pend; base += blockDim.x) {
@@ -2017,6 +2020,13 @@ static inline int grid_for(uint64_t n, int threads, int cap)
     return (int)g;
 }
 
+// Workgroups of k_g2p_update: enough that xcd_range gives none more than 256 of the at
+// most p_hint sorted positions (one per thread).
+static inline int g2p_groups(uint64_t p_hint)
+{
+    return grid_for(p_hint, 256, std::numeric_limits<int>::max() / 2);
+}
+
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys)
 {
     const int T = 256;
@@ -2061,8 +2071,8 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (w.any_adaptive) k_g2p_contact<true><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
         else k_g2p_contact<false><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
         auto g2p = w.any_adaptive ? k_g2p_update<true, true> : k_g2p_update<true, false>;
-        if (ev) hipExtLaunchKernelGGL(g2p, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st, ev[7], ev[8], 0, w);
-        else g2p<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint)), dim3(T), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<g2p_groups(lc.p_hint), T, 0, st>>>(w);
         if (cs != st) {
             (void)hipEventRecord(lc.join, cs);
             (void)hipStreamWaitEvent(st, lc.join, 0);
@@ -2076,8 +2086,8 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         else k_grid_update<false, false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
         auto g2p = w.any_adaptive ? k_g2p_update<false, true> : k_g2p_update<false, false>;
-        if (ev) hipExtLaunchKernelGGL(g2p, dim3(grid_for(lc.p_hint, T, cap)), dim3(T), 0, st, ev[7], ev[8], 0, w);
-        else g2p<<<grid_for(lc.p_hint, T, cap), T, 0, st>>>(w);
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint)), dim3(T), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<g2p_groups(lc.p_hint), T, 0, st>>>(w);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
 }
