@@ -32,6 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # 16-lane SIMDs, four cycles each), 256 CUs at 2.4 GHz = 614.4 G wave-instructions/s.
 VALU_PEAK_G = 256 * 2.4
 SEEDS = {"cube": 0x4E584D01, "sand": 0x4E584D02, "fluid": 0x4E584D03}
+# The kernel variants the bench workloads launch (no CPIC, no adaptive substeps), under the
+# names rocprofv3 records them with in profiles/counters_<workload>.json.
+PROFILE_KERNELS = {"k_p2g": "k_p2g<false>", "k_g2p_update": "k_g2p_update<false, false>"}
 
 
 def build_scene(name: str):
@@ -356,11 +359,14 @@ def main(argv=None):
     counters = counters_from_profiles(args.workload)
     figures, valu = {}, {}
     for kernel, kname, kernel_ms, stage_ms, nbytes in (
-            ("k_p2g", "k_p2g<false>", stages.p2g_kernel_ms, stages.p2g_ms, p2g_algorithmic_bytes(n, nblocks)),
-            ("k_g2p_update", "k_g2p_update<false>", stages.g2p_update_kernel_ms, stages.g2p_update_ms,
+            ("k_p2g", PROFILE_KERNELS["k_p2g"], stages.p2g_kernel_ms, stages.p2g_ms, p2g_algorithmic_bytes(n, nblocks)),
+            ("k_g2p_update", PROFILE_KERNELS["k_g2p_update"], stages.g2p_update_kernel_ms, stages.g2p_update_ms,
              g2p_algorithmic_bytes(n, nblocks, plastic))):
         achieved = nbytes / (kernel_ms * 1e-3) / 1e9
         prof = counters.get(kname, {})
+        if counters and not prof:
+            print(f"bench: no profile figures for {kname} in profiles/counters_{args.workload}.json",
+                  file=sys.stderr)
         figures[kernel] = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                            "traffic": prof.get("traffic_bytes"),
