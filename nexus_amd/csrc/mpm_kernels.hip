@@ -1221,8 +1221,10 @@ struct GatherGrid {
     }
 };
 
+// Returns the position and flags it wrote (xo, flo), for the caller's key insertion.
 template <class Gather, bool CPIC>
-__device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, uint32_t d, Gather &gather)
+__device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, uint32_t d, Gather &gather, f3 &xo,
+                                             uint32_t &flo)
 {
     {
         const uint32_t slot = w.perm[pos];
@@ -1431,6 +1433,8 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
         if (sand) { w.out.plane(PS1)[pos] = ps1; w.out.plane(PS2)[pos] = ps2; }
         w.out.uplane(PFLAGS)[pos] = fl;
         w.out.uplane(PID)[pos] = id;
+        xo = npos;
+        flo = fl;
     }
 }
 
@@ -1637,9 +1641,9 @@ __device__ inline void prepare_next_sort(const WorldDev &w)
 // plain particles and k_g2p_contact the contact ones, so that the contact code does not
 // weigh on the register budget of the common path.  Then the new positions give
 // the next substep's block keys and insertions (touch_keys): the output arena is the next
-// input arena, in the same order, so the next substep starts at its scan.  That second
-// loop runs whole waves (lanes past the range are invalid) over the positions this same
-// thread has just written.
+// input arena, in the same order, so the next substep starts at its scan.  The keys are
+// taken by whole waves (lanes past the range are invalid) from the position and flags the
+// thread has just written, still in its registers.
 template <bool CPIC, bool ADAPT>
 __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
 {
@@ -1647,24 +1651,17 @@ __global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
     xcd_range(w.g->p_out, pbegin, pend);
     GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
     const uint32_t pos = pbegin + threadIdx.x;
-    if (pos < pend) {
+    bool valid = pos < pend;
+    f3 xo = mk3(0, 0, 0);
+    uint32_t flo = 0u;
+    if (valid) {
         const uint32_t d = w.sblock[pos];
-        if (!(CPIC && contact_particle(w, pos, d))) g2p_particle<GatherGrid, false>(w, pos, d, grid);
+        if (CPIC && contact_particle(w, pos, d)) valid = false;   // keyed by k_g2p_contact
+        else g2p_particle<GatherGrid, false>(w, pos, d, grid, xo, flo);
     }
-    const uint32_t l = lane_id();
-    for (uint32_t base = pbegin + (threadIdx.x & ~(uint32_t)(WAVE - 1)); base < pend; base += blockDim.x) {
-        const uint32_t pos = base + l;
-        bool valid = pos < pend;
-        uint32_t s = 0;
-        float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (valid) {
-            s = flags_scene(w.out.uplane(PFLAGS)[pos]);
-            if (CPIC) valid = !contact_particle(w, pos, w.sblock[pos]);   // keyed by k_g2p_contact
-            x = w.out.plane(PX)[pos];
-            y = w.out.plane(PY)[pos];
-            z = w.out.plane(PZ)[pos];
-        }
-        touch_keys(w, pos, valid, s, x, y, z);
+    if (pbegin + (threadIdx.x & ~(uint32_t)(WAVE - 1)) < pend) {   // the wave holds positions
+        const uint32_t s = valid ? flags_scene(flo) : 0u;
+        touch_keys(w, pos, valid, s, xo.x, xo.y, xo.z);
         if (ADAPT) next_dt_bound(w, pos, valid, s);
     }
     prepare_next_sort(w);
@@ -1679,7 +1676,9 @@ __global__ void __launch_bounds__(256) k_g2p_contact(WorldDev w)
     GatherGrid grid{w.bup, w.gridv, w.gridv_i, w.ncdf, NONE, NONE, NONE, NONE, NONE, NONE, NONE, NONE};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t pos = w.clist[i];
-        g2p_particle<GatherGrid, true>(w, pos, w.sblock[pos], grid);
+        f3 xo;
+        uint32_t flo;
+        g2p_particle<GatherGrid, true>(w, pos, w.sblock[pos], grid, xo, flo);
     }
     const uint32_t l = lane_id();
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(WAVE - 1)); base < n;
