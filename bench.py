@@ -272,6 +272,40 @@ def counters_from_profiles(workload: str) -> dict:
         return {}
 
 
+def measured_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> dict:
+    """HBM bandwidth a plain device-to-device copy reaches on this GPU (torch `copy_` of a
+    1 GiB buffer, timed with events on the current stream): bytes read + bytes written per
+    copy / its average time, beside the datasheet's 8 TB/s."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x, y
+    return {"value": 2 * nbytes / (ms * 1e-3) / 1e9, "unit": "GB/s", "bytes_per_copy": 2 * nbytes, "ms_per_copy": ms,
+            "copies": reps}
+
+
+def copy_in_child() -> dict | None:
+    """measured_copy_gbs in a child process: torch's HIP runtime and the library's do not
+    share one process."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--measure-copy"], capture_output=True,
+                           text=True, timeout=300)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        return json.loads(lines[-1]) if r.returncode == 0 and lines else None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        return None
+
+
 def calibrate_copy(nbytes: int = 1 << 30, reps: int = 4):
     """A plain device-to-device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -307,12 +341,17 @@ def main(argv=None):
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--measure-copy", action="store_true",
+                    help="print only the measured device-copy bandwidth (run by the bench in a child process)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args(argv)
 
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.measure_copy:
+        print(json.dumps(measured_copy_gbs()), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (scenes shard across GPUs "
@@ -405,6 +444,12 @@ def main(argv=None):
     if valu:
         out["roofline_valu"] = valu
     st.close()
+    # The copy bandwidth measured on this GPU, beside the datasheet peak the rooflines use.
+    copy = copy_in_child()
+    out["hbm_copy"] = copy
+    if copy:
+        for fig in (out["roofline"], out["roofline_other"]):
+            fig["frac_of_copy"] = fig["achieved"] / copy["value"]
     if not args.no_cpic:
         out["cpic"] = bench_cpic(pipe, args.cpic_steps, autotune=not args.no_autotune)
     if not args.no_batch:
