@@ -1417,8 +1417,6 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 npos = mk3(px, py, pz);
             }
         }
-        float *o = nullptr;
-        (void)o;
         w.out.plane(PX)[pos] = npos.x; w.out.plane(PY)[pos] = npos.y; w.out.plane(PZ)[pos] = npos.z;
         w.out.plane(PVX)[pos] = vel.x; w.out.plane(PVY)[pos] = vel.y; w.out.plane(PVZ)[pos] = vel.z;
         w.out.plane(PC0)[pos] = C.c0.x; w.out.plane(PC1)[pos] = C.c0.y; w.out.plane(PC2)[pos] = C.c0.z;
