@@ -316,10 +316,11 @@ nxm_status nxm_scene_read_colliders(nxm_scene *scene, nxm_cuboid *out, size_t n)
 /* MpmState::use_cpic (src_mpm/pipeline.rs:69, default off here): compatible
  * particle-in-cell transfers (pipeline.rs:433-460 with the use_cpic branches of p2g.rs,
  * grid_update.rs, g2p.rs and particle_update.rs) instead of the grid collider pass.
- * The contact distance field comes from the cuboids (grid_update_cdf.rs) and from the
- * mesh colliders' rigid particles (p2g_cdf.rs).  Switching it resets every particle's
- * contact field to zero (as for new particles); synchronizes when the scene is
- * resident; switching it off is refused while mesh colliders are set. */
+ * The contact distance field comes from the analytic colliders (grid_update_cdf.rs)
+ * and from the mesh colliders' rigid particles (p2g_cdf.rs); with it off, mesh colliders
+ * act through the grid collider pass's projection.  Switching it resets every
+ * particle's contact field to zero (as for new particles); synchronizes when the scene
+ * is resident. */
 nxm_status nxm_scene_set_cpic(nxm_scene *scene, int enabled);
 /* Per-particle contact field of the last step (Kinematics::cdf, solver/particle.rs:
  * 35-50 without rigid_vel, which lives only inside the step): affinity word (bit i:
