@@ -771,9 +771,12 @@ __device__ inline void load_flush_table(uint16_t *fdst, uint32_t l)
     if (l < 54u) reinterpret_cast<uint64_t *>(fdst)[l] = reinterpret_cast<const uint64_t *>(FLUSH_CODE.c)[l];
 }
 
-__device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const uint32_t *bad, float4 *stage,
+// With zero (a block whose particles do not reach every region), the flush also zeroes the
+// words it reads, which are all the words the pass added to: the tile is then clean for the
+// next pass without a full clear.
+__device__ __forceinline__ void p2g_flush(unsigned long long *tile, const uint32_t *bad, float4 *stage,
                                           uint32_t d, float smom, float sm, uint32_t l, uint32_t touched,
-                                          const uint16_t *fdst)
+                                          const uint16_t *fdst, bool zero)
 {
     __syncthreads();
     const double inv_smom = 1.0 / (double)smom, inv_sm = 1.0 / (double)sm;   // exact: powers of two
@@ -785,6 +788,10 @@ __device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const 
         float v4[4];
         #pragma unroll
         for (int c = 0; c < 4; ++c) v4[c] = Fixed::back(tile[c * TILE_NODES + t], c < 3 ? inv_smom : inv_sm);
+        if (zero) {
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) tile[c * TILE_NODES + t] = 0ull;
+        }
         if (any_bad) {
             #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -797,10 +804,13 @@ __device__ __forceinline__ void p2g_flush(const unsigned long long *tile, const 
     __syncthreads();
 }
 
-// Empties the tile and its non-finite marks for the next scatter pass.
-__device__ __forceinline__ void p2g_clear(unsigned long long *tile, uint32_t *bad, uint32_t l)
+// Empties the tile (unless the last flush left it clean) and its non-finite marks for the
+// next scatter pass.  Blocks that reach all 8 regions (dense ones) clear here: one tight
+// loop costs them less than zeroing inside the flush.
+__device__ __forceinline__ void p2g_clear(unsigned long long *tile, uint32_t *bad, uint32_t l, bool dirty)
 {
-    for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
+    if (dirty)
+        for (int k = l; k < 4 * TILE_NODES; k += WAVE) tile[k] = 0;
     if (l <= (uint32_t)bad_any<false>()) bad[l] = 0u;
 }
 
@@ -997,6 +1007,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const uint32_t *FL = w.in.uplane(PFLAGS);
     __shared__ __align__(8) uint16_t fdst[TILE_NODES];
     load_flush_table(fdst, l);   // read after the first __syncthreads
+    bool dirty = true;           // the tile may hold non-zero words (p2g_clear, p2g_flush)
     // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const uint32_t g = (G % NUM_XCD == 0) ? (b % NUM_XCD) * (G / NUM_XCD) + b / NUM_XCD : b;
@@ -1029,7 +1040,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
             const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
             const bool inc = CPIC && __ballot((pa & 0xFFFFu) != 0u) != 0ull;
-            p2g_clear(tile, bad, l);
+            p2g_clear(tile, bad, l, dirty);
             if (inc) load_tile_affinity(w, d, true, naff, l);
             __syncthreads();
             if (on) {
@@ -1037,15 +1048,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 else spill = p2g_one<CPIC, PART_ALL>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
             }
             const uint32_t touched = wave_or_mask8(spill);
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst, touched != 0xFFu);
+            dirty = touched == 0xFFu;
             if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
             if (CPIC && inc) {
                 // Second pass: the incompatible contributions of the particles with affinity.
-                p2g_clear(tile, bad, l);
+                p2g_clear(tile, bad, l, dirty);
                 __syncthreads();
                 if (on && (pa & 0xFFFFu) != 0u)
                     p2g_one<CPIC, PART_INCOMPATIBLE>(tile, bad, p, v, sc, smom, sm, bmom, bm, l % 3u, naff, pa);
-                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst);
+                p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst, touched != 0xFFu);
+                dirty = touched == 0xFFu;
             }
             unlink_untouched(w, d, touched, l);
             continue;
@@ -1067,21 +1080,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         const float smom = fixed_scale(FX::BITS, bound_exponent(bmom), np);
         const float sm = fixed_scale(FX::BITS, bound_exponent(bm), np);
         const bool inc = CPIC && __ballot(anyc) != 0ull;
-        p2g_clear(tile, bad, l);
+        p2g_clear(tile, bad, l, dirty);
         if (inc) load_tile_affinity(w, d, true, naff, l);
         __syncthreads();
         const uint32_t spill =
             inc ? p2g_block_pass<CPIC, PART_COMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff)
                 : p2g_block_pass<CPIC, PART_ALL>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
         const uint32_t touched = wave_or_mask8(spill);
-        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst);
+        p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage), d, smom, sm, l, touched, fdst, touched != 0xFFu);
+        dirty = touched == 0xFFu;
         if (CPIC && l == 0) w.binc[d] = inc ? 1u : 0u;
         if (CPIC && inc) {
             // Second pass: the incompatible contributions of the particles with affinity.
-            p2g_clear(tile, bad, l);
+            p2g_clear(tile, bad, l, dirty);
             __syncthreads();
             p2g_block_pass<CPIC, PART_INCOMPATIBLE>(w, tile, bad, first, np, v, sc, smom, sm, bmom, bm, l, naff);
-            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst);
+            p2g_flush(tile, bad, reinterpret_cast<float4 *>(w.stage_i), d, smom, sm, l, touched, fdst, touched != 0xFFu);
+            dirty = touched == 0xFFu;
         }
         unlink_untouched(w, d, touched, l);
     }
