@@ -239,7 +239,7 @@ __device__ inline uint32_t scan_len(const WorldDev &w) { return MODE == MODE_SLO
 
 // Runs of SCAN_TILE elements per scan tile for a scan of len elements (at most
 // SCAN_MAX_TILES tiles).
-constexpr uint32_t SCAN_MAX_TILES = 1024;
+constexpr uint32_t SCAN_MAX_TILES = 4096;
 __host__ __device__ inline uint32_t scan_chunks(uint32_t len)
 {
     const uint32_t per = (uint32_t)SCAN_TILE * SCAN_MAX_TILES;
@@ -305,6 +305,9 @@ __device__ inline void scan_out(const WorldDev &w, uint32_t i, uint32_t r, uint3
             w.bscene[r] = s;
             w.bvid[r] = key == OVERFLOW_KEY ? make_int4(OVERFLOW_VID, 0, 0, 0) : unpack_vid(key);
             w.bprim[r] = w.hmark[i];
+            // This substep's count starts at zero (k_block_count adds to it; the previous
+            // substep's stay readable until here; only the dense ids handed out are read).
+            w.bnp[r] = 0u;
         } else {
             w.hvals[i] = NONE;
             atomicOr(&w.ctr[s].err, ERR_GRID_CAPACITY);
@@ -339,11 +342,6 @@ __global__ void __launch_bounds__(256) k_scan(WorldDev w)
     const uint32_t chunks = scan_chunks(len);
     const uint32_t tile_len = SCAN_TILE * chunks;
     const uint32_t ntiles = min((len + tile_len - 1) / tile_len, w.scan_stride - 1u);
-    if (MODE == MODE_SLOTS) {
-        // The block counts of this substep start at zero (k_block_count adds to them; the
-        // previous substep's stay readable until here).
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.b_total; i += gridDim.x * blockDim.x) w.bnp[i] = 0u;
-    }
     unsigned long long *state = w.scan_state + (MODE == MODE_SLOTS ? 0u : w.scan_stride);
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(&w.g->ticket[MODE], 1u);
