@@ -267,9 +267,12 @@ def counters_from_profiles(workload: str) -> dict:
         return {}
     try:
         with open(path) as f:
-            return json.load(f).get("kernels", {})
+            d = json.load(f)
     except (OSError, ValueError):
         return {}
+    kernels = d.get("kernels", {})
+    source = f"profiles/counters_{workload}.json (recorded at commit {d.get('commit', '?')})"
+    return {k: dict(v, source=source) for k, v in kernels.items()}
 
 
 def measured_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> dict:
@@ -408,14 +411,15 @@ def main(argv=None):
                   file=sys.stderr)
         figures[kernel] = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": prof.get("traffic_bytes"),
+                           "traffic": prof.get("traffic_bytes"), "traffic_source": prof.get("source"),
                            "algorithmic_bytes": nbytes, "avg_launch_ms": kernel_ms, "stage_ms": stage_ms,
                            "launches_timed": nprof}
         if prof.get("valu_insts"):
             rate = prof["valu_insts"] / (kernel_ms * 1e-3) / 1e9
             valu[kernel] = {"kernel": kernel, "bound": "valu", "achieved": rate, "peak": VALU_PEAK_G,
                             "unit": "G VALU wave-instructions/s", "frac": rate / VALU_PEAK_G,
-                            "valu_insts_per_launch": prof["valu_insts"], "avg_launch_ms": kernel_ms}
+                            "valu_insts_per_launch": prof["valu_insts"], "avg_launch_ms": kernel_ms,
+                            "source": prof.get("source")}
     dominant = max(figures, key=lambda k: figures[k]["avg_launch_ms"])
     out = {
         "metric": METRIC,
