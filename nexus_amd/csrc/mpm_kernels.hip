@@ -56,6 +56,12 @@ __device__ inline uint32_t run_end(uint64_t heads, uint32_t h)
 // the same L2.  The grid must be a multiple of 8 workgroups.
 constexpr uint32_t NUM_XCD = 8;
 
+// Workgroup size of k_g2p_update (one particle per thread): two waves, measured fastest for
+// the plain kernel (256: sand column G2P 1 us, clumped fluid 5-7 % slower; 64 and 512 slower
+// too); CPIC worlds keep 256, which shares the GPU better with the contact G2P running
+// beside it (128: CPIC sand column 4 % slower).
+template <bool CPIC> constexpr int g2p_threads() { return CPIC ? 256 : 128; }
+
 __device__ inline void xcd_range(uint32_t n, uint32_t &begin, uint32_t &end)
 {
     const uint32_t g = gridDim.x, b = blockIdx.x;
@@ -1646,7 +1652,7 @@ __device__ inline void prepare_next_sort(const WorldDev &w)
 }
 
 // One thread per sorted particle, nodes read from the grid: the launch (g2p_groups) gives
-// every workgroup at most 256 positions, and a thread takes at most one, so no loop keeps
+// every workgroup at most g2p_threads positions, and a thread takes at most one, so no loop keeps
 // the arenas' 60-odd plane addresses live in scalar registers (they spilled to vector
 // lanes and were read back for every particle).  In CPIC worlds this kernel takes the
 // plain particles and k_g2p_contact the contact ones, so that the contact code does not
@@ -1656,7 +1662,7 @@ __device__ inline void prepare_next_sort(const WorldDev &w)
 // taken by whole waves (lanes past the range are invalid) from the position and flags the
 // thread has just written, still in its registers.
 template <bool CPIC, bool ADAPT>
-__global__ void __launch_bounds__(256) k_g2p_update(WorldDev w)
+__global__ void __launch_bounds__(g2p_threads<CPIC>()) k_g2p_update(WorldDev w)
 {
     uint32_t pbegin, pend;
     xcd_range(w.g->p_out, pbegin, pend);
@@ -2030,11 +2036,11 @@ static inline int grid_for(uint64_t n, int threads, int cap)
     return (int)g;
 }
 
-// Workgroups of k_g2p_update: enough that xcd_range gives none more than 256 of the at
-// most p_hint sorted positions (one per thread).
-static inline int g2p_groups(uint64_t p_hint)
+// Workgroups of k_g2p_update: enough that xcd_range gives none more than `threads` (its
+// workgroup size) of the at most p_hint sorted positions (one per thread).
+static inline int g2p_groups(uint64_t p_hint, int threads)
 {
-    return grid_for(p_hint, 256, std::numeric_limits<int>::max() / 2);
+    return grid_for(p_hint, threads, std::numeric_limits<int>::max() / 2);
 }
 
 void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, hipEvent_t *ev, bool keys)
@@ -2081,8 +2087,9 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         if (w.any_adaptive) k_g2p_contact<true><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
         else k_g2p_contact<false><<<grid_for(lc.p_hint, T, cap), T, 0, cs>>>(w);
         auto g2p = w.any_adaptive ? k_g2p_update<true, true> : k_g2p_update<true, false>;
-        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint)), dim3(T), 0, st, ev[7], ev[8], 0, w);
-        else g2p<<<g2p_groups(lc.p_hint), T, 0, st>>>(w);
+        constexpr int GT = g2p_threads<true>();
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint, GT)), dim3(GT), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<g2p_groups(lc.p_hint, GT), GT, 0, st>>>(w);
         if (cs != st) {
             (void)hipEventRecord(lc.join, cs);
             (void)hipStreamWaitEvent(st, lc.join, 0);
@@ -2096,8 +2103,9 @@ void launch_substep(const WorldDev &w, const LaunchConfig &lc, hipStream_t st, h
         else k_grid_update<false, false><<<grid_for((uint64_t)w.b_total * 64, T, cap), T, 0, st>>>(w);
         if (ev) (void)hipEventRecord(ev[3], st);
         auto g2p = w.any_adaptive ? k_g2p_update<false, true> : k_g2p_update<false, false>;
-        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint)), dim3(T), 0, st, ev[7], ev[8], 0, w);
-        else g2p<<<g2p_groups(lc.p_hint), T, 0, st>>>(w);
+        constexpr int GT = g2p_threads<false>();
+        if (ev) hipExtLaunchKernelGGL(g2p, dim3(g2p_groups(lc.p_hint, GT)), dim3(GT), 0, st, ev[7], ev[8], 0, w);
+        else g2p<<<g2p_groups(lc.p_hint, GT), GT, 0, st>>>(w);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
 }
