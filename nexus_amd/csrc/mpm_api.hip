@@ -1595,7 +1595,7 @@ class World {
         std::vector<nxm_launch_config> c;
         if (stage == 0) for (uint32_t k : {16u, 32u, 64u}) c.push_back({cu * k, b.p2g_block_cost, b.grid_groups});
         if (stage == 1) for (uint32_t k : {4u, 16u, 64u}) c.push_back({b.p2g_groups, k, b.grid_groups});
-        if (stage == 2) for (uint32_t k : {4u, 8u, 16u, 32u}) c.push_back({b.p2g_groups, b.p2g_block_cost, cu * k});
+        if (stage == 2) for (uint32_t k : {4u, 8u, 16u, 32u, 64u}) c.push_back({b.p2g_groups, b.p2g_block_cost, cu * k});
         return c;
     }
 
