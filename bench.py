@@ -133,9 +133,10 @@ def bench_cpic(pipe, steps: int, warmup: int = 40, autotune: bool = True) -> dic
         el = time.perf_counter() - t0
         stages, _ = st.profile_end()
         aff = st.read_cdf()["affinity"] if cpic else None
-        res[cpic] = (el, stages, st.sort_size(), aff)
+        cfg = dict(zip(("p2g_groups", "p2g_block_cost", "grid_groups"), st.launch_config()))
+        res[cpic] = (el, stages, st.sort_size(), aff, cfg)
         st.close()
-    el, stages, nb, aff = res[True]
+    el, stages, nb, aff, cfg = res[True]
     return {
         "workload": "Drucker-Prager sand column, 1,048,576 particles, 64x scale (h=0.25, dt=3.2e-3), 16 seeded "
                     "cuboid colliders, CPIC",
@@ -144,6 +145,7 @@ def bench_cpic(pipe, steps: int, warmup: int = 40, autotune: bool = True) -> dic
         "steps": steps,
         "active_blocks": nb,
         "particles_with_affinity": int((aff & 0xFFFF != 0).sum()),
+        "launch_config": cfg,
         "stages_ms": {"sort": stages.sort_ms, "cdf_and_p2g": stages.p2g_ms, "grid_update": stages.grid_update_ms,
                       "g2p_update": stages.g2p_update_ms},
         "same_scene_without_cpic": {"particle_substeps_per_s": sc.n * steps / res[False][0],
