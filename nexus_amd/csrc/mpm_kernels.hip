@@ -57,8 +57,8 @@ __device__ inline uint32_t run_end(uint64_t heads, uint32_t h)
 constexpr uint32_t NUM_XCD = 8;
 
 // Workgroup size of k_g2p_update (one particle per thread): two waves, measured fastest for
-// the plain kernel (256: sand column G2P 1 us, clumped fluid 5-7 % slower; 64 and 512 slower
-// too); CPIC worlds keep 256, which shares the GPU better with the contact G2P running
+// the plain kernel (256: sand column G2P 1 us, clumped fluid 5-7 % slower; 192: the fluid 6 %
+// slower; 64 and 512 slower too); CPIC worlds keep 256, which shares the GPU better with the contact G2P running
 // beside it (128: CPIC sand column 4 % slower).
 template <bool CPIC> constexpr int g2p_threads() { return CPIC ? 256 : 128; }
 
