@@ -1587,13 +1587,13 @@ class World {
         const uint32_t cu = (uint32_t)ctx_->num_cus;
         return nxm_launch_config{cu * 32u, 16u, cu * 8u};
     }
-    uint32_t max_p2g_groups() const { return (uint32_t)ctx_->num_cus * 256u; }
+    uint32_t max_p2g_groups() const { return (uint32_t)ctx_->num_cus * 512u; }
 
     std::vector<nxm_launch_config> stage_candidates(int stage, const nxm_launch_config &b) const
     {
         const uint32_t cu = (uint32_t)ctx_->num_cus;
         std::vector<nxm_launch_config> c;
-        if (stage == 0) for (uint32_t k : {64u, 128u, 256u}) c.push_back({cu * k, b.p2g_block_cost, b.grid_groups});
+        if (stage == 0) for (uint32_t k : {128u, 256u, 512u}) c.push_back({cu * k, b.p2g_block_cost, b.grid_groups});
         if (stage == 1) for (uint32_t k : {4u, 16u, 64u}) c.push_back({b.p2g_groups, k, b.grid_groups});
         if (stage == 2) for (uint32_t k : {4u, 8u, 16u, 32u, 64u}) c.push_back({b.p2g_groups, b.p2g_block_cost, cu * k});
         return c;
@@ -1683,7 +1683,7 @@ class World {
     nxm_status set_launch_config(const nxm_launch_config &c)
     {
         if (c.p2g_groups == 0 || c.p2g_groups > max_p2g_groups() || c.p2g_groups % 8 != 0)
-            return fail(NXM_ERR_INVALID_ARGUMENT, "p2g_groups must be a positive multiple of 8 up to 256 per CU");
+            return fail(NXM_ERR_INVALID_ARGUMENT, "p2g_groups must be a positive multiple of 8 up to 512 per CU");
         if (c.grid_groups == 0) return fail(NXM_ERR_INVALID_ARGUMENT, "grid_groups must be positive");
         tune_on_ = false;
         tuning_ = false;

@@ -114,17 +114,17 @@ def test_failed_append_keeps_particle_count(pipeline):
 
 
 def test_launch_config_limits(pipeline):
-    """The largest P2G workgroup count the tuner tries (256 per CU) is accepted, one step
+    """The largest P2G workgroup count the tuner tries (512 per CU) is accepted, one step
     above it is refused on the host, and the batch steps on with the accepted shape."""
     st = mpm.MpmState.from_scene(pipeline, scenes.elastic_cube(5, side=0.08, center=(0.5, 0.5, 0.5)))
     batch = mpm.Batch(pipeline)
     batch.add(st)
     cu = batch.launch_config()[0] // 32            # the default is 32 per CU
-    batch.set_launch_config(cu * 256, 4, cu * 64)
+    batch.set_launch_config(cu * 512, 4, cu * 64)
     with pytest.raises(mpm.MpmError) as e:
-        batch.set_launch_config(cu * 256 + 8, 4, cu * 64)
+        batch.set_launch_config(cu * 512 + 8, 4, cu * 64)
     assert e.value.code == 1
-    assert tuple(batch.launch_config()) == (cu * 256, 4, cu * 64)
+    assert tuple(batch.launch_config()) == (cu * 512, 4, cu * 64)
     batch.step()
     batch.synchronize()
     assert np.all(np.isfinite(st.read_particles()["x"]))

@@ -282,14 +282,14 @@ def test_launch_shapes_do_not_change_results(pipeline):
         if mode == "small":
             batch.set_launch_config(cu * 16, 4, cu * 4)
         elif mode == "large":
-            batch.set_launch_config(cu * 256, 64, cu * 64)   # the largest shapes the tuner tries
+            batch.set_launch_config(cu * 512, 64, cu * 64)   # the largest shapes the tuner tries
         elif mode == "autotune":
             batch.set_autotune(True)
         batch.run(32)
         results.append([s.read_particles() for s in states])
         if mode == "autotune":
             g, k, gg = batch.launch_config()
-            assert g in (cu * 32, cu * 64, cu * 128, cu * 256) and k in (4, 16, 64)
+            assert g in (cu * 32, cu * 128, cu * 256, cu * 512) and k in (4, 16, 64)
             assert gg in (cu * 4, cu * 8, cu * 16, cu * 32, cu * 64)
         batch.close()
         for s in states:
