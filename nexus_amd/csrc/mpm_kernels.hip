@@ -1893,8 +1893,10 @@ __device__ __forceinline__ uint32_t particle_cdf_one(const WorldDev &w, uint32_t
             // Sides: kept for colliders known before, from the weighted signed
             // distances for newly reached ones.
             pa |= prev & ((prev & 0xFFFFu) << 16);
-            for (uint32_t c = 0; c < sc.col_count; ++c) {
-                if ((prev >> c) & 1u) continue;
+            // Only colliders some stencil node is affine to can give a side (for the others
+            // every term is zero), and only those not known before need one.
+            for (uint32_t todo = pa & ~prev & 0xFFFFu; todo != 0u; todo &= todo - 1u) {
+                const uint32_t c = (uint32_t)__ffs(todo) - 1u;
                 float acc = 0.0f;
                 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
