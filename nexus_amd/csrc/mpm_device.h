@@ -768,8 +768,8 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
     float best = MAX_FLT_REF;
     f3 normal = mk3(0, 0, 0);
     uint32_t best_id = 0;
-    for (uint32_t i = 0; i < ncol; ++i) {
-        if (!(mask & (1u << i))) continue;
+    for (uint32_t m = ncol >= 32u ? mask : (mask & ((1u << ncol) - 1u)); m != 0u; m &= m - 1u) {
+        const uint32_t i = (uint32_t)__ffs(m) - 1u;   // the masked colliders, ascending
         bool inside;
         f3 proj;
         if (cols[i].shape == SHAPE_TRIMESH) {
@@ -822,8 +822,9 @@ __device__ inline void node_cdf(const ColliderDev *cols, uint32_t ncol, uint32_t
     dist = MAX_FLT_REF;
     aff = 0u;
     closest = NONE;
-    for (uint32_t i = 0; i < ncol; ++i) {
-        if (!(mask & (1u << i)) || cols[i].shape == SHAPE_TRIMESH) continue;   // meshes: k_mesh_cdf
+    for (uint32_t m = ncol >= 32u ? mask : (mask & ((1u << ncol) - 1u)); m != 0u; m &= m - 1u) {
+        const uint32_t i = (uint32_t)__ffs(m) - 1u;   // the masked colliders, ascending
+        if (cols[i].shape == SHAPE_TRIMESH) continue;   // meshes: k_mesh_cdf
         bool inside;
         const f3 proj = project_shape(cols[i], pt, inside);
         const f3 dpt = proj - pt;

@@ -1362,8 +1362,8 @@ __device__ __forceinline__ void g2p_particle(const WorldDev &w, uint32_t pos, ui
                 div = gradv.c0.x + gradv.c1.y + gradv.c2.z;
                 if (cp) {
                     // g2p.rs:248-258: the bodies' velocities at the particle over its affinities
-                    for (uint32_t c = 0; c < sc.col_count; ++c)
-                        if ((pa >> c) & 1u) rigid_vel = rigid_vel + body_point_velocity(w.cols[sc.col_start + c], mk3(px, py, pz));
+                    for (uint32_t m = pa & ((1u << sc.col_count) - 1u); m != 0u; m &= m - 1u)   // col_count <= 16
+                        rigid_vel = rigid_vel + body_point_velocity(w.cols[sc.col_start + (uint32_t)__ffs(m) - 1u], mk3(px, py, pz));
                 }
             }
             // Particle update.  CPIC: a particle deeper than 0.05 h behind a surface is
