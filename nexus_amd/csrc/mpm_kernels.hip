@@ -1122,20 +1122,37 @@ __device__ inline void clear_hash(const WorldDev &w)
 // fields and skip the collider pass (solver/grid_update.rs:26-50).  MESH: the world has
 // mesh colliders in scenes without CPIC, whose projection the collider pass then runs
 // (its registers kept out of the common variant).
+// A wave-uniform read of memory that no kernel running at the time writes, through the
+// constant address space: the compiler then issues it as a scalar load.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned dwords");
+    using cword = const __attribute__((address_space(4))) uint32_t;
+    T r;
+    uint32_t *o = reinterpret_cast<uint32_t *>(&r);
+    #pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = ((cword *)p)[k];
+    return r;
+}
+
 template <bool CPIC, bool MESH>
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
     clear_hash(w);   // not read after the block tables (k_block_count, k_mesh_cdf) are built
-    uint32_t nbegin, nend;
-    xcd_range(w.g->n_blocks * NODES_PER_BLOCK, nbegin, nend);
-    for (uint32_t n = nbegin + threadIdx.x; n < nend; n += blockDim.x) {
-        const uint32_t d = n >> 6;
-        const uint32_t s = w.bscene[d];
-        const int4 v = w.bvid[d];
+    // Whole blocks per workgroup: a wave's 64 consecutive nodes are then one block, and the
+    // per-block reads (scene, block id, tile sources, collider mask) are wave-uniform.
+    uint32_t bbegin, bend;
+    xcd_range(w.g->n_blocks, bbegin, bend);
+    const uint32_t nend = bend * NODES_PER_BLOCK;
+    for (uint32_t n = bbegin * NODES_PER_BLOCK + threadIdx.x; n < nend; n += blockDim.x) {
+        const uint32_t d = __builtin_amdgcn_readfirstlane(n >> 6);
+        const uint32_t s = ld_uniform(w.bscene + d);
+        const int4 v = ld_uniform(w.bvid + d);
         if (v.x == OVERFLOW_VID) continue;
         const uint32_t li = n & 63u;
         const int i = li & 3, j = (li >> 2) & 3, k = li >> 4;
-        const SceneDev sc = w.scenes[s];
+        const SceneDev sc = ld_uniform(w.scenes + s);
         const bool cp = CPIC && sc.cpic;
         float mx = 0.0f, my = 0.0f, mz = 0.0f, m = 0.0f;
         float ix = 0.0f, iy = 0.0f, iz = 0.0f, im = 0.0f;
@@ -1143,7 +1160,7 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
         for (int e = 0; e < 8; ++e) {
             const int ex = ASSOC[e][0], ey = ASSOC[e][1], ez = ASSOC[e][2];
             if ((ex && i > 1) || (ey && j > 1) || (ez && k > 1)) continue;
-            const uint32_t src = w.bdn[8 * d + e];
+            const uint32_t src = ld_uniform(w.bdn + 8 * d + e);
             if (src == NONE) continue;
             const int nx = ex ? 2 : 4, ny = ey ? 2 : 4;
             const size_t at = (size_t)src * TILE_NODES + REGION_OFFSET[e] + i + nx * (j + ny * k);
@@ -1152,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             my += q.y;
             mz += q.z;
             m += q.w;
-            if (cp && w.binc[src]) {
+            if (cp && ld_uniform(w.binc + src)) {
                 const float4 qi = reinterpret_cast<const float4 *>(w.stage_i)[at];
                 ix += qi.x;
                 iy += qi.y;
@@ -1181,7 +1198,7 @@ __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
             vi.z = fminf(fmaxf(vi.z, -lim), lim);
             w.gridv_i[n] = make_float4(vi.x, vi.y, vi.z, im);
         }
-        const uint32_t mask = cp ? 0u : w.bmask[d];
+        const uint32_t mask = cp ? 0u : ld_uniform(w.bmask + d);
         if (mask) {
             const f3 pt = mk3((float)(v.x * 4 + i), (float)(v.y * 4 + j), (float)(v.z * 4 + k)) * sc.h;
             const MeshSetDev ms{w.mesh_v, w.mesh_t, w.mesh_vn, w.mesh_en, w.mesh_grid, w.mesh_cell, w.mesh_ctri, w.col_mesh};
