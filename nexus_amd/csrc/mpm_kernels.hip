@@ -62,6 +62,29 @@ constexpr uint32_t NUM_XCD = 8;
 // beside it (128: CPIC sand column 4 % slower).
 template <bool CPIC> constexpr int g2p_threads() { return CPIC ? 256 : 128; }
 
+// A wave-uniform read of memory that no kernel running at the time writes, through the
+// constant address space: the compiler then issues it as a scalar load.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned dwords");
+    using cword = const __attribute__((address_space(4))) uint32_t;
+    T r;
+    uint32_t *o = reinterpret_cast<uint32_t *>(&r);
+    #pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = ((cword *)p)[k];
+    return r;
+}
+
+// ld_uniform where SCALAR holds (a kernel variant with scalar registers to spare), else a
+// plain load.
+template <bool SCALAR, class T>
+__device__ __forceinline__ T ld_maybe_uniform(const T *p)
+{
+    if constexpr (SCALAR) return ld_uniform(p);
+    else return *p;
+}
+
 __device__ inline void xcd_range(uint32_t n, uint32_t &begin, uint32_t &end)
 {
     const uint32_t g = gridDim.x, b = blockIdx.x;
@@ -1015,15 +1038,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     // Cost-balanced block range, XCD-contiguous (see p2g_split and xcd_range).
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const uint32_t g = (G % NUM_XCD == 0) ? (b % NUM_XCD) * (G / NUM_XCD) + b / NUM_XCD : b;
-    const uint32_t dbegin = w.p2g_start[g], dend = w.p2g_start[g + 1];
+    // Per-block words by scalar loads in the plain variant (the CPIC one spills more with them).
+    const uint32_t dbegin = ld_maybe_uniform<!CPIC>(w.p2g_start + g), dend = ld_maybe_uniform<!CPIC>(w.p2g_start + g + 1);
     for (uint32_t d = dbegin; d < dend; ++d) {
-        const uint32_t s = w.bscene[d];
-        const uint32_t np = w.bnp[d];
-        const int4 v = w.bvid[d];
+        const uint32_t s = ld_maybe_uniform<!CPIC>(w.bscene + d);
+        const uint32_t np = ld_maybe_uniform<!CPIC>(w.bnp + d);
+        const int4 v = ld_maybe_uniform<!CPIC>(w.bvid + d);
         if (np == 0u || v.x == OVERFLOW_VID) continue;
         const SceneDev sc = w.scenes[s];
         const float h = sc.h;
-        const uint32_t first = w.bfirst[d];
+        const uint32_t first = ld_maybe_uniform<!CPIC>(w.bfirst + d);
         float bmom = 0.0f, bm = 0.0f;
         if (np <= (uint32_t)WAVE && np > 21u) {
             // One particle per lane: load it once, bound, then scatter from registers.
@@ -1122,20 +1146,6 @@ __device__ inline void clear_hash(const WorldDev &w)
 // fields and skip the collider pass (solver/grid_update.rs:26-50).  MESH: the world has
 // mesh colliders in scenes without CPIC, whose projection the collider pass then runs
 // (its registers kept out of the common variant).
-// A wave-uniform read of memory that no kernel running at the time writes, through the
-// constant address space: the compiler then issues it as a scalar load.
-template <class T>
-__device__ __forceinline__ T ld_uniform(const T *p)
-{
-    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned dwords");
-    using cword = const __attribute__((address_space(4))) uint32_t;
-    T r;
-    uint32_t *o = reinterpret_cast<uint32_t *>(&r);
-    #pragma unroll
-    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = ((cword *)p)[k];
-    return r;
-}
-
 template <bool CPIC, bool MESH>
 __global__ void __launch_bounds__(256) k_grid_update(WorldDev w)
 {
