@@ -119,6 +119,29 @@ constexpr uint32_t SHAPE_CAPSULE = 2u;
 constexpr uint32_t SHAPE_CONE = 3u;
 constexpr uint32_t SHAPE_CYLINDER = 4u;
 constexpr uint32_t SHAPE_TRIMESH = 6u;
+// A wave-uniform read of memory that no kernel running at the time writes, through the
+// constant address space: the compiler then issues it as a scalar load.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned dwords");
+    using cword = const __attribute__((address_space(4))) uint32_t;
+    T r;
+    uint32_t *o = reinterpret_cast<uint32_t *>(&r);
+    #pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = ((cword *)p)[k];
+    return r;
+}
+
+// ld_uniform where SCALAR holds (a kernel variant with scalar registers to spare), else a
+// plain load.
+template <bool SCALAR, class T>
+__device__ __forceinline__ T ld_maybe_uniform(const T *p)
+{
+    if constexpr (SCALAR) return ld_uniform(p);
+    else return *p;
+}
+
 struct ColliderDev {
     float he[3];
     float q[4];
@@ -759,7 +782,8 @@ __device__ inline f3 bc_project(uint32_t ty, float friction, f3 vel, f3 n)
 // mask: colliders that can reach the node's block (bit i for collider i); the others
 // can be neither within 1.5 h of the node nor contain it, so skipping them leaves the
 // reference's result unchanged.  MESH: mesh colliders (their mesh ms.col_mesh[col_base +
-// i]) report only triangles within 1.5 h; without it the caller's scenes have none.
+// i]) report only triangles within 1.5 h; without it the caller's scenes have none.  The
+// colliders are read with ld_uniform: no kernel running beside the caller writes them.
 template <bool MESH>
 __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32_t mask, float h, float dt, f3 pt, f3 vel,
                                   const MeshSetDev &ms, uint32_t col_base)
@@ -772,10 +796,11 @@ __device__ inline f3 collide_node(const ColliderDev *cols, uint32_t ncol, uint32
         const uint32_t i = (uint32_t)__ffs(m) - 1u;   // the masked colliders, ascending
         bool inside;
         f3 proj;
-        if (cols[i].shape == SHAPE_TRIMESH) {
-            if (!MESH || !project_trimesh(ms, ms.col_mesh[col_base + i], cols[i], cap, pt, proj, inside)) continue;
+        const ColliderDev c = ld_uniform(cols + i);   // mask, and so i, is uniform over the wave
+        if (c.shape == SHAPE_TRIMESH) {
+            if (!MESH || !project_trimesh(ms, ms.col_mesh[col_base + i], c, cap, pt, proj, inside)) continue;
         } else {
-            proj = project_shape(cols[i], pt, inside);
+            proj = project_shape(c, pt, inside);
         }
         f3 dpt = proj - pt;
         bool within = fabsf(dpt.x) <= cap && fabsf(dpt.y) <= cap && fabsf(dpt.z) <= cap;

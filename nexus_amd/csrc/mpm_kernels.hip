@@ -62,29 +62,6 @@ constexpr uint32_t NUM_XCD = 8;
 // beside it (128: CPIC sand column 4 % slower).
 template <bool CPIC> constexpr int g2p_threads() { return CPIC ? 256 : 128; }
 
-// A wave-uniform read of memory that no kernel running at the time writes, through the
-// constant address space: the compiler then issues it as a scalar load.
-template <class T>
-__device__ __forceinline__ T ld_uniform(const T *p)
-{
-    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "whole aligned dwords");
-    using cword = const __attribute__((address_space(4))) uint32_t;
-    T r;
-    uint32_t *o = reinterpret_cast<uint32_t *>(&r);
-    #pragma unroll
-    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = ((cword *)p)[k];
-    return r;
-}
-
-// ld_uniform where SCALAR holds (a kernel variant with scalar registers to spare), else a
-// plain load.
-template <bool SCALAR, class T>
-__device__ __forceinline__ T ld_maybe_uniform(const T *p)
-{
-    if constexpr (SCALAR) return ld_uniform(p);
-    else return *p;
-}
-
 __device__ inline void xcd_range(uint32_t n, uint32_t &begin, uint32_t &end)
 {
     const uint32_t g = gridDim.x, b = blockIdx.x;
